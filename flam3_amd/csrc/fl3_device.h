@@ -1,0 +1,74 @@
+/*
+ * fl3_device.h - internal C interface between the host render driver
+ * (fl3_render.c) and the HIP kernels (fl3_device.hip).  Device buffers live
+ * in a per-device context owned by fl3_device.hip and are reused across
+ * renders; every call returns 0 on success and sets fl3d_error() otherwise.
+ */
+#ifndef FL3_DEVICE_H
+#define FL3_DEVICE_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "fl3_host.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char *fl3d_error(void);
+
+/* Histogram + accumulator for one render, cleared. */
+int fl3d_begin(int width, int height, int bits);
+/* Zero the histogram (start of each batch, rect.c:721). */
+int fl3d_clear_buckets(void);
+
+typedef struct {
+   int fuse;            /* 15, or 100 with earlyclip (rect.c:36-37, 266) */
+   int64_t nsamples;    /* plotted samples in this launch */
+   uint64_t key[2];     /* walker stream key drawn from the frame's ISAAC */
+   int exact;           /* saturating adds on every channel */
+   int nwalkers;        /* 0: chosen by the library */
+} fl3d_iter_params;
+
+/* Iterate one job (one genome at one temporal sample) and deposit into the
+ * histogram.  *saturated is set when a hit counter reached UINT_MAX in the
+ * fast mode (the caller then repeats the batch with exact = 1). */
+int fl3d_iterate(const void *job, size_t job_bytes, const fl3d_iter_params *p,
+                 double *badvals, int *saturated);
+
+typedef struct {
+   int oversample;
+   double k1, k2, curve;
+} fl3d_de_params;
+
+/* Density estimation into the accumulator (rect.c:57-251). */
+int fl3d_density(const fl3d_de_params *p, const fl3_de_filters *de);
+/* Log scaling without DE (rect.c:947-974). */
+int fl3d_logscale(double k1, double k2);
+
+typedef struct {
+   int image_width, image_height;  /* output pixels (height halved for fields) */
+   int oversample, de_offset, filter_width;
+   int nchan, transp, bytes_per_channel, earlyclip;
+   double g, linrange, vibrancy, highpow;
+   double background[3];
+} fl3d_tm_params;
+
+/* Spatial filter + tone map + quantise (rect.c:1067-1247), written to a
+ * host buffer with a row stride of out_stride bytes. */
+int fl3d_tonemap(const fl3d_tm_params *p, const double *filter, void *out_host, size_t out_stride);
+
+/* Replay iteration: nwalkers FP64 walkers, walker w being reference thread w
+ * with ISAAC state isaac[36w..36w+35] (count, 16 results, 16 memory words,
+ * a, b, c; updated in place), batch_size samples each in sub-batches of sbs. */
+int fl3d_replay(const void *job, size_t job_bytes, uint32_t *isaac, int nwalkers,
+                double batch_size, int sbs, int fuse, double *badvals);
+
+/* Device-resident copies for tests. */
+int fl3d_download_buckets(void *host, size_t bytes);
+int fl3d_download_accum(void *host, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

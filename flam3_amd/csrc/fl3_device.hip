@@ -1,0 +1,553 @@
+// fl3_device.hip - host side of the MI355X (gfx950) render path: the
+// per-device context (buffers reused across renders, one stream), kernel
+// launches, and the C interfaces over them.  The kernels themselves live in
+// fl3_kernels_fast.hip and fl3_kernels_exact.hip:
+//
+//  * walker kernels (chaos game + histogram deposit, rect.c:253-527 with
+//    flam3.c:232-299 and variations.c:2129-2399 inlined): one thread is one
+//    walker; the genome lives in LDS in the iteration precision.
+//      - k_iter_fast: FP32 walkers with their own counter streams (keyed from
+//        the frame's ISAAC stream), for production renders;
+//      - k_iter_replay: FP64 walkers that each run exactly one reference
+//        thread's ISAAC stream and sub-batch loop (replay / verification);
+//      - k_iterate_points: FP64 batched flam3_iterate.
+//  * k_deposit_points: deposit a given point stream (stage test).
+//  * k_de / k_logscale: density estimation or plain log scaling into the
+//    accumulator (rect.c:57-251, 947-974).
+//  * k_earlyclip + k_tonemap: spatial filter, log/gamma/vibrancy/highlight
+//    tone map and quantisation fused into one pass (rect.c:1067-1247).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <mutex>
+#include <type_traits>
+
+#include "fl3_pack.h"
+#include "fl3_core.h"
+#include "fl3_launch.h"
+#include "fl3_tonemath.h"
+#include "fl3_device.h"
+#include "fl3h.h"
+
+// ------------------------------------------------------------------ errors
+
+static thread_local char g_err[512];
+
+static void set_err(const char *fmt, const char *a = "", long b = 0) {
+   snprintf(g_err, sizeof(g_err), fmt, a, b);
+}
+
+#define HIPCHK(call)                                                          \
+   do {                                                                       \
+      hipError_t e_ = (call);                                                 \
+      if (e_ != hipSuccess) {                                                 \
+         snprintf(g_err, sizeof(g_err), "%s failed: %s (line %d)", #call,    \
+                  hipGetErrorString(e_), __LINE__);                           \
+         return 1;                                                            \
+      }                                                                       \
+   } while (0)
+
+extern "C" const char *fl3d_error(void) { return g_err; }
+extern "C" const char *fl3h_last_error(void) { return g_err; }
+
+// ------------------------------------------------------------------ device context
+
+struct DevBuf {
+   void *p = nullptr;
+   size_t cap = 0;
+   int ensure(size_t bytes) {
+      if (bytes <= cap) return 0;
+      if (p) (void)hipFree(p);
+      p = nullptr;
+      cap = 0;
+      size_t want = bytes + bytes / 8;
+      if (hipMalloc(&p, want) != hipSuccess) {
+         snprintf(g_err, sizeof(g_err), "hipMalloc of %zu bytes failed", want);
+         p = nullptr;
+         return 1;
+      }
+      cap = want;
+      return 0;
+   }
+};
+
+struct Ctx {
+   int ready = 0;
+   int device = 0;
+   int ncu = 256;
+   hipStream_t stream = nullptr;
+   DevBuf buckets, accum, job, table, image, isaac, points, misc;
+   unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
+   int width = 0, height = 0, bits = 33;
+};
+
+static Ctx g_ctx;
+static std::recursive_mutex g_mu;
+
+static int ctx_init() {
+   if (g_ctx.ready) return 0;
+   HIPCHK(hipGetDevice(&g_ctx.device));
+   hipDeviceProp_t prop;
+   HIPCHK(hipGetDeviceProperties(&prop, g_ctx.device));
+   g_ctx.ncu = prop.multiProcessorCount;
+   HIPCHK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+   HIPCHK(hipMalloc((void **)&g_ctx.counters, 64));
+   g_ctx.ready = 1;
+   return 0;
+}
+
+static size_t bucket_bytes(int bits) { return bits == 64 ? 40 : 20; }
+static size_t accum_bytes(int bits) { return bits == 64 ? 32 : 16; }
+
+extern "C" int fl3d_begin(int width, int height, int bits) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   if (ctx_init()) return 1;
+   size_t n = (size_t)width * height;
+   if (g_ctx.buckets.ensure(n * bucket_bytes(bits))) return 1;
+   if (g_ctx.accum.ensure(n * accum_bytes(bits))) return 1;
+   g_ctx.width = width;
+   g_ctx.height = height;
+   g_ctx.bits = bits;
+   HIPCHK(hipMemsetAsync(g_ctx.accum.p, 0, n * accum_bytes(bits), g_ctx.stream));
+   HIPCHK(hipMemsetAsync(g_ctx.buckets.p, 0, n * bucket_bytes(bits), g_ctx.stream));
+   return 0;
+}
+
+extern "C" int fl3d_clear_buckets(void) {
+   size_t n = (size_t)g_ctx.width * g_ctx.height;
+   HIPCHK(hipMemsetAsync(g_ctx.buckets.p, 0, n * bucket_bytes(g_ctx.bits), g_ctx.stream));
+   return 0;
+}
+
+static int upload_job(const void *job, size_t bytes) {
+   if (g_ctx.job.ensure(bytes)) return 1;
+   HIPCHK(hipMemcpyAsync(g_ctx.job.p, job, bytes, hipMemcpyHostToDevice, g_ctx.stream));
+   return 0;
+}
+
+static int read_counters(double *badvals, int *saturated) {
+   unsigned long long c[2];
+   HIPCHK(hipMemcpyAsync(c, g_ctx.counters, sizeof(c), hipMemcpyDeviceToHost, g_ctx.stream));
+   HIPCHK(hipStreamSynchronize(g_ctx.stream));
+   if (badvals) *badvals = (double)c[0];
+   if (saturated) *saturated = c[1] != 0;
+   return 0;
+}
+
+// Walkers per launch: enough resident waves to hide the atomic latency,
+// never fewer than ~64 samples per walker.
+static long long choose_walkers(long long nsamples, int requested) {
+   if (requested > 0) return requested;
+   long long maxw = (long long)g_ctx.ncu * 8 * 256;
+   long long w = nsamples / 64;
+   if (w > maxw) w = maxw;
+   if (w < 256) w = 256;
+   return (w + 255) / 256 * 256;
+}
+
+extern "C" int fl3d_iterate(const void *job, size_t job_bytes, const fl3d_iter_params *p,
+                            double *badvals, int *saturated) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   const fl3_job_hdr *h = (const fl3_job_hdr *)job;
+   if (upload_job(job, job_bytes)) return 1;
+   HIPCHK(hipMemsetAsync(g_ctx.counters, 0, 16, g_ctx.stream));
+   if (p->nsamples > 0) {
+      IterArgs a;
+      memset(&a, 0, sizeof(a));
+      a.job = (const unsigned char *)g_ctx.job.p;
+      a.buckets = g_ctx.buckets.p;
+      a.nsamples = p->nsamples;
+      a.fuse = p->fuse;
+      a.key0 = p->key[0];
+      a.key1 = p->key[1];
+      a.badvals = g_ctx.counters;
+      a.sat = (uint32_t *)(g_ctx.counters + 1);
+      const long long W = choose_walkers(p->nsamples, p->nwalkers);
+      const dim3 grid((unsigned)(W / 256)), block(256);
+      const size_t lds = lds_bytes_for<float>(h);
+      if (fl3k_launch_iter_fast(&a, grid.x, lds, g_ctx.stream, g_ctx.bits, p->exact)) {
+         set_err("k_iter_fast launch failed");
+         return 1;
+      }
+   }
+   return read_counters(badvals, saturated);
+}
+
+// Replay launch: isaac holds 36 words per walker (host-seeded); updated.
+static int iterate_replay(const void *job, size_t job_bytes, uint32_t *isaac_host, int nwalkers,
+                          double batch_size, int sbs, int fuse, double *badvals) {
+   const fl3_job_hdr *h = (const fl3_job_hdr *)job;
+   if (upload_job(job, job_bytes)) return 1;
+   size_t ib = sizeof(uint32_t) * 36 * (size_t)nwalkers;
+   if (g_ctx.isaac.ensure(ib)) return 1;
+   HIPCHK(hipMemcpyAsync(g_ctx.isaac.p, isaac_host, ib, hipMemcpyHostToDevice, g_ctx.stream));
+   HIPCHK(hipMemsetAsync(g_ctx.counters, 0, 16, g_ctx.stream));
+   IterArgs a;
+   memset(&a, 0, sizeof(a));
+   a.job = (const unsigned char *)g_ctx.job.p;
+   a.buckets = g_ctx.buckets.p;
+   a.fuse = fuse;
+   a.badvals = g_ctx.counters;
+   a.sat = (uint32_t *)(g_ctx.counters + 1);
+   a.isaac = (uint32_t *)g_ctx.isaac.p;
+   a.batch_size = batch_size;
+   a.sbs = sbs;
+   a.nwalkers = nwalkers;
+   const dim3 grid((nwalkers + 63) / 64), block(64);
+   const size_t lds = lds_bytes_for<double>(h);
+   if (fl3k_launch_replay(&a, grid.x, lds, g_ctx.stream, g_ctx.bits)) {
+      set_err("k_iter_replay launch failed");
+      return 1;
+   }
+   HIPCHK(hipMemcpyAsync(isaac_host, g_ctx.isaac.p, ib, hipMemcpyDeviceToHost, g_ctx.stream));
+   return read_counters(badvals, nullptr);
+}
+
+static int upload_de(const fl3_de_filters *de, double **coefs, double **widths) {
+   size_t nc = (size_t)de->nfilters * de->kernel_size, nw = de->nfilters;
+   if (g_ctx.table.ensure(sizeof(double) * (nc + nw))) return 1;
+   *coefs = (double *)g_ctx.table.p;
+   *widths = *coefs + nc;
+   HIPCHK(hipMemcpyAsync(*coefs, de->filter_coefs, sizeof(double) * nc, hipMemcpyHostToDevice, g_ctx.stream));
+   HIPCHK(hipMemcpyAsync(*widths, de->filter_widths, sizeof(double) * nw, hipMemcpyHostToDevice, g_ctx.stream));
+   return 0;
+}
+
+extern "C" int fl3d_density(const fl3d_de_params *p, const fl3_de_filters *de) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   DEArgs a;
+   memset(&a, 0, sizeof(a));
+   double *coefs, *widths;
+   if (upload_de(de, &coefs, &widths)) return 1;
+   a.buckets = g_ctx.buckets.p;
+   a.accum = g_ctx.accum.p;
+   a.coefs = coefs;
+   a.widths = widths;
+   a.width = g_ctx.width;
+   a.height = g_ctx.height;
+   a.oversample = p->oversample;
+   a.max_filtered_counts = de->max_filtered_counts;
+   a.max_filter_index = de->max_filter_index;
+   a.kernel_size = de->kernel_size;
+   a.k1 = p->k1;
+   a.k2 = p->k2;
+   a.curve = p->curve;
+   const int os = p->oversample;
+   const int ni = g_ctx.width - 2 * (os - 1);
+   const int nj = g_ctx.height - 2 * (os - 1) + 1;
+   if (ni <= 0 || nj <= 0) return 0;
+   if (fl3k_launch_de(&a, (ni + 255) / 256, nj, g_ctx.stream, g_ctx.bits)) {
+      set_err("k_de launch failed");
+      return 1;
+   }
+   return 0;
+}
+
+extern "C" int fl3d_logscale(double k1, double k2) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   const long n = (long)g_ctx.width * g_ctx.height;
+   if (fl3k_launch_logscale(g_ctx.buckets.p, g_ctx.accum.p, n, k1, k2, g_ctx.stream, g_ctx.bits)) {
+      set_err("k_logscale launch failed");
+      return 1;
+   }
+   return 0;
+}
+
+extern "C" int fl3d_tonemap(const fl3d_tm_params *p, const double *filter, void *out_host, size_t out_stride) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   const size_t row = (size_t)p->image_width * p->nchan * p->bytes_per_channel;
+   const size_t img = row * p->image_height;
+   const size_t fb = sizeof(double) * p->filter_width * p->filter_width;
+   if (g_ctx.image.ensure(img > 0 ? img : 1)) return 1;
+   if (g_ctx.misc.ensure(fb)) return 1;
+   HIPCHK(hipMemcpyAsync(g_ctx.misc.p, filter, fb, hipMemcpyHostToDevice, g_ctx.stream));
+   TMArgs a;
+   memset(&a, 0, sizeof(a));
+   a.accum = g_ctx.accum.p;
+   a.filter = (const double *)g_ctx.misc.p;
+   a.out = (unsigned char *)g_ctx.image.p;
+   a.W = g_ctx.width;
+   a.iw = p->image_width;
+   a.ih = p->image_height;
+   a.os = p->oversample;
+   a.de_offset = p->de_offset;
+   a.fw = p->filter_width;
+   a.nchan = p->nchan;
+   a.transp = p->transp;
+   a.bpc = p->bytes_per_channel;
+   a.earlyclip = p->earlyclip;
+   a.g = p->g;
+   a.linrange = p->linrange;
+   a.vibrancy = p->vibrancy;
+   a.highpow = p->highpow;
+   for (int k = 0; k < 3; k++) a.bg[k] = p->background[k];
+   const long n = (long)g_ctx.width * g_ctx.height;
+   if (p->earlyclip) {
+      if (fl3k_launch_earlyclip(g_ctx.accum.p, n, &a, g_ctx.stream, g_ctx.bits)) {
+         set_err("k_earlyclip launch failed");
+         return 1;
+      }
+   }
+   if (p->image_width > 0 && p->image_height > 0) {
+      if (fl3k_launch_tonemap(&a, g_ctx.stream, g_ctx.bits)) {
+         set_err("k_tonemap launch failed");
+         return 1;
+      }
+      HIPCHK(hipMemcpy2DAsync(out_host, out_stride, g_ctx.image.p, row, row, p->image_height,
+                              hipMemcpyDeviceToHost, g_ctx.stream));
+   }
+   HIPCHK(hipStreamSynchronize(g_ctx.stream));
+   return 0;
+}
+
+extern "C" int fl3d_download_buckets(void *host, size_t bytes) {
+   HIPCHK(hipMemcpyAsync(host, g_ctx.buckets.p, bytes, hipMemcpyDeviceToHost, g_ctx.stream));
+   HIPCHK(hipStreamSynchronize(g_ctx.stream));
+   return 0;
+}
+
+extern "C" int fl3d_download_accum(void *host, size_t bytes) {
+   HIPCHK(hipMemcpyAsync(host, g_ctx.accum.p, bytes, hipMemcpyDeviceToHost, g_ctx.stream));
+   HIPCHK(hipStreamSynchronize(g_ctx.stream));
+   return 0;
+}
+
+extern "C" int fl3d_replay(const void *job, size_t job_bytes, uint32_t *isaac_host, int nwalkers,
+                           double batch_size, int sbs, int fuse, double *badvals) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   return iterate_replay(job, job_bytes, isaac_host, nwalkers, batch_size, sbs, fuse, badvals);
+}
+
+// ------------------------------------------------------------------ stage entry points (fl3h.h)
+
+// Pack a randctx into the 36-word device form.
+static void pack_isaac(const randctx *rc, uint32_t *s) {
+   s[0] = (uint32_t)rc->randcnt;
+   for (int i = 0; i < 16; i++) s[1 + i] = (uint32_t)rc->randrsl[i];
+   for (int i = 0; i < 16; i++) s[17 + i] = (uint32_t)rc->randmem[i];
+   s[33] = (uint32_t)rc->randa;
+   s[34] = (uint32_t)rc->randb;
+   s[35] = (uint32_t)rc->randc;
+}
+
+// Back to the host layout.  randmem words the device rewrote are 32-bit;
+// words it never touched keep their host (possibly 64-bit) value.
+static void unpack_isaac(const uint32_t *s, randctx *rc) {
+   const int refilled = (uint32_t)rc->randc != s[35];
+   rc->randcnt = s[0];
+   for (int i = 0; i < 16; i++) rc->randrsl[i] = s[1 + i];
+   if (refilled)
+      for (int i = 0; i < 16; i++) rc->randmem[i] = s[17 + i];
+   rc->randa = s[33];
+   rc->randb = s[34];
+   rc->randc = s[35];
+}
+
+extern "C" int fl3h_iterate(flam3_genome *cp, int n, int fuse, double *samples,
+                            const unsigned short *xform_distrib, randctx *rc, int nwalkers,
+                            int *badvals) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   if (ctx_init()) return 1;
+   if (nwalkers < 1 || n < 0) { set_err("fl3h_iterate: bad sizes"); return 1; }
+   // flam3_iterate recomputes the per-variation constants of every xform
+   // on the caller's genome (flam3.c:246-247)
+   for (int i = 0; i < cp->num_xforms; i++) fl3_xform_precalc(&cp->xform[i]);
+   size_t jb = 0;
+   void *job = fl3_pack_job(cp, xform_distrib, NULL, &jb);
+   if (!job) return 1;
+   const fl3_job_hdr *h = (const fl3_job_hdr *)job;
+   uint32_t *st = (uint32_t *)malloc(sizeof(uint32_t) * 36 * nwalkers);
+   for (int w = 0; w < nwalkers; w++) pack_isaac(&rc[w], st + 36 * w);
+   size_t sb = sizeof(double) * 4 * (size_t)nwalkers;
+   size_t ob = sizeof(double) * 4 * (size_t)n * nwalkers;
+   int rcode = 1;
+   do {
+      if (upload_job(job, jb)) break;
+      if (g_ctx.isaac.ensure(sizeof(uint32_t) * 36 * nwalkers + sizeof(int) * nwalkers)) break;
+      if (g_ctx.points.ensure(sb + ob + 16)) break;
+      double *dstart = (double *)g_ctx.points.p;
+      double *dout = dstart + 4 * (size_t)nwalkers;
+      int *dbad = (int *)((uint32_t *)g_ctx.isaac.p + 36 * nwalkers);
+      if (hipMemcpyAsync(g_ctx.isaac.p, st, sizeof(uint32_t) * 36 * nwalkers, hipMemcpyHostToDevice, g_ctx.stream) != hipSuccess) break;
+      double *starts = (double *)malloc(sb);
+      for (int w = 0; w < nwalkers; w++)
+         for (int k = 0; k < 4; k++) starts[4 * w + k] = samples[(size_t)w * 4 * n + k];
+      if (hipMemcpyAsync(dstart, starts, sb, hipMemcpyHostToDevice, g_ctx.stream) != hipSuccess) { set_err("device copy failed"); break; }
+      PointsArgs a;
+      a.job = (const unsigned char *)g_ctx.job.p;
+      a.start = dstart;
+      a.out = dout;
+      a.isaac = (uint32_t *)g_ctx.isaac.p;
+      a.badvals = dbad;
+      a.n = n;
+      a.fuse = fuse;
+      a.nwalkers = nwalkers;
+      if (fl3k_launch_points(&a, (nwalkers + 63) / 64, lds_bytes_for<double>(h), g_ctx.stream)) {
+         set_err("k_iterate_points launch failed");
+         free(starts);
+         break;
+      }
+      int *bad = (int *)malloc(sizeof(int) * nwalkers);
+      if (hipMemcpyAsync(samples, dout, ob, hipMemcpyDeviceToHost, g_ctx.stream) != hipSuccess) { set_err("device copy failed"); break; }
+      if (hipMemcpyAsync(st, g_ctx.isaac.p, sizeof(uint32_t) * 36 * nwalkers, hipMemcpyDeviceToHost, g_ctx.stream) != hipSuccess) { set_err("device copy failed"); break; }
+      if (hipMemcpyAsync(bad, dbad, sizeof(int) * nwalkers, hipMemcpyDeviceToHost, g_ctx.stream) != hipSuccess) { set_err("device copy failed"); break; }
+      if (hipStreamSynchronize(g_ctx.stream) != hipSuccess) { set_err("k_iterate_points failed"); free(starts); free(bad); break; }
+      for (int w = 0; w < nwalkers; w++) {
+         unpack_isaac(st + 36 * w, &rc[w]);
+         if (badvals) badvals[w] = bad[w];
+      }
+      free(starts);
+      free(bad);
+      rcode = 0;
+   } while (0);
+   free(st);
+   free(job);
+   return rcode;
+}
+
+extern "C" int fl3h_xform_preview(flam3_genome *cp, int xi, double range, int numvals, int depth,
+                                  double *result, randctx *rc) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   if (ctx_init()) return 1;
+   if (xi < 0 || xi >= cp->num_xforms || numvals < 0) { set_err("fl3h_xform_preview: bad arguments"); return 1; }
+   // as flam3.c:313-326: weight 1 for xi while preparing
+   const double oldweight = cp->xform[xi].density;
+   cp->xform[xi].density = 1.0;
+   if (fl3_prepare_precalc_flags(cp)) {
+      cp->xform[xi].density = oldweight;
+      set_err("fl3h_xform_preview: precalc failed");
+      return 1;
+   }
+   fl3_xform_precalc(&cp->xform[xi]);
+   size_t jb = 0;
+   void *job = fl3_pack_job(cp, NULL, NULL, &jb);
+   cp->xform[xi].density = oldweight;
+   if (!job) return 1;
+   const fl3_job_hdr *h = (const fl3_job_hdr *)job;
+   const size_t nres = 2 * (size_t)(2 * numvals + 1) * (2 * numvals + 1);
+   uint32_t st[36];
+   pack_isaac(rc, st);
+   int rcode = 1;
+   do {
+      if (upload_job(job, jb)) break;
+      if (g_ctx.isaac.ensure(sizeof(st))) break;
+      if (g_ctx.points.ensure(sizeof(double) * nres)) break;
+      if (hipMemcpyAsync(g_ctx.isaac.p, st, sizeof(st), hipMemcpyHostToDevice, g_ctx.stream) != hipSuccess) { set_err("device copy failed"); break; }
+      if (fl3k_launch_preview((const unsigned char *)g_ctx.job.p, xi, range, numvals, depth,
+                              (double *)g_ctx.points.p, (uint32_t *)g_ctx.isaac.p, lds_bytes_for<double>(h),
+                              g_ctx.stream)) {
+         set_err("k_xform_preview launch failed");
+         break;
+      }
+      if (hipMemcpyAsync(result, g_ctx.points.p, sizeof(double) * nres, hipMemcpyDeviceToHost, g_ctx.stream) != hipSuccess) { set_err("device copy failed"); break; }
+      if (hipMemcpyAsync(st, g_ctx.isaac.p, sizeof(st), hipMemcpyDeviceToHost, g_ctx.stream) != hipSuccess) { set_err("device copy failed"); break; }
+      if (hipStreamSynchronize(g_ctx.stream) != hipSuccess) { set_err("k_xform_preview failed"); break; }
+      unpack_isaac(st, rc);
+      rcode = 0;
+   } while (0);
+   free(job);
+   return rcode;
+}
+
+extern "C" int fl3h_deposit_points(const fl3h_camera *cam, const double *points, long long npoints,
+                                   int bits, void *buckets) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   if (ctx_init()) return 1;
+   if (bits != 32 && bits != 33 && bits != 64) { set_err("fl3h_deposit_points: bits must be 32, 33 or 64"); return 1; }
+   size_t jb = 0;
+   void *job = fl3_pack_camera_job(cam, &jb);
+   if (!job) return 1;
+   const fl3_job_hdr *h = (const fl3_job_hdr *)job;
+   const size_t cells = (size_t)cam->width * cam->height;
+   const size_t bb = cells * bucket_bytes(bits);
+   const size_t pb = sizeof(double) * 4 * (size_t)npoints;
+   int rcode = 1;
+   do {
+      if (upload_job(job, jb)) break;
+      if (g_ctx.points.ensure(pb + 16)) break;
+      if (g_ctx.misc.ensure(bb)) break;
+      if (hipMemcpyAsync(g_ctx.points.p, points, pb, hipMemcpyHostToDevice, g_ctx.stream) != hipSuccess) { set_err("device copy failed"); break; }
+      if (hipMemcpyAsync(g_ctx.misc.p, buckets, bb, hipMemcpyHostToDevice, g_ctx.stream) != hipSuccess) { set_err("device copy failed"); break; }
+      if (hipMemsetAsync(g_ctx.counters, 0, 16, g_ctx.stream) != hipSuccess) { set_err("device copy failed"); break; }
+      const size_t lds = lds_bytes_for<double>(h);
+      const dim3 grid(npoints > 0 ? (unsigned)((npoints + 255) / 256 < 4096 ? (npoints + 255) / 256 : 4096) : 1);
+      if (fl3k_launch_deposit((const unsigned char *)g_ctx.job.p, (const double *)g_ctx.points.p, npoints,
+                              g_ctx.misc.p, (uint32_t *)(g_ctx.counters + 1), grid.x, lds, g_ctx.stream, bits)) {
+         set_err("k_deposit_points launch failed");
+         break;
+      }
+      if (hipMemcpyAsync(buckets, g_ctx.misc.p, bb, hipMemcpyDeviceToHost, g_ctx.stream) != hipSuccess) { set_err("device copy failed"); break; }
+      if (hipStreamSynchronize(g_ctx.stream) != hipSuccess) { set_err("k_deposit_points failed"); break; }
+      rcode = 0;
+   } while (0);
+   free(job);
+   return rcode;
+}
+
+extern "C" int fl3h_density_estimate(const void *buckets, int width, int height, int bits, int oversample,
+                                     double k1, double k2, double curve, int max_filtered_counts,
+                                     int max_filter_index, int kernel_size, int nfilters,
+                                     const double *filter_widths, const double *filter_coefs, void *accum) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   if (ctx_init()) return 1;
+   if (bits != 33 && bits != 64) { set_err("fl3h_density_estimate: bits must be 33 or 64"); return 1; }
+   const size_t cells = (size_t)width * height;
+   if (g_ctx.buckets.ensure(cells * bucket_bytes(bits)) || g_ctx.accum.ensure(cells * accum_bytes(bits))) return 1;
+   g_ctx.width = width;
+   g_ctx.height = height;
+   g_ctx.bits = bits;
+   HIPCHK(hipMemcpyAsync(g_ctx.buckets.p, buckets, cells * bucket_bytes(bits), hipMemcpyHostToDevice, g_ctx.stream));
+   HIPCHK(hipMemcpyAsync(g_ctx.accum.p, accum, cells * accum_bytes(bits), hipMemcpyHostToDevice, g_ctx.stream));
+   fl3_de_filters de;
+   de.max_filtered_counts = max_filtered_counts;
+   de.max_filter_index = max_filter_index;
+   de.kernel_size = kernel_size;
+   de.nfilters = nfilters;
+   de.filter_widths = (double *)filter_widths;
+   de.filter_coefs = (double *)filter_coefs;
+   fl3d_de_params p;
+   p.oversample = oversample;
+   p.k1 = k1;
+   p.k2 = k2;
+   p.curve = curve;
+   if (max_filter_index > 0) {
+      if (fl3d_density(&p, &de)) return 1;
+   } else {
+      if (fl3d_logscale(k1, k2)) return 1;
+   }
+   HIPCHK(hipMemcpyAsync(accum, g_ctx.accum.p, cells * accum_bytes(bits), hipMemcpyDeviceToHost, g_ctx.stream));
+   HIPCHK(hipStreamSynchronize(g_ctx.stream));
+   return 0;
+}
+
+extern "C" int fl3h_tonemap(const void *accum, int width, int height, int bits, const fl3h_tonemap_params *tp,
+                            const double *filter, void *out) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   if (ctx_init()) return 1;
+   const size_t cells = (size_t)width * height;
+   if (g_ctx.accum.ensure(cells * accum_bytes(bits))) return 1;
+   g_ctx.width = width;
+   g_ctx.height = height;
+   g_ctx.bits = bits;
+   HIPCHK(hipMemcpyAsync(g_ctx.accum.p, accum, cells * accum_bytes(bits), hipMemcpyHostToDevice, g_ctx.stream));
+   fl3d_tm_params p;
+   p.image_width = tp->image_width;
+   p.image_height = tp->image_height;
+   p.oversample = tp->oversample;
+   p.de_offset = tp->de_offset;
+   p.filter_width = tp->filter_width;
+   p.nchan = tp->nchan;
+   p.transp = tp->transp;
+   p.bytes_per_channel = tp->bytes_per_channel;
+   p.earlyclip = tp->earlyclip;
+   p.g = tp->g;
+   p.linrange = tp->linrange;
+   p.vibrancy = tp->vibrancy;
+   p.highpow = tp->highpow;
+   for (int k = 0; k < 3; k++) p.background[k] = tp->background[k];
+   return fl3d_tonemap(&p, filter, out, (size_t)tp->image_width * tp->nchan * tp->bytes_per_channel);
+}
+

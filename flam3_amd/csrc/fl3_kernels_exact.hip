@@ -1,0 +1,413 @@
+// fl3_kernels_exact.hip - FP64 kernels that must follow the reference's
+// arithmetic operation by operation: replay walkers, batched iterate, xform
+// preview, point deposit, density estimation, log scaling and the tone map.
+// Compiled with FP contraction off (the reference's x86-64 build has no FMA).
+#include <type_traits>
+#include "fl3_core.h"
+#include "fl3_tonemath.h"
+#include "fl3_launch.h"
+
+// Replay walkers: walker w is reference iteration thread w (rect.c:253-527)
+// with its ISAAC state seeded on the host as rect.c:858-865 does, running
+// the same sub-batch loop in FP64 (exact-mode deposits).
+template <int BITS>
+__global__ __launch_bounds__(64) void k_iter_replay(IterArgs a) {
+   unsigned char *lds = fl3_lds;
+   GenL<double> G;
+   load_genome<double>(a.job, lds, G);
+   __syncthreads();
+   const int wid = blockIdx.x * blockDim.x + threadIdx.x;
+   if (wid >= a.nwalkers) return;
+   IsaacRng rng;
+   rng.load(a.isaac + 36 * wid);
+   uint32_t bad = 0;
+   const double bs = a.batch_size;
+   const bool use_final = G.final_index >= 0;
+   for (double sub = 0; sub < bs; sub += a.sbs) {
+      int sbs = (sub + a.sbs > bs) ? (int)(bs - sub) : a.sbs;
+      double p[4], q[4];
+      p[0] = u11<double>(rng);
+      p[1] = u11<double>(rng);
+      p[2] = u01<double>(rng);
+      p[3] = u01<double>(rng);
+      int lastxf = 0, consec = 0;
+      for (int i = -a.fuse; i < sbs;) {
+         const uint32_t r = rng.next();
+         const int fn = select_xform(G, G.chaos_enable ? lastxf : 0, r & 16383u);
+         if (apply_xform<double>(G, fn, p, q, rng)) {
+            consec++;
+            bad++;
+            if (consec < 5) {
+               p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
+               continue;
+            }
+            consec = 0;
+         } else
+            consec = 0;
+         lastxf = fn + 1;
+         p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
+         if (use_final) {
+            if (G.final_opacity == 1.0 || u01<double>(rng) < G.final_opacity) {
+               apply_xform<double>(G, G.final_index, p, q, rng);
+               q[3] = p[3];
+            }
+         }
+         if (i >= 0) deposit<double, BITS, true>(G, q, a.buckets, a.sat);
+         i++;
+      }
+   }
+   rng.store(a.isaac + 36 * wid);
+   if (bad) atomicAdd(a.badvals, (unsigned long long)bad);
+}
+
+// Batched flam3_iterate (flam3.c:232-299) in FP64: walker w starts from
+// start[4w..4w+3] with ISAAC state w and writes n samples.
+
+__global__ __launch_bounds__(64) void k_iterate_points(PointsArgs a) {
+   unsigned char *lds = fl3_lds;
+   GenL<double> G;
+   load_genome<double>(a.job, lds, G);
+   __syncthreads();
+   const int wid = blockIdx.x * blockDim.x + threadIdx.x;
+   if (wid >= a.nwalkers) return;
+   IsaacRng rng;
+   rng.load(a.isaac + 36 * wid);
+   double p[4], q[4];
+   for (int k = 0; k < 4; k++) p[k] = a.start[4 * wid + k];
+   int lastxf = 0, consec = 0, bad = 0;
+   const bool use_final = G.final_index >= 0;
+   double *out = a.out + (size_t)wid * a.n * 4;
+   for (int i = -a.fuse; i < a.n;) {
+      const uint32_t r = rng.next();
+      const int fn = select_xform(G, G.chaos_enable ? lastxf : 0, r & 16383u);
+      if (apply_xform<double>(G, fn, p, q, rng)) {
+         consec++;
+         bad++;
+         if (consec < 5) {
+            p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
+            continue;
+         }
+         consec = 0;
+      } else
+         consec = 0;
+      lastxf = fn + 1;
+      p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
+      if (use_final) {
+         if (G.final_opacity == 1.0 || u01<double>(rng) < G.final_opacity) {
+            apply_xform<double>(G, G.final_index, p, q, rng);
+            q[3] = p[3];
+         }
+      }
+      if (i >= 0) {
+         out[4 * i + 0] = q[0]; out[4 * i + 1] = q[1];
+         out[4 * i + 2] = q[2]; out[4 * i + 3] = q[3];
+      }
+      i++;
+   }
+   rng.store(a.isaac + 36 * wid);
+   a.badvals[wid] = bad;
+}
+
+// Apply xform xi `depth` times to each point of a grid (flam3.c:301-349).
+__global__ __launch_bounds__(64) void k_xform_preview(const unsigned char *job, int xi, double range,
+                                                      int numvals, int depth, double *result,
+                                                      uint32_t *isaac) {
+   unsigned char *lds = fl3_lds;
+   GenL<double> G;
+   load_genome<double>(job, lds, G);
+   __syncthreads();
+   if (blockIdx.x != 0 || threadIdx.x != 0) return;
+   IsaacRng rng;
+   rng.load(isaac);
+   const double incr = range / (double)numvals;
+   int outi = 0;
+   for (int xx = -numvals; xx <= numvals; xx++)
+      for (int yy = -numvals; yy <= numvals; yy++) {
+         double p[4] = {(double)xx * incr, (double)yy * incr, 0.0, 0.0}, q[4];
+         for (int d = 0; d < depth; d++) {
+            apply_xform<double>(G, xi, p, q, rng);
+            p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
+         }
+         result[outi] = p[0];
+         result[outi + 1] = p[1];
+         outi += 2;
+      }
+   rng.store(isaac);
+}
+
+// Deposit a given point stream (stage test of rect.c:411-516).
+template <int BITS>
+__global__ __launch_bounds__(256) void k_deposit_points(const unsigned char *job, const double *pts, long long n,
+                                                        void *buckets, uint32_t *sat) {
+   unsigned char *lds = fl3_lds;
+   GenL<double> G;
+   load_genome<double>(job, lds, G);
+   __syncthreads();
+   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+        i += (long long)gridDim.x * blockDim.x) {
+      double q[4] = {pts[4 * i], pts[4 * i + 1], pts[4 * i + 2], pts[4 * i + 3]};
+      deposit<double, BITS, true>(G, q, buckets, sat);
+   }
+}
+
+// ------------------------------------------------------------------ density estimation
+
+
+template <typename BT>
+__device__ __forceinline__ double bval(const BT *b, long cell, int ch) {
+   return (double)b[cell * 5 + ch];
+}
+
+template <typename AT>
+__device__ __forceinline__ void acc_add(AT *acc, long cell, const double *c) {
+   for (int k = 0; k < 4; k++) unsafeAtomicAdd(acc + cell * 4 + k, (AT)c[k]);
+}
+
+// One thread per source cell.  Rows at or beyond the histogram height (the
+// reference reads one row past its array for oversample 1 and 2) count as
+// empty.
+template <typename BT, typename AT>
+__global__ __launch_bounds__(256) void k_de(DEArgs a) {
+   const int os = a.oversample;
+   const int ss = os / 2;
+   const bool scf = !(os & 1);
+   const double scfact = pow(os / (os + 1.0), 2.0);
+   const int W = a.width, H = a.height;
+   const int i = (os - 1) + blockIdx.x * blockDim.x + threadIdx.x;
+   const int j = (os - 1) + blockIdx.y;
+   if (i >= W - (os - 1)) return;
+   if (j >= H) return;  // rows past the end hold nothing
+   const BT *b = (const BT *)a.buckets;
+   const long cell = i + (long)j * W;
+   const double c4 = bval(b, cell, 4), c3 = bval(b, cell, 3);
+   if (c4 == 0 || c3 == 0) return;
+   double fsel = 0.0;
+   for (int ii = -ss; ii <= ss; ii++)
+      for (int jj = -ss; jj <= ss; jj++) {
+         const int y = j + jj;
+         if (y < H) fsel += bval(b, (i + ii) + (long)y * W, 4) / 255.0;
+      }
+   if (scf) fsel *= scfact;
+   int fi;
+   if (fsel > a.max_filtered_counts)
+      fi = a.max_filter_index;
+   else if (fsel <= 100)
+      fi = (int)ceil(fsel) - 1;
+   else
+      fi = 100 + (int)floor(pow(fsel - 100, a.curve));
+   if (fi > a.max_filter_index) fi = a.max_filter_index;
+   int cidx = fi * a.kernel_size;
+   const int afw = (int)ceil(a.widths[fi]) - 1;
+   const double c0 = bval(b, cell, 0), c1 = bval(b, cell, 1), c2 = bval(b, cell, 2);
+   const double lsb = a.k1 * log(1.0 + c3 * a.k2);
+   AT *acc = (AT *)a.accum;
+   for (int jj = 0; jj <= afw; jj++) {
+      for (int ii = 0; ii <= jj; ii++, cidx++) {
+         const double coef = a.coefs[cidx];
+         if (coef == 0.0) continue;
+         const double ls = coef * lsb / c3;
+         const double c[4] = {c0 * ls, c1 * ls, c2 * ls, c3 * ls};
+         int dx[8], dy[8], nn;
+         if (jj == 0 && ii == 0) {
+            dx[0] = 0; dy[0] = 0; nn = 1;
+         } else if (ii == 0) {
+            dx[0] = jj; dy[0] = 0; dx[1] = -jj; dy[1] = 0;
+            dx[2] = 0; dy[2] = jj; dx[3] = 0; dy[3] = -jj; nn = 4;
+         } else if (jj == ii) {
+            dx[0] = ii; dy[0] = jj; dx[1] = -ii; dy[1] = jj;
+            dx[2] = ii; dy[2] = -jj; dx[3] = -ii; dy[3] = -jj; nn = 4;
+         } else {
+            dx[0] = ii; dy[0] = jj; dx[1] = -ii; dy[1] = jj;
+            dx[2] = ii; dy[2] = -jj; dx[3] = -ii; dy[3] = -jj;
+            dx[4] = jj; dy[4] = ii; dx[5] = -jj; dy[5] = ii;
+            dx[6] = jj; dy[6] = -ii; dx[7] = -jj; dy[7] = -ii; nn = 8;
+         }
+         for (int t = 0; t < nn; t++) {
+            const int x = i + dx[t], y = j + dy[t];
+            if (y >= 0 && y < H && x >= 0 && x < W) acc_add(acc, x + (long)y * W, c);
+         }
+      }
+   }
+}
+
+// Log scaling of every cell without DE (rect.c:947-974).
+template <typename BT, typename AT>
+__global__ __launch_bounds__(256) void k_logscale(const BT *b, AT *acc, long ncells, double k1, double k2) {
+   const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (cell >= ncells) return;
+   const double c3 = (double)b[cell * 5 + 3];
+   if (c3 == 0.0) return;
+   const double ls = (k1 * log(1.0 + c3 * k2)) / c3;
+   for (int k = 0; k < 4; k++) {
+      const double c = (double)b[cell * 5 + k] * ls;
+      AT &d = acc[cell * 4 + k];
+      if constexpr (std::is_same<AT, uint32_t>::value) {
+         // 32-bit integer accumulator: saturating (flam3.c:3774-3776)
+         d = (0xffffffffu - d > c) ? (uint32_t)((double)d + c) : 0xffffffffu;
+      } else {
+         d = (AT)((double)d + c);
+      }
+   }
+}
+
+// ------------------------------------------------------------------ tone map
+
+
+__device__ __forceinline__ void tone_pixel(const double *t, double *out_rgb, double &alpha, const TMArgs &a) {
+   double ls, newrgb[3];
+   const double tmp = t[3] / FL3_PREFILTER_WHITE;
+   if (t[3] <= 0) {
+      alpha = 0.0;
+      ls = 0.0;
+   } else {
+      alpha = fl3_calc_alpha(tmp, a.g, a.linrange);
+      ls = a.vibrancy * 256.0 * alpha / tmp;
+      if (alpha < 0.0) alpha = 0.0;
+      if (alpha > 1.0) alpha = 1.0;
+   }
+   fl3_calc_newrgb(t, ls, a.highpow, newrgb);
+   for (int k = 0; k < 3; k++) {
+      double v = newrgb[k];
+      v += (1.0 - a.vibrancy) * 256.0 * pow(t[k] / FL3_PREFILTER_WHITE, a.g);
+      if (a.nchan <= 3 || a.transp == 0)
+         v += ((1.0 - alpha) * a.bg[k]);
+      else
+         v = (alpha > 0) ? v / alpha : 0;
+      if (v > 255) v = 255;
+      if (v < 0) v = 0;
+      out_rgb[k] = v;
+   }
+}
+
+// earlyclip: tone map every accumulator cell in place before filtering
+// (rect.c:1086-1135).
+template <typename AT>
+__global__ __launch_bounds__(256) void k_earlyclip(AT *acc, long ncells, TMArgs a) {
+   const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (cell >= ncells) return;
+   AT *ac = acc + cell * 4;
+   double t[4] = {(double)ac[0], (double)ac[1], (double)ac[2], (double)ac[3]}, rgb[3], alpha;
+   tone_pixel(t, rgb, alpha, a);
+   for (int k = 0; k < 3; k++) ac[k] = (AT)rgb[k];
+   ac[3] = (AT)alpha;
+}
+
+template <typename AT>
+__global__ __launch_bounds__(256) void k_tonemap(TMArgs a) {
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   const int j = blockIdx.y;
+   if (i >= a.iw || j >= a.ih) return;
+   const AT *acc = (const AT *)a.accum;
+   const int x = a.de_offset + i * a.os, y = a.de_offset + j * a.os;
+   double t[4] = {0.0, 0.0, 0.0, 0.0};
+   for (int ii = 0; ii < a.fw; ii++)
+      for (int jj = 0; jj < a.fw; jj++) {
+         const double k = a.filter[ii + jj * a.fw];
+         const AT *ac = acc + ((x + ii) + (long)(y + jj) * a.W) * 4;
+         t[0] += k * (double)ac[0];
+         t[1] += k * (double)ac[1];
+         t[2] += k * (double)ac[2];
+         t[3] += k * (double)ac[3];
+      }
+   if (!a.earlyclip) {
+      double rgb[3], alpha;
+      tone_pixel(t, rgb, alpha, a);
+      t[0] = rgb[0]; t[1] = rgb[1]; t[2] = rgb[2];
+      t[3] = alpha;
+   }
+   unsigned char *p = a.out + (size_t)a.nchan * a.bpc * ((size_t)i + (size_t)j * a.iw);
+   for (int k = 0; k < 3; k++) {
+      double v = t[k];
+      if (v > 255) v = 255;
+      if (v < 0) v = 0;
+      if (a.bpc == 2)
+         ((unsigned short *)p)[k] = (unsigned short)(v * 256.0);
+      else
+         p[k] = (unsigned char)v;
+   }
+   if (t[3] > 1) t[3] = 1;
+   if (t[3] < 0) t[3] = 0;
+   if (a.nchan > 3) {
+      if (a.transp == 1) {
+         if (a.bpc == 2) ((unsigned short *)p)[3] = (unsigned short)(t[3] * 65535);
+         else p[3] = (unsigned char)(t[3] * 255);
+      } else {
+         if (a.bpc == 2) ((unsigned short *)p)[3] = 65535;
+         else p[3] = 255;
+      }
+   }
+}
+
+// ------------------------------------------------------------------ launchers
+
+extern "C" int fl3k_launch_replay(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int bits) {
+   if (bits == 64)
+      hipLaunchKernelGGL((k_iter_replay<64>), dim3(grid), dim3(64), lds, s, *a);
+   else
+      hipLaunchKernelGGL((k_iter_replay<33>), dim3(grid), dim3(64), lds, s, *a);
+   return hipGetLastError() != hipSuccess;
+}
+
+extern "C" int fl3k_launch_points(const PointsArgs *a, unsigned grid, size_t lds, hipStream_t s) {
+   hipLaunchKernelGGL(k_iterate_points, dim3(grid), dim3(64), lds, s, *a);
+   return hipGetLastError() != hipSuccess;
+}
+
+extern "C" int fl3k_launch_preview(const unsigned char *job, int xi, double range, int numvals, int depth,
+                                   double *result, uint32_t *isaac, size_t lds, hipStream_t s) {
+   hipLaunchKernelGGL(k_xform_preview, dim3(1), dim3(64), lds, s, job, xi, range, numvals, depth, result, isaac);
+   return hipGetLastError() != hipSuccess;
+}
+
+extern "C" int fl3k_launch_deposit(const unsigned char *job, const double *pts, long long n, void *buckets,
+                                   uint32_t *sat, unsigned grid, size_t lds, hipStream_t s, int bits) {
+   if (bits == 64)
+      hipLaunchKernelGGL(k_deposit_points<64>, dim3(grid), dim3(256), lds, s, job, pts, n, buckets, sat);
+   else
+      hipLaunchKernelGGL(k_deposit_points<33>, dim3(grid), dim3(256), lds, s, job, pts, n, buckets, sat);
+   return hipGetLastError() != hipSuccess;
+}
+
+extern "C" int fl3k_launch_de(const DEArgs *a, unsigned gx, unsigned gy, hipStream_t s, int bits) {
+   if (bits == 64)
+      hipLaunchKernelGGL((k_de<double, double>), dim3(gx, gy), dim3(256), 0, s, *a);
+   else
+      hipLaunchKernelGGL((k_de<uint32_t, float>), dim3(gx, gy), dim3(256), 0, s, *a);
+   return hipGetLastError() != hipSuccess;
+}
+
+extern "C" int fl3k_launch_logscale(const void *b, void *acc, long n, double k1, double k2, hipStream_t s,
+                                    int bits) {
+   const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+   if (bits == 64)
+      hipLaunchKernelGGL((k_logscale<double, double>), grid, block, 0, s, (const double *)b, (double *)acc, n, k1, k2);
+   else if (bits == 33)
+      hipLaunchKernelGGL((k_logscale<uint32_t, float>), grid, block, 0, s, (const uint32_t *)b, (float *)acc, n, k1,
+                         k2);
+   else
+      hipLaunchKernelGGL((k_logscale<uint32_t, uint32_t>), grid, block, 0, s, (const uint32_t *)b, (uint32_t *)acc,
+                         n, k1, k2);
+   return hipGetLastError() != hipSuccess;
+}
+
+extern "C" int fl3k_launch_earlyclip(void *acc, long n, const TMArgs *a, hipStream_t s, int bits) {
+   const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+   if (bits == 64)
+      hipLaunchKernelGGL((k_earlyclip<double>), grid, block, 0, s, (double *)acc, n, *a);
+   else if (bits == 33)
+      hipLaunchKernelGGL((k_earlyclip<float>), grid, block, 0, s, (float *)acc, n, *a);
+   else
+      hipLaunchKernelGGL((k_earlyclip<uint32_t>), grid, block, 0, s, (uint32_t *)acc, n, *a);
+   return hipGetLastError() != hipSuccess;
+}
+
+extern "C" int fl3k_launch_tonemap(const TMArgs *a, hipStream_t s, int bits) {
+   const dim3 grid((a->iw + 255) / 256, a->ih), block(256);
+   if (bits == 64)
+      hipLaunchKernelGGL((k_tonemap<double>), grid, block, 0, s, *a);
+   else if (bits == 33)
+      hipLaunchKernelGGL((k_tonemap<float>), grid, block, 0, s, *a);
+   else
+      hipLaunchKernelGGL((k_tonemap<uint32_t>), grid, block, 0, s, *a);
+   return hipGetLastError() != hipSuccess;
+}

@@ -1,0 +1,24 @@
+// fl3_launch.h - host-callable launchers of the kernels in
+// fl3_kernels_fast.hip and fl3_kernels_exact.hip.  Each returns non-zero if
+// the launch failed.
+#ifndef FL3_LAUNCH_H
+#define FL3_LAUNCH_H
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+#include "fl3_kargs.h"
+
+extern "C" {
+int fl3k_launch_iter_fast(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int bits, int exact);
+int fl3k_launch_replay(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int bits);
+int fl3k_launch_points(const PointsArgs *a, unsigned grid, size_t lds, hipStream_t s);
+int fl3k_launch_preview(const unsigned char *job, int xi, double range, int numvals, int depth, double *result,
+                        uint32_t *isaac, size_t lds, hipStream_t s);
+int fl3k_launch_deposit(const unsigned char *job, const double *pts, long long n, void *buckets, uint32_t *sat,
+                        unsigned grid, size_t lds, hipStream_t s, int bits);
+int fl3k_launch_de(const DEArgs *a, unsigned gx, unsigned gy, hipStream_t s, int bits);
+int fl3k_launch_logscale(const void *b, void *acc, long n, double k1, double k2, hipStream_t s, int bits);
+int fl3k_launch_earlyclip(void *acc, long n, const TMArgs *a, hipStream_t s, int bits);
+int fl3k_launch_tonemap(const TMArgs *a, hipStream_t s, int bits);
+}
+#endif

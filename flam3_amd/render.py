@@ -1,0 +1,65 @@
+"""Python front end of flam3_render (libflam3_amd.so).
+
+``render(genomes, ...)`` mirrors what flam3-render does around the library
+call (flam3-render.c:108-425: frame set-up from environment-style settings,
+one flam3_render call, the image buffer) and returns the image as a numpy
+array of shape (height, width, nchan).
+"""
+import ctypes
+
+import numpy as np
+
+from . import abi
+from ._lib import check, fl3h_render_opts, lib
+from .genome import Genome, genome_array
+
+
+def make_frame(genome_ptr, ngenomes, *, bits=33, bpc=1, earlyclip=0, time=0.0, nthreads=1,
+               sub_batch_size=10000, pixel_aspect=1.0, seed="flam3", progress=None):
+    """A flam3_frame as flam3-render fills it (flam3-render.c:150-230)."""
+    f = abi.flam3_frame()
+    f.pixel_aspect_ratio = pixel_aspect
+    f.genomes = ctypes.cast(genome_ptr, ctypes.c_void_p)
+    f.ngenomes = ngenomes
+    f.verbose = 0
+    f.bits = bits
+    f.bytes_per_channel = bpc
+    f.earlyclip = earlyclip
+    f.time = time
+    f.progress = ctypes.cast(progress, ctypes.c_void_p) if progress is not None else None
+    f.progress_parameter = None
+    f.nthreads = nthreads
+    f.sub_batch_size = sub_batch_size
+    lib().fl3h_init_frame(ctypes.byref(f), seed.encode() if seed is not None else None)
+    return f
+
+
+def render_frame(frame, width, height, *, nchan=3, transp=0, field=0, replay=False, nwalkers=0):
+    """Run fl3h_render on a prepared frame; returns (image, stats)."""
+    bpc = frame.bytes_per_channel
+    dtype = np.uint16 if bpc == 2 else np.uint8
+    img = np.zeros((height, width, nchan), dtype=dtype)
+    stats = abi.stat_struct()
+    opts = fl3h_render_opts(1 if replay else 0, int(nwalkers))
+    rc = lib().fl3h_render(ctypes.byref(frame), img.ctypes.data_as(ctypes.c_void_p), field, nchan, transp,
+                           ctypes.byref(stats), ctypes.byref(opts))
+    check(rc, "flam3_render")
+    return img, {"badvals": stats.badvals, "num_iters": stats.num_iters,
+                 "render_seconds": stats.render_seconds}
+
+
+def render(genomes, *, nchan=3, transp=0, bits=33, bpc=1, earlyclip=0, time=0.0, nthreads=1,
+           sub_batch_size=10000, seed="flam3", replay=False, nwalkers=0, field=0):
+    """Render Genome descriptions (one still, or an animation frame at
+    ``time`` when several control points are given)."""
+    if isinstance(genomes, Genome):
+        genomes = [genomes]
+    arr, owned = genome_array(genomes)
+    try:
+        f = make_frame(ctypes.addressof(arr), len(genomes), bits=bits, bpc=bpc, earlyclip=earlyclip,
+                       time=time, nthreads=nthreads, sub_batch_size=sub_batch_size, seed=seed)
+        return render_frame(f, genomes[0].width, genomes[0].height, nchan=nchan, transp=transp,
+                            field=field, replay=replay, nwalkers=nwalkers)
+    finally:
+        for gp in owned:
+            lib().fl3h_genome_free(gp)

@@ -1,0 +1,124 @@
+/*
+ * fl3h.h - C ABI of the MI355X flam3 render path (libflam3_amd.so).
+ *
+ * The drop-in entry points are flam3_render and flam3_render_memory_required
+ * with the reference's signatures and semantics.  The fl3h_* functions expose
+ * the individual stages on host buffers so that each can be checked against
+ * the reference in isolation; the reference has these stages only as static
+ * functions or inline loops, cited per function below.  Every function
+ * returns 0 on success, or non-zero with a message in fl3h_last_error().
+ * Plain C types only; no device pointers cross this interface.
+ */
+#ifndef FL3H_H
+#define FL3H_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "flam3_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Replaces flam3_render (flam3.h:623, flam3.c:3894-3932; body rect.c:529-1286).
+ * out: W*H*nchan*bytes_per_channel bytes, rows of W pixels (every other row
+ * with field = even/odd).  Returns 1 on invalid parameters (message on
+ * stderr), 0 otherwise, including after an abort through f->progress. */
+int flam3_render(flam3_frame *f, void *out, int field, int nchan, int transp, stat_struct *stats);
+
+/* Replaces flam3_render_memory_required (flam3.h:625, flam3.c:3874-3887):
+ * the reference's estimate, ss^2 * W * H * bytes * 9. */
+double flam3_render_memory_required(flam3_frame *f);
+
+/* Options of the extended render call. */
+typedef struct {
+   int replay;     /* 0: FP32 production walkers.  1: FP64 walkers that each
+                      replay one reference iteration thread (f->nthreads of
+                      them), with the reference's ISAAC streams and sub-batches */
+   int nwalkers;   /* production walkers per launch, 0 = library choice */
+} fl3h_render_opts;
+
+/* flam3_render with options; flam3_render(f, ...) == fl3h_render(f, ..., NULL). */
+int fl3h_render(flam3_frame *f, void *out, int field, int nchan, int transp, stat_struct *stats,
+                const fl3h_render_opts *opts);
+
+/* Batched flam3_iterate (flam3.h:555, flam3.c:232-299) in FP64 on the GPU:
+ * walker w runs the reference loop from samples[w*4n .. w*4n+3] with ISAAC
+ * context rc[w] and writes its n samples to samples[w*4n ...]; each rc[w] is
+ * advanced as the reference advances it.  Like the reference, the caller has
+ * run the precalc flags and built xform_distrib (flam3_create_xform_distrib
+ * layout); the per-variation constants are recomputed on cp in place.
+ * badvals (may be NULL) receives the per-walker bad-value counts. */
+int fl3h_iterate(flam3_genome *cp, int n, int fuse, double *samples,
+                 const unsigned short *xform_distrib, randctx *rc, int nwalkers, int *badvals);
+
+/* flam3_xform_preview (flam3.h:543, flam3.c:301-349) on the GPU in FP64:
+ * apply xform xi `depth` times to each point of the (2*numvals+1)^2 grid over
+ * [-range, range]^2; result receives x,y pairs.  Prepares cp like the
+ * reference does (precalc flags with xi's weight set to 1 temporarily). */
+int fl3h_xform_preview(flam3_genome *cp, int xi, double range, int numvals, int depth,
+                       double *result, randctx *rc);
+
+/* Camera and palette of one temporal sample, as rect.c:778-827 sets them up. */
+typedef struct {
+   double bounds[4];       /* x0 y0 x1 y1, inclusive, gutter included */
+   double rot[2][2];
+   double rot_center[2];
+   int rotate;             /* genome rotate != 0 */
+   double ws0, wb0s0, hs1, hb1s1;
+   int width, height;      /* histogram cells */
+   int cmap_size;          /* 256 - apo_palette */
+   int palette_mode;       /* 0 step, 1 linear */
+   double dmap[256][4];    /* palette colour x 255 x temporal weight */
+} fl3h_camera;
+
+/* Deposit loop of rect.c:411-516 for a given point stream (x, y, colour,
+ * visibility per point) into a histogram in the reference layout: 5 channels
+ * per cell, uint32 (bits 32/33; truncated, saturating adds as
+ * flam3.c:3690-3706) or double (bits 64).  buckets is read and updated. */
+int fl3h_deposit_points(const fl3h_camera *cam, const double *points, long long npoints,
+                        int bits, void *buckets);
+
+/* Density estimation (de_thread, rect.c:57-251, with add_c_to_accum
+ * flam3.c:3827-3835) or, when max_filter_index == 0, the plain log scaling
+ * of rect.c:947-974.  buckets: width*height cells of 5 channels (uint32 for
+ * bits 33, double for 64); accum: 4 channels (float / double), read and
+ * updated.  The filter bank is flam3_create_de_filters' (filters.c:272). */
+int fl3h_density_estimate(const void *buckets, int width, int height, int bits, int oversample,
+                          double k1, double k2, double curve, int max_filtered_counts,
+                          int max_filter_index, int kernel_size, int nfilters,
+                          const double *filter_widths, const double *filter_coefs, void *accum);
+
+typedef struct {
+   int image_width, image_height, oversample, de_offset, filter_width;
+   int nchan, transp, bytes_per_channel, earlyclip;
+   double g;               /* 1 / mean gamma */
+   double linrange;        /* gam_lin_thresh */
+   double vibrancy, highpow;
+   double background[3];   /* already scaled by 256 / nsamples as rect.c:1079 */
+} fl3h_tonemap_params;
+
+/* Spatial filter + tone map + quantisation of rect.c:1067-1247 (including
+ * the earlyclip pre-pass) from an accumulator (4 channels, float for bits 33,
+ * double for 64, uint32 for 32) into out (image_width*image_height*nchan*bpc). */
+int fl3h_tonemap(const void *accum, int width, int height, int bits, const fl3h_tonemap_params *tp,
+                 const double *filter, void *out);
+
+/* Seed f->rc from an explicit string the way flam3_init_frame seeds it from
+ * the isaac_seed environment variable (flam3.h:639, flam3.c:2494-2516);
+ * NULL seeds from the current time. */
+void fl3h_init_frame(flam3_frame *f, const char *isaac_seed);
+
+/* Genome construction for callers without the XML parser: defaults as
+ * clear_cp (flam3.c:1270) and parse_flame_element leave them, nstd standard
+ * xforms (+ a final xform) with all variation weights 0.  Free with
+ * fl3h_genome_free. */
+flam3_genome *fl3h_genome_new(int nstd, int has_final);
+void fl3h_genome_free(flam3_genome *g);
+
+const char *fl3h_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
