@@ -25,8 +25,9 @@ VAR_INDEX = {n: i for i, n in enumerate(abi.VARIATION_NAMES)}
 WORKLOAD_VARS = ["julia", "swirl", "horseshoe", "polar", "handkerchief", "heart", "disc", "spiral",
                  "hyperbolic", "diamond", "ex", "linear"]
 
-_KERNELS = ["gaussian", "hermite", "box", "triangle", "bell", "bspline", "mitchell", "blackman",
-            "catrom", "hanning", "hamming", "lanczos3", "lanczos2", "quadratic"]
+# Spatial filter names by flam3.h kernel id (flam3.h:673-686, parser.c:418-449).
+_KERNELS = ["gaussian", "hermite", "box", "triangle", "bell", "bspline", "lanczos3", "lanczos2",
+            "mitchell", "blackman", "catrom", "hamming", "hanning", "quadratic"]
 
 
 @dataclass

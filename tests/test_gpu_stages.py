@@ -1,0 +1,293 @@
+"""GPU stage parity: each HIP stage of libflam3_amd.so (through its C ABI)
+against the oracle restatement on the same inputs.
+
+Tolerances: integer histogram work bit-exact; FP64 iteration 1e-9 relative
+over 16 steps (the first refill of the walkers' ISAAC streams falls inside
+that window); the FP64 tone map within 1 LSB at 8 bits per channel.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from flam3_amd import abi
+from flam3_amd import genome as G
+from flam3_amd._lib import fl3h_camera, fl3h_tonemap_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _prepared(ref, g):
+    p = ref.Parsed(G.flame_document([g]))
+    R = ref.ref()
+    assert R.prepare_precalc_flags(p.ptr) == 0
+    dist = R.flam3_create_xform_distrib(p.ptr)
+    return p, dist
+
+
+# ---------------------------------------------------------------- iteration
+
+@pytest.mark.parametrize("mk,steps,tol", [
+    (lambda: G.synthetic_small(seed=51, width=64, height=64), 16, 1e-9),
+    (lambda: G.synthetic_8k(seed=53, width=64, height=64), 16, 1e-9),
+    (lambda: G.synthetic_1080p(seed=52, width=64, height=64), 8, 1e-9),
+], ids=["contractive", "xaos_final", "expanding"])
+def test_iterate_fp64_replay(prod, oracle, ref, mk, steps, tol):
+    """fl3h_iterate (FP64, device ISAAC) vs or_iterate with the same ISAAC
+    contexts and start points.  The expanding genome amplifies the ulp-level
+    libm differences by ~10x per step, so its window is 8 steps; every
+    walker's RNG state must match exactly at the end either way."""
+    p, dist = _prepared(ref, mk())
+    try:
+        W, n = 64, steps
+        rng = np.random.default_rng(0)
+        starts = rng.uniform(-1, 1, (W, 4))
+        starts[:, 2:] = np.abs(starts[:, 2:])
+        want = np.zeros((W, 4 * n))
+        rc_o = (abi.randctx * W)()
+        rc_g = (abi.randctx * W)()
+        for w in range(W):
+            rc_o[w] = oracle.seeded_rc("walker%d" % w)
+            rc_g[w] = oracle.seeded_rc("walker%d" % w)
+            want[w, :4] = starts[w]
+            oracle.lib().or_iterate(p.ptr, n, 0, want[w].ctypes.data_as(ctypes.c_void_p), dist,
+                                    ctypes.byref(rc_o[w]))
+        got = np.zeros((W, 4 * n))
+        got[:, :4] = starts
+        bad = np.zeros(W, dtype=np.int32)
+        assert prod.lib().fl3h_iterate(p.ptr, n, 0, got.ctypes.data_as(ctypes.c_void_p), dist, rc_g, W,
+                                       bad.ctypes.data_as(ctypes.c_void_p)) == 0, prod.last_error()
+        rel = np.abs(got - want) / np.maximum(1.0, np.abs(want))
+        assert rel.max() <= tol, rel.reshape(W, n, 4).max(axis=(0, 2))
+        for w in range(W):
+            assert list(rc_o[w].randrsl) == list(rc_g[w].randrsl)
+            assert rc_o[w].randcnt == rc_g[w].randcnt
+    finally:
+        p.close()
+
+
+def _variation_genome(vi):
+    rng = np.random.default_rng(1000 + vi)
+    name = abi.VARIATION_NAMES[vi]
+    x = G.Xform(weight=1.0, color=0.5, coefs=tuple(rng.uniform(-1, 1, 6)),
+                variations={name: float(rng.uniform(0.3, 1.2))})
+    if name == "pre_blur":
+        x.variations["linear"] = 1.0
+    for f, _ in abi.flam3_xform._fields_:
+        if f.startswith(name.split("_")[0] + "_"):
+            x.params[f] = float(rng.uniform(0.2, 2.0))
+    return G.Genome(width=32, height=32, xforms=[x])
+
+
+# Variations with a discontinuity (floor/fmod/rint/int cast or a branch on
+# the coordinates): an ulp of libm difference can move a grid point across
+# it, so a few of the 169 points may differ by O(1) while the RNG stays in step.
+DISCONTINUOUS = {"rings", "fan", "fan2", "rings2", "rectangles", "secant2", "ngon", "boarders", "cell", "cpow",
+                 "modulus", "oscilloscope", "split", "stripes", "wedge", "wedge_julia", "wedge_sph", "whorl",
+                 "lazysusan", "loonie", "bipolar", "separation", "splits", "bent", "bent2", "julian",
+                 "juliascope", "pie", "twintrian", "elliptic", "edisc", "flower", "conic", "super_shape"}
+
+
+@pytest.mark.parametrize("vi", range(99))
+def test_variation_fp64(prod, oracle, vi):
+    """fl3h_xform_preview (one xform, 3 applications on a 13x13 grid) vs the
+    oracle: within 1e-9 relative; for variations with discontinuities at
+    most 3 of 169 points may land on the other side of an edge."""
+    g = _variation_genome(vi)
+    name = abi.VARIATION_NAMES[vi]
+    nv, depth = 6, 3
+    gp = g.to_struct()
+    try:
+        got = np.zeros(2 * (2 * nv + 1) ** 2)
+        rc = oracle.seeded_rc("var%d" % vi)
+        assert prod.lib().fl3h_xform_preview(gp, 0, 2.0, nv, depth, got.ctypes.data_as(ctypes.c_void_p),
+                                             ctypes.byref(rc)) == 0, prod.last_error()
+    finally:
+        G.free_struct(gp)
+    import pyref
+    with pyref.Parsed(G.flame_document([g])) as p:
+        pyref.ref().prepare_precalc_flags(p.ptr)
+        pyref.ref().xform_precalc(p.ptr, 0)
+        rco = oracle.seeded_rc("var%d" % vi)
+        want = []
+        incr = 2.0 / nv
+        for xx in range(-nv, nv + 1):
+            for yy in range(-nv, nv + 1):
+                pt = np.array([xx * incr, yy * incr, 0.0, 0.0])
+                q = np.zeros(4)
+                for _ in range(depth):
+                    oracle.lib().or_apply_xform(p.ptr, 0, pt.ctypes.data_as(ctypes.c_void_p),
+                                                q.ctypes.data_as(ctypes.c_void_p), ctypes.byref(rco))
+                    pt[:] = q
+                want += [pt[0], pt[1]]
+    want = np.array(want)
+    both_nan = np.isnan(want) & np.isnan(got)
+    rel = np.where(both_nan, 0.0, np.abs(got - want) / np.maximum(1.0, np.abs(want)))
+    nbad = int((rel.reshape(-1, 2).max(axis=1) > 1e-9).sum())
+    allowed = 3 if name in DISCONTINUOUS else 0
+    assert nbad <= allowed, (name, nbad, rel.max())
+    assert list(rc.randrsl) == list(rco.randrsl) and rc.randcnt == rco.randcnt
+
+
+# ---------------------------------------------------------------- deposit
+
+def _camera(rng, width=64, height=48, rotate=False, linear=False, scalar=1.0):
+    cam = fl3h_camera()
+    cam.bounds[0], cam.bounds[1], cam.bounds[2], cam.bounds[3] = -2.0, -1.5, 2.0, 1.5
+    ang = 0.3 if rotate else 0.0
+    cam.rot[0][0] = np.cos(ang)
+    cam.rot[0][1] = -np.sin(ang)
+    cam.rot[1][0] = np.sin(ang)
+    cam.rot[1][1] = np.cos(ang)
+    cam.rot_center[0], cam.rot_center[1] = 0.1, -0.2
+    cam.rotate = 1 if rotate else 0
+    cam.width, cam.height = width, height
+    cam.ws0 = width * (1.0 / 4.0)
+    cam.wb0s0 = cam.ws0 * -2.0
+    cam.hs1 = height * (1.0 / 3.0)
+    cam.hb1s1 = cam.hs1 * -1.5
+    cam.cmap_size = 256
+    cam.palette_mode = 1 if linear else 0
+    pal = rng.uniform(0, 1, (256, 4))
+    for j in range(256):
+        for k in range(4):
+            cam.dmap[j][k] = pal[j, k] * 255.0 * scalar
+    return cam
+
+
+def _points(rng, n, vis_mode):
+    pts = np.zeros((n, 4))
+    pts[:, 0] = rng.uniform(-2.3, 2.3, n)
+    pts[:, 1] = rng.uniform(-1.8, 1.8, n)
+    pts[:, 2] = rng.uniform(-0.05, 1.05, n)
+    pts[:, 3] = 1.0 if vis_mode == "one" else rng.choice([0.0, 0.25, 0.7, 1.0], n)
+    return pts
+
+
+@pytest.mark.parametrize("bits", [33, 64])
+@pytest.mark.parametrize("vis_mode,rotate,linear,scalar", [("one", False, False, 1.0), ("mixed", True, False, 0.6),
+                                                           ("mixed", False, True, 1.0)])
+def test_deposit_points(prod, oracle, bits, vis_mode, rotate, linear, scalar):
+    """Same point stream -> same histogram: bit-exact hit counts and colour
+    sums for uint32 buckets (the reference's atomic-add semantics), and
+    FP64 buckets within 1e-12 relative (summation order differs)."""
+    rng = np.random.default_rng(7)
+    cam = _camera(rng, rotate=rotate, linear=linear, scalar=scalar)
+    pts = _points(rng, 200000, vis_mode)
+    dt = np.float64 if bits == 64 else np.uint32
+    want = np.zeros((48, 64, 5), dtype=dt)
+    oracle.lib().or_deposit(ctypes.byref(cam), pts.ctypes.data_as(ctypes.c_void_p), len(pts), bits, 1,
+                            want.ctypes.data_as(ctypes.c_void_p))
+    got = np.zeros_like(want)
+    assert prod.lib().fl3h_deposit_points(ctypes.byref(cam), pts.ctypes.data_as(ctypes.c_void_p), len(pts), bits,
+                                          got.ctypes.data_as(ctypes.c_void_p)) == 0, prod.last_error()
+    if bits == 64:
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=0)
+    else:
+        assert np.array_equal(got[:, :, 4], want[:, :, 4])
+        assert np.array_equal(got, want)
+
+
+def test_deposit_saturates_like_reference(prod, oracle):
+    """uint32 channels saturate at UINT_MAX instead of wrapping
+    (flam3.c:3690-3706)."""
+    rng = np.random.default_rng(8)
+    cam = _camera(rng, width=4, height=3)
+    pts = np.tile(np.array([[0.0, 0.0, 0.5, 1.0]]), (1000, 1))
+    start = np.zeros((3, 4, 5), dtype=np.uint32)
+    start[:, :, :] = 0xffffffff - 100000
+    want = start.copy()
+    oracle.lib().or_deposit(ctypes.byref(cam), pts.ctypes.data_as(ctypes.c_void_p), len(pts), 33, 1,
+                            want.ctypes.data_as(ctypes.c_void_p))
+    got = start.copy()
+    assert prod.lib().fl3h_deposit_points(ctypes.byref(cam), pts.ctypes.data_as(ctypes.c_void_p), len(pts), 33,
+                                          got.ctypes.data_as(ctypes.c_void_p)) == 0
+    assert np.array_equal(got, want)
+    assert got.max() == 0xffffffff
+
+
+# ---------------------------------------------------------------- density estimation + tone map
+
+def _buckets(rng, W, H, bits):
+    b = np.zeros((H, W, 5), dtype=np.float64)
+    hits = rng.poisson(rng.uniform(0.0, 3.0, (H, W)) ** 3)
+    hits[: H // 6] = 0
+    b[:, :, 4] = hits * 255
+    b[:, :, 3] = hits * 255
+    for k in range(3):
+        b[:, :, k] = np.floor(hits * rng.uniform(0, 255, (H, W)))
+    return b if bits == 64 else b.astype(np.uint32)
+
+
+def _de_tables(ref, rad, mn, curve, ss):
+    mfc, mfi, ks = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    wp, cp_ = ctypes.POINTER(ctypes.c_double)(), ctypes.POINTER(ctypes.c_double)()
+    assert ref.ref().ref_de_filters(rad, mn, curve, ss, ctypes.byref(mfc), ctypes.byref(mfi), ctypes.byref(ks),
+                                    ctypes.byref(wp), ctypes.byref(cp_)) == 0
+    return mfc.value, mfi.value, ks.value, wp, cp_
+
+
+@pytest.mark.parametrize("bits,ss,rad", [(33, 1, 9.0), (33, 2, 9.0), (33, 3, 5.0), (64, 2, 7.0)])
+def test_density_estimation(prod, oracle, ref, bits, ss, rad):
+    rng = np.random.default_rng(9 + ss)
+    W, H = 150, 90
+    b = _buckets(rng, W, H, bits)
+    mfc, mfi, ks, wp, cp_ = _de_tables(ref, rad, 0.0, 0.4, ss)
+    adt = np.float64 if bits == 64 else np.float32
+    want = np.zeros((H, W, 4), dtype=adt)
+    k1, k2 = 4.0 * 255 * 268 / 256, 1.0 / 2000.0
+    oracle.lib().or_density(b.ctypes.data_as(ctypes.c_void_p), W, H, bits, ss, k1, k2, 0.4, mfc, mfi, ks, wp, cp_,
+                            want.ctypes.data_as(ctypes.c_void_p), 0)
+    got = np.zeros_like(want)
+    assert prod.lib().fl3h_density_estimate(b.ctypes.data_as(ctypes.c_void_p), W, H, bits, ss, k1, k2, 0.4, mfc,
+                                            mfi, ks, mfi + 1, wp, cp_,
+                                            got.ctypes.data_as(ctypes.c_void_p)) == 0, prod.last_error()
+    tol = 1e-12 if bits == 64 else 2e-6
+    np.testing.assert_allclose(got, want, rtol=tol, atol=tol * float(np.abs(want).max()))
+
+
+def _tm_params(W, H, ss, fw, **kw):
+    de_offset = kw.pop("de_offset", 3)
+    iw, ih = (W - 2 * de_offset - fw) // ss, (H - 2 * de_offset - fw) // ss
+    tp = fl3h_tonemap_params(image_width=iw, image_height=ih, oversample=ss, de_offset=de_offset, filter_width=fw,
+                             nchan=kw.get("nchan", 3), transp=kw.get("transp", 0),
+                             bytes_per_channel=kw.get("bpc", 1), earlyclip=kw.get("earlyclip", 0),
+                             g=1.0 / kw.get("gamma", 4.0), linrange=0.01, vibrancy=kw.get("vibrancy", 1.0),
+                             highpow=kw.get("highpow", -1.0))
+    bg = kw.get("bg", (0.0, 0.0, 0.0))
+    for k in range(3):
+        tp.background[k] = bg[k]
+    return tp
+
+
+@pytest.mark.parametrize("case", [
+    dict(), dict(earlyclip=1), dict(nchan=4, transp=1), dict(bpc=2, nchan=4), dict(vibrancy=0.4, highpow=1.0),
+    dict(highpow=0.0, bg=(40.0, 80.0, 120.0), gamma=2.2)], ids=lambda c: "-".join("%s%s" % kv for kv in c.items())
+    or "default")
+@pytest.mark.parametrize("bits", [33, 64, 32])
+def test_tonemap(prod, oracle, case, bits):
+    """Same accumulator -> same image, within 1 LSB at 8 bpc (and 256 at
+    16 bpc, which scales 8-bit values by 256)."""
+    rng = np.random.default_rng(11)
+    W, H, ss, fw = 70, 50, 2, 4
+    adt = {33: np.float32, 64: np.float64, 32: np.uint32}[bits]
+    acc = (rng.uniform(0, 1, (H, W, 4)) ** 4 * 3000).astype(adt)
+    acc[:, :, 3] = np.maximum(acc[:, :, 3], acc[:, :, :3].max(axis=2) * 0.5).astype(adt)
+    filt = np.outer(np.array([0.1, 0.4, 0.4, 0.1]), np.array([0.1, 0.4, 0.4, 0.1])).ravel()
+    tp = _tm_params(W, H, ss, fw, **case)
+    nchan, bpc = tp.nchan, tp.bytes_per_channel
+    dt = np.uint16 if bpc == 2 else np.uint8
+    want = np.zeros((tp.image_height, tp.image_width, nchan), dtype=dt)
+    a1 = acc.copy()
+    oracle.lib().or_tonemap(a1.ctypes.data_as(ctypes.c_void_p), W, H, bits, ctypes.byref(tp),
+                            filt.ctypes.data_as(ctypes.c_void_p), want.ctypes.data_as(ctypes.c_void_p),
+                            tp.image_width)
+    got = np.zeros_like(want)
+    a2 = acc.copy()
+    assert prod.lib().fl3h_tonemap(a2.ctypes.data_as(ctypes.c_void_p), W, H, bits, ctypes.byref(tp),
+                                   filt.ctypes.data_as(ctypes.c_void_p),
+                                   got.ctypes.data_as(ctypes.c_void_p)) == 0, prod.last_error()
+    d = np.abs(got.astype(np.int64) - want.astype(np.int64))
+    lsb = 256 if bpc == 2 else 1
+    assert d.max() <= lsb, d.max()
+    assert (d > 0).mean() < 0.01
