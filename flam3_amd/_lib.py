@@ -22,7 +22,8 @@ EXPORTS = [
 
 
 class fl3h_render_opts(ctypes.Structure):
-    _fields_ = [("replay", ctypes.c_int), ("nwalkers", ctypes.c_int)]
+    _fields_ = [("replay", ctypes.c_int), ("nwalkers", ctypes.c_int), ("deposit", ctypes.c_int),
+                ("chunk_samples", ctypes.c_longlong)]
 
 
 class fl3h_camera(ctypes.Structure):

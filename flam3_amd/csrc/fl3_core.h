@@ -429,6 +429,36 @@ __device__ __forceinline__ void deposit(const GenL<R> &G, const R *q, void *buck
    }
 }
 
+// ------------------------------------------------------------------ binned deposit records
+//
+// A plotted point reduced to what its deposit needs: bits 0-31 cell index
+// (FL3_REC_NONE: nothing to deposit), 32-39 palette index, 40-47 index of
+// the xform whose visibility the point carries, 48-63 the linear-palette
+// blend fraction in 16-bit fixed point.
+#define FL3_REC_NONE 0xffffffffu
+
+template <typename R>
+__device__ __forceinline__ unsigned long long make_record(const GenL<R> &G, const R *q, int fn) {
+#pragma clang fp contract(off)
+   R x = q[0], y = q[1];
+   if (G.rotate) {
+      R x0 = x - G.rc0, y0 = y - G.rc1;
+      x = x0 * G.rot00 + y0 * G.rot01 + G.rc0;
+      y = x0 * G.rot10 + y0 * G.rot11 + G.rc1;
+   }
+   if (!(x >= G.b0 && y >= G.b1 && x <= G.b2 && y <= G.b3) || q[3] == R(0)) return FL3_REC_NONE;
+   const long ix = (long)(int)(G.ws0 * x - G.wb0s0);
+   const long iy = (long)(int)(G.hs1 * y - G.hb1s1);
+   const long cell = ix + (long)G.width * iy;
+   if (cell < 0 || cell >= (long)G.width * G.height) return FL3_REC_NONE;
+   int ci;
+   R frac;
+   point_color(G, q[2], ci, frac);
+   R f = frac < R(0) ? R(0) : (frac > R(1) ? R(1) : frac);
+   const unsigned long long qf = (unsigned long long)(f * R(65535.0) + R(0.5));
+   return (unsigned long long)cell | ((unsigned long long)ci << 32) | ((unsigned long long)fn << 40) | (qf << 48);
+}
+
 // Sum of per-walker bad-value counts: wave reduction, one atomic per wave.
 __device__ __forceinline__ void add_bad(unsigned long long *dst, uint32_t bad) {
    unsigned long long v = bad;

@@ -28,6 +28,8 @@ typedef struct {
    uint64_t key[2];     /* walker stream key drawn from the frame's ISAAC */
    int exact;           /* saturating adds on every channel */
    int nwalkers;        /* 0: chosen by the library */
+   int deposit;         /* 0 auto, 1 atomics, 2 tile-binned */
+   int64_t chunk;       /* binned: samples per chunk, 0 = library choice */
 } fl3d_iter_params;
 
 /* Iterate one job (one genome at one temporal sample) and deposit into the

@@ -78,8 +78,10 @@ struct Ctx {
    int ready = 0;
    int device = 0;
    int ncu = 256;
+   size_t lds_max = 65536;   // LDS bytes one workgroup may allocate
    hipStream_t stream = nullptr;
-   DevBuf buckets, accum, job, table, image, isaac, points, misc;
+   DevBuf buckets, accum, job, table, image, isaac, points, misc, findex;
+   DevBuf rec, sorted, wstate, bins;   // tile-binned deposit
    unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
    int width = 0, height = 0, bits = 33;
 };
@@ -93,9 +95,87 @@ static int ctx_init() {
    hipDeviceProp_t prop;
    HIPCHK(hipGetDeviceProperties(&prop, g_ctx.device));
    g_ctx.ncu = prop.multiProcessorCount;
+   g_ctx.lds_max = prop.sharedMemPerBlock;
    HIPCHK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
    HIPCHK(hipMalloc((void **)&g_ctx.counters, 64));
    g_ctx.ready = 1;
+   return 0;
+}
+
+// ------------------------------------------------------------------ stage timing
+//
+// Optional HIP-event timing of each stage on the context's stream (the
+// stream every kernel runs on).  Pending intervals are folded in at the next
+// synchronisation point of the render.
+struct StageTimer {
+   int enabled = 0;
+   hipEvent_t ev[64][2];
+   int stage[64];
+   int n = 0, created = 0;
+   double ms[5] = {0, 0, 0, 0, 0};
+   long long launches[5] = {0, 0, 0, 0, 0};
+   long long samples = 0, cells = 0;
+};
+static StageTimer g_prof;
+enum { ST_ITER = 0, ST_DE = 1, ST_TONE = 2, ST_BIN = 3, ST_OTHER = 4 };
+
+static void prof_flush() {
+   for (int i = 0; i < g_prof.n; i++) {
+      float ms = 0.0f;
+      if (hipEventSynchronize(g_prof.ev[i][1]) == hipSuccess &&
+          hipEventElapsedTime(&ms, g_prof.ev[i][0], g_prof.ev[i][1]) == hipSuccess) {
+         g_prof.ms[g_prof.stage[i]] += ms;
+         g_prof.launches[g_prof.stage[i]]++;
+      }
+   }
+   g_prof.n = 0;
+}
+
+static int prof_begin(int stage) {
+   if (!g_prof.enabled) return -1;
+   if (!g_prof.created) {
+      for (int i = 0; i < 64; i++) {
+         if (hipEventCreate(&g_prof.ev[i][0]) != hipSuccess || hipEventCreate(&g_prof.ev[i][1]) != hipSuccess)
+            return -1;
+      }
+      g_prof.created = 1;
+   }
+   if (g_prof.n == 64) prof_flush();
+   const int k = g_prof.n++;
+   g_prof.stage[k] = stage;
+   (void)hipEventRecord(g_prof.ev[k][0], g_ctx.stream);
+   return k;
+}
+
+static void prof_end(int k) {
+   if (k >= 0) (void)hipEventRecord(g_prof.ev[k][1], g_ctx.stream);
+}
+
+extern "C" void fl3h_profile_reset(int enable) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   prof_flush();
+   g_prof.enabled = enable;
+   for (int i = 0; i < 5; i++) {
+      g_prof.ms[i] = 0.0;
+      g_prof.launches[i] = 0;
+   }
+   g_prof.samples = 0;
+   g_prof.cells = 0;
+}
+
+extern "C" int fl3h_profile_get(fl3h_stage_times *out) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   prof_flush();
+   out->iterate_ms = g_prof.ms[ST_ITER];
+   out->iterate_launches = g_prof.launches[ST_ITER];
+   out->density_ms = g_prof.ms[ST_DE];
+   out->density_launches = g_prof.launches[ST_DE];
+   out->tonemap_ms = g_prof.ms[ST_TONE];
+   out->tonemap_launches = g_prof.launches[ST_TONE];
+   out->samples = g_prof.samples;
+   out->cells = g_prof.cells;
+   out->bin_ms = g_prof.ms[ST_BIN];
+   out->bin_launches = g_prof.launches[ST_BIN];
    return 0;
 }
 
@@ -148,6 +228,79 @@ static long long choose_walkers(long long nsamples, int requested) {
    return (w + 255) / 256 * 256;
 }
 
+// Tile shift of the binned deposit for the current histogram (cells per
+// tile = 1 << shift), or 0 when the histogram is too large for it.
+static int bin_shift() {
+   const long long ncells = (long long)g_ctx.width * g_ctx.height;
+   for (int sh = 11; sh <= 12; sh++)
+      if (((ncells + (1LL << sh) - 1) >> sh) <= FL3_MAX_BINS && fl3k_bin_lds_bytes(sh) <= g_ctx.lds_max) return sh;
+   return 0;
+}
+
+// Binned deposit: the walkers write records in chunks of `per` rows (the
+// walker state carries over between chunks, so the trajectories are those of
+// a single launch); each chunk is tile-sorted and accumulated.
+static int iterate_binned(IterArgs a, long long W, size_t lds, const fl3d_iter_params *p, int shift) {
+   const long long ncells = (long long)g_ctx.width * g_ctx.height;
+   const int nbins = (int)((ncells + (1LL << shift) - 1) >> shift);
+   long long per = p->chunk > 0 ? (p->chunk + W - 1) / W : (FL3_BIN_CHUNK_SAMPLES / W);
+   if (per < 1) per = 1;
+   const long long rows_total = (p->nsamples + W - 1) / W;
+   if (per > rows_total) per = rows_total;
+   if ((double)W * per >= 4.0e9) {
+      set_err("binned deposit chunk exceeds 2^32 records");
+      return 1;
+   }
+   const size_t rec_bytes = sizeof(unsigned long long) * (size_t)W * per;
+   if (g_ctx.rec.ensure(rec_bytes) || g_ctx.sorted.ensure(rec_bytes)) return 1;
+   if (g_ctx.wstate.ensure(sizeof(uint32_t) * 10 * (size_t)W)) return 1;
+   const size_t bin_words = 2 * (size_t)nbins + 2 * ((size_t)nbins + 1) + 4;
+   if (g_ctx.bins.ensure(sizeof(unsigned) * bin_words)) return 1;
+   unsigned *bw = (unsigned *)g_ctx.bins.p;
+   BinArgs b;
+   memset(&b, 0, sizeof(b));
+   b.job = a.job;
+   b.buckets = a.buckets;
+   b.rec = (const unsigned long long *)g_ctx.rec.p;
+   b.sorted = (unsigned long long *)g_ctx.sorted.p;
+   b.bin_total = bw;
+   b.bin_fill = bw + nbins;
+   b.bin_start = bw + 2 * nbins;
+   b.item_start = bw + 3 * nbins + 1;
+   b.item_counter = bw + 4 * nbins + 2;
+   b.shift = shift;
+   b.nbins = nbins;
+   b.chunk = FL3_BIN_ITEM_RECORDS;
+   b.width = g_ctx.width;
+   b.height = g_ctx.height;
+   a.rec = (unsigned long long *)g_ctx.rec.p;
+   a.wstate = (uint32_t *)g_ctx.wstate.p;
+   for (long long done = 0; done < p->nsamples;) {
+      const long long c = p->nsamples - done < W * per ? p->nsamples - done : W * per;
+      a.nsamples = c;
+      a.rows_out = (c + W - 1) / W;
+      a.resume = done > 0;
+      int pk = prof_begin(ST_ITER);
+      int lrc = fl3k_launch_iter_fast(&a, (unsigned)(W / 256), lds, g_ctx.stream, g_ctx.bits, 1);
+      prof_end(pk);
+      if (lrc) {
+         set_err("k_iter_fast launch failed");
+         return 1;
+      }
+      HIPCHK(hipMemsetAsync(bw, 0, sizeof(unsigned) * bin_words, g_ctx.stream));
+      b.n = W * a.rows_out;
+      pk = prof_begin(ST_BIN);
+      lrc = fl3k_launch_bin(&b, g_ctx.ncu, g_ctx.stream);
+      prof_end(pk);
+      if (lrc) {
+         set_err("binned deposit launch failed");
+         return 1;
+      }
+      done += c;
+   }
+   return 0;
+}
+
 extern "C" int fl3d_iterate(const void *job, size_t job_bytes, const fl3d_iter_params *p,
                             double *badvals, int *saturated) {
    std::lock_guard<std::recursive_mutex> lk(g_mu);
@@ -166,11 +319,26 @@ extern "C" int fl3d_iterate(const void *job, size_t job_bytes, const fl3d_iter_p
       a.badvals = g_ctx.counters;
       a.sat = (uint32_t *)(g_ctx.counters + 1);
       const long long W = choose_walkers(p->nsamples, p->nwalkers);
-      const dim3 grid((unsigned)(W / 256)), block(256);
       const size_t lds = lds_bytes_for<float>(h);
-      if (fl3k_launch_iter_fast(&a, grid.x, lds, g_ctx.stream, g_ctx.bits, p->exact)) {
-         set_err("k_iter_fast launch failed");
+      const int shift = bin_shift();
+      const bool binnable = g_ctx.bits != 64 && h->nxf <= 256 && shift > 0;
+      if (p->deposit == 2 && !binnable) {
+         set_err("tile-binned deposit needs bits 32/33, at most 256 xforms and at most %s%ld tiles", "",
+                 (long)FL3_MAX_BINS);
          return 1;
+      }
+      const bool binned = binnable && (p->deposit == 2 || (p->deposit == 0 && p->nsamples >= FL3_BIN_MIN_SAMPLES));
+      g_prof.samples += p->nsamples;
+      if (binned) {
+         if (iterate_binned(a, W, lds, p, shift)) return 1;
+      } else {
+         const int pk = prof_begin(ST_ITER);
+         const int lrc = fl3k_launch_iter_fast(&a, (unsigned)(W / 256), lds, g_ctx.stream, g_ctx.bits, p->exact);
+         prof_end(pk);
+         if (lrc) {
+            set_err("k_iter_fast launch failed");
+            return 1;
+         }
       }
    }
    return read_counters(badvals, saturated);
@@ -235,12 +403,21 @@ extern "C" int fl3d_density(const fl3d_de_params *p, const fl3_de_filters *de) {
    a.k1 = p->k1;
    a.k2 = p->k2;
    a.curve = p->curve;
-   const int os = p->oversample;
-   const int ni = g_ctx.width - 2 * (os - 1);
-   const int nj = g_ctx.height - 2 * (os - 1) + 1;
-   if (ni <= 0 || nj <= 0) return 0;
-   if (fl3k_launch_de(&a, (ni + 255) / 256, nj, g_ctx.stream, g_ctx.bits)) {
-      set_err("k_de launch failed");
+   int half = 0;
+   while ((half + 1) * (half + 2) / 2 < de->kernel_size) half++;
+   a.half = half;
+   if (de->max_filter_index >= 1024) {
+      set_err("density estimation: more than 1024 kernels");
+      return 1;
+   }
+   if (g_ctx.findex.ensure(sizeof(uint16_t) * (size_t)g_ctx.width * g_ctx.height)) return 1;
+   a.findex = (uint16_t *)g_ctx.findex.p;
+   const int pk = prof_begin(ST_DE);
+   const int lrc = fl3k_launch_de_tiled(&a, g_ctx.stream, g_ctx.bits);
+   prof_end(pk);
+   g_prof.cells = (long long)g_ctx.width * g_ctx.height;
+   if (lrc) {
+      set_err("density estimation launch failed");
       return 1;
    }
    return 0;
@@ -249,7 +426,10 @@ extern "C" int fl3d_density(const fl3d_de_params *p, const fl3_de_filters *de) {
 extern "C" int fl3d_logscale(double k1, double k2) {
    std::lock_guard<std::recursive_mutex> lk(g_mu);
    const long n = (long)g_ctx.width * g_ctx.height;
-   if (fl3k_launch_logscale(g_ctx.buckets.p, g_ctx.accum.p, n, k1, k2, g_ctx.stream, g_ctx.bits)) {
+   const int pk = prof_begin(ST_DE);
+   const int lrc = fl3k_launch_logscale(g_ctx.buckets.p, g_ctx.accum.p, n, k1, k2, g_ctx.stream, g_ctx.bits);
+   prof_end(pk);
+   if (lrc) {
       set_err("k_logscale launch failed");
       return 1;
    }
@@ -292,7 +472,10 @@ extern "C" int fl3d_tonemap(const fl3d_tm_params *p, const double *filter, void 
       }
    }
    if (p->image_width > 0 && p->image_height > 0) {
-      if (fl3k_launch_tonemap(&a, g_ctx.stream, g_ctx.bits)) {
+      const int pk = prof_begin(ST_TONE);
+      const int lrc = fl3k_launch_tonemap(&a, g_ctx.stream, g_ctx.bits);
+      prof_end(pk);
+      if (lrc) {
          set_err("k_tonemap launch failed");
          return 1;
       }

@@ -17,6 +17,37 @@ struct IterArgs {
    double batch_size;       // replay: samples per walker (double, rect.c:840)
    int sbs;                 // replay: sub-batch size
    int nwalkers;
+   // binned deposit: one 8-byte record per plotted sample, at i*W + walker;
+   // rows_out rows are written (walkers with fewer samples pad with no-ops)
+   unsigned long long *rec;
+   long long rows_out;
+   // walker state kept between launches of one batch (10 words per walker,
+   // structure of arrays): resume = continue from it without fuse steps
+   uint32_t *wstate;
+   int resume;
+};
+
+// Binned deposit limits: tiles per histogram (the sort keeps a counter per
+// tile in LDS), samples per chunk of records, records per accumulation work
+// item (bounds a tile cell's LDS partial sum by 2^20 * 255 < 2^32) and the
+// batch size from which the library prefers binning to atomics.
+#define FL3_MAX_BINS 8192
+#define FL3_BIN_CHUNK_SAMPLES (1LL << 28)
+#define FL3_BIN_ITEM_RECORDS (1 << 20)
+#define FL3_BIN_MIN_SAMPLES (1LL << 24)
+
+// Binned deposit (walker records -> tile-sorted records -> LDS tiles).
+struct BinArgs {
+   const unsigned char *job;       // for palette / visibility tables
+   void *buckets;
+   const unsigned long long *rec;  // records in walker order
+   unsigned long long *sorted;     // records grouped by tile
+   long long n;                    // records
+   unsigned *bin_total, *bin_fill, *bin_start, *item_start, *item_counter;
+   int shift;                      // cells per tile = 1 << shift
+   int nbins;                      // tiles
+   int chunk;                      // records per accumulation work item
+   int width, height;
 };
 
 struct PointsArgs {
@@ -33,8 +64,10 @@ struct DEArgs {
    void *accum;
    const double *coefs;
    const double *widths;
+   uint16_t *findex;        // per cell filter index, 0xffff = no contribution
    int width, height, oversample;
    int max_filtered_counts, max_filter_index, kernel_size;
+   int half;                // largest kernel radius (cells)
    double k1, k2, curve;
 };
 

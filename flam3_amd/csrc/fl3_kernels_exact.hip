@@ -153,80 +153,109 @@ __global__ __launch_bounds__(256) void k_deposit_points(const unsigned char *job
 // ------------------------------------------------------------------ density estimation
 
 
+// ---- tiled density estimation ------------------------------------------
+//
+// The reference scatters each source cell's kernel into the accumulator
+// (rect.c:126-173).  Here pass 1 picks every source's kernel (the hit-count
+// neighbourhood sum and index selection of rect.c:92-117) and pass 2 gives
+// each workgroup one output tile held in LDS: it visits every source within
+// the largest kernel radius of the tile, adds the part of that source's
+// kernel that falls inside the tile with LDS atomics, and writes the tile
+// back once.  No global atomics; each accumulator cell is read and written
+// once per batch.
+#define DE_SKIP 0xffffu
+#define DE_TW 64
+#define DE_TH 64
+
 template <typename BT>
-__device__ __forceinline__ double bval(const BT *b, long cell, int ch) {
-   return (double)b[cell * 5 + ch];
-}
-
-template <typename AT>
-__device__ __forceinline__ void acc_add(AT *acc, long cell, const double *c) {
-   for (int k = 0; k < 4; k++) unsafeAtomicAdd(acc + cell * 4 + k, (AT)c[k]);
-}
-
-// One thread per source cell.  Rows at or beyond the histogram height (the
-// reference reads one row past its array for oversample 1 and 2) count as
-// empty.
-template <typename BT, typename AT>
-__global__ __launch_bounds__(256) void k_de(DEArgs a) {
+__global__ __launch_bounds__(256) void k_de_prep(DEArgs a) {
    const int os = a.oversample;
    const int ss = os / 2;
    const bool scf = !(os & 1);
    const double scfact = pow(os / (os + 1.0), 2.0);
    const int W = a.width, H = a.height;
-   const int i = (os - 1) + blockIdx.x * blockDim.x + threadIdx.x;
-   const int j = (os - 1) + blockIdx.y;
-   if (i >= W - (os - 1)) return;
-   if (j >= H) return;  // rows past the end hold nothing
-   const BT *b = (const BT *)a.buckets;
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   const int j = blockIdx.y;
+   if (i >= W) return;
    const long cell = i + (long)j * W;
-   const double c4 = bval(b, cell, 4), c3 = bval(b, cell, 3);
-   if (c4 == 0 || c3 == 0) return;
-   double fsel = 0.0;
-   for (int ii = -ss; ii <= ss; ii++)
-      for (int jj = -ss; jj <= ss; jj++) {
-         const int y = j + jj;
-         if (y < H) fsel += bval(b, (i + ii) + (long)y * W, 4) / 255.0;
+   uint16_t out = DE_SKIP;
+   const BT *b = (const BT *)a.buckets;
+   if (i >= os - 1 && i < W - (os - 1) && j >= os - 1) {
+      const double c4 = (double)b[cell * 5 + 4], c3 = (double)b[cell * 5 + 3];
+      if (c4 != 0 && c3 != 0) {
+         double fsel = 0.0;
+         for (int ii = -ss; ii <= ss; ii++)
+            for (int jj = -ss; jj <= ss; jj++) {
+               const int y = j + jj;
+               if (y < H) fsel += (double)b[((i + ii) + (long)y * W) * 5 + 4] / 255.0;
+            }
+         if (scf) fsel *= scfact;
+         int fi;
+         if (fsel > a.max_filtered_counts)
+            fi = a.max_filter_index;
+         else if (fsel <= 100)
+            fi = (int)ceil(fsel) - 1;
+         else
+            fi = 100 + (int)floor(pow(fsel - 100, a.curve));
+         if (fi > a.max_filter_index) fi = a.max_filter_index;
+         out = (uint16_t)fi;
       }
-   if (scf) fsel *= scfact;
-   int fi;
-   if (fsel > a.max_filtered_counts)
-      fi = a.max_filter_index;
-   else if (fsel <= 100)
-      fi = (int)ceil(fsel) - 1;
-   else
-      fi = 100 + (int)floor(pow(fsel - 100, a.curve));
-   if (fi > a.max_filter_index) fi = a.max_filter_index;
-   int cidx = fi * a.kernel_size;
-   const int afw = (int)ceil(a.widths[fi]) - 1;
-   const double c0 = bval(b, cell, 0), c1 = bval(b, cell, 1), c2 = bval(b, cell, 2);
-   const double lsb = a.k1 * log(1.0 + c3 * a.k2);
+   }
+   a.findex[cell] = out;
+}
+
+template <typename BT, typename AT>
+__global__ __launch_bounds__(256) void k_de_tile(DEArgs a) {
+   __shared__ AT tile[DE_TH * DE_TW * 4];
+   __shared__ int radius[1024];
+   const int W = a.width, H = a.height, R = a.half;
+   const int tx0 = blockIdx.x * DE_TW, ty0 = blockIdx.y * DE_TH;
+   for (int e = threadIdx.x; e < DE_TH * DE_TW * 4; e += blockDim.x) tile[e] = (AT)0;
+   for (int e = threadIdx.x; e <= a.max_filter_index && e < 1024; e += blockDim.x)
+      radius[e] = (int)ceil(a.widths[e]) - 1;
+   __syncthreads();
+   const BT *b = (const BT *)a.buckets;
+   const int sx0 = max(0, tx0 - R), sx1 = min(W, tx0 + DE_TW + R);
+   const int sy0 = max(0, ty0 - R), sy1 = min(H, ty0 + DE_TH + R);
+   const int sw = sx1 - sx0, nsrc = sw * (sy1 - sy0);
+   for (int s = threadIdx.x; s < nsrc; s += blockDim.x) {
+      const int sx = sx0 + s % sw, sy = sy0 + s / sw;
+      const long cell = sx + (long)sy * W;
+      const unsigned fi = a.findex[cell];
+      if (fi == DE_SKIP) continue;
+      const int r = radius[fi];
+      const int x0 = max(sx - r, tx0), x1 = min(sx + r, min(tx0 + DE_TW, W) - 1);
+      const int y0 = max(sy - r, ty0), y1 = min(sy + r, min(ty0 + DE_TH, H) - 1);
+      if (x0 > x1 || y0 > y1) continue;
+      const double c0 = (double)b[cell * 5 + 0], c1 = (double)b[cell * 5 + 1];
+      const double c2 = (double)b[cell * 5 + 2], c3 = (double)b[cell * 5 + 3];
+      const double lsb = a.k1 * log(1.0 + c3 * a.k2);
+      const double *coefs = a.coefs + (long)fi * a.kernel_size;
+      for (int y = y0; y <= y1; y++) {
+         const int dy = abs(y - sy);
+         for (int x = x0; x <= x1; x++) {
+            const int dx = abs(x - sx);
+            const int hi = max(dx, dy), lo = min(dx, dy);
+            if (hi > r) continue;
+            const double coef = coefs[hi * (hi + 1) / 2 + lo];
+            if (coef == 0.0) continue;
+            const double ls = coef * lsb / c3;
+            AT *t = tile + ((y - ty0) * DE_TW + (x - tx0)) * 4;
+            atomicAdd(t + 0, (AT)(c0 * ls));
+            atomicAdd(t + 1, (AT)(c1 * ls));
+            atomicAdd(t + 2, (AT)(c2 * ls));
+            atomicAdd(t + 3, (AT)(c3 * ls));
+         }
+      }
+   }
+   __syncthreads();
    AT *acc = (AT *)a.accum;
-   for (int jj = 0; jj <= afw; jj++) {
-      for (int ii = 0; ii <= jj; ii++, cidx++) {
-         const double coef = a.coefs[cidx];
-         if (coef == 0.0) continue;
-         const double ls = coef * lsb / c3;
-         const double c[4] = {c0 * ls, c1 * ls, c2 * ls, c3 * ls};
-         int dx[8], dy[8], nn;
-         if (jj == 0 && ii == 0) {
-            dx[0] = 0; dy[0] = 0; nn = 1;
-         } else if (ii == 0) {
-            dx[0] = jj; dy[0] = 0; dx[1] = -jj; dy[1] = 0;
-            dx[2] = 0; dy[2] = jj; dx[3] = 0; dy[3] = -jj; nn = 4;
-         } else if (jj == ii) {
-            dx[0] = ii; dy[0] = jj; dx[1] = -ii; dy[1] = jj;
-            dx[2] = ii; dy[2] = -jj; dx[3] = -ii; dy[3] = -jj; nn = 4;
-         } else {
-            dx[0] = ii; dy[0] = jj; dx[1] = -ii; dy[1] = jj;
-            dx[2] = ii; dy[2] = -jj; dx[3] = -ii; dy[3] = -jj;
-            dx[4] = jj; dy[4] = ii; dx[5] = -jj; dy[5] = ii;
-            dx[6] = jj; dy[6] = -ii; dx[7] = -jj; dy[7] = -ii; nn = 8;
-         }
-         for (int t = 0; t < nn; t++) {
-            const int x = i + dx[t], y = j + dy[t];
-            if (y >= 0 && y < H && x >= 0 && x < W) acc_add(acc, x + (long)y * W, c);
-         }
-      }
+   for (int e = threadIdx.x; e < DE_TH * DE_TW; e += blockDim.x) {
+      const int x = tx0 + e % DE_TW, y = ty0 + e / DE_TW;
+      if (x >= W || y >= H) continue;
+      AT *d = acc + (x + (long)y * W) * 4;
+      const AT *t = tile + e * 4;
+      for (int k = 0; k < 4; k++) d[k] += t[k];
    }
 }
 
@@ -368,11 +397,16 @@ extern "C" int fl3k_launch_deposit(const unsigned char *job, const double *pts, 
    return hipGetLastError() != hipSuccess;
 }
 
-extern "C" int fl3k_launch_de(const DEArgs *a, unsigned gx, unsigned gy, hipStream_t s, int bits) {
-   if (bits == 64)
-      hipLaunchKernelGGL((k_de<double, double>), dim3(gx, gy), dim3(256), 0, s, *a);
-   else
-      hipLaunchKernelGGL((k_de<uint32_t, float>), dim3(gx, gy), dim3(256), 0, s, *a);
+extern "C" int fl3k_launch_de_tiled(const DEArgs *a, hipStream_t s, int bits) {
+   const dim3 g1((a->width + 255) / 256, a->height), b1(256);
+   const dim3 g2((a->width + DE_TW - 1) / DE_TW, (a->height + DE_TH - 1) / DE_TH), b2(256);
+   if (bits == 64) {
+      hipLaunchKernelGGL((k_de_prep<double>), g1, b1, 0, s, *a);
+      hipLaunchKernelGGL((k_de_tile<double, double>), g2, b2, 0, s, *a);
+   } else {
+      hipLaunchKernelGGL((k_de_prep<uint32_t>), g1, b1, 0, s, *a);
+      hipLaunchKernelGGL((k_de_tile<uint32_t, float>), g2, b2, 0, s, *a);
+   }
    return hipGetLastError() != hipSuccess;
 }
 

@@ -10,13 +10,15 @@
 
 extern "C" {
 int fl3k_launch_iter_fast(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int bits, int exact);
+int fl3k_launch_bin(const BinArgs *a, int ncu, hipStream_t s);
+size_t fl3k_bin_lds_bytes(int shift);
 int fl3k_launch_replay(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int bits);
 int fl3k_launch_points(const PointsArgs *a, unsigned grid, size_t lds, hipStream_t s);
 int fl3k_launch_preview(const unsigned char *job, int xi, double range, int numvals, int depth, double *result,
                         uint32_t *isaac, size_t lds, hipStream_t s);
 int fl3k_launch_deposit(const unsigned char *job, const double *pts, long long n, void *buckets, uint32_t *sat,
                         unsigned grid, size_t lds, hipStream_t s, int bits);
-int fl3k_launch_de(const DEArgs *a, unsigned gx, unsigned gy, hipStream_t s, int bits);
+int fl3k_launch_de_tiled(const DEArgs *a, hipStream_t s, int bits);
 int fl3k_launch_logscale(const void *b, void *acc, long n, double k1, double k2, hipStream_t s, int bits);
 int fl3k_launch_earlyclip(void *acc, long n, const TMArgs *a, hipStream_t s, int bits);
 int fl3k_launch_tonemap(const TMArgs *a, hipStream_t s, int bits);

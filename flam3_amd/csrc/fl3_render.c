@@ -308,6 +308,8 @@ int fl3h_render(flam3_frame *spec, void *out, int field, int nchan, int transp, 
                ip.key[1] = keys[2 * ts + 1];
                ip.exact = exact;
                ip.nwalkers = opts ? opts->nwalkers : 0;
+               ip.deposit = opts ? opts->deposit : 0;
+               ip.chunk = opts ? opts->chunk_samples : 0;
                if (fl3d_iterate(job, job_bytes, &ip, &bad, &sat)) {
                   free(job);
                   goto devfail;

@@ -34,13 +34,14 @@ def make_frame(genome_ptr, ngenomes, *, bits=33, bpc=1, earlyclip=0, time=0.0, n
     return f
 
 
-def render_frame(frame, width, height, *, nchan=3, transp=0, field=0, replay=False, nwalkers=0):
+def render_frame(frame, width, height, *, nchan=3, transp=0, field=0, replay=False, nwalkers=0, deposit=0,
+                 chunk_samples=0):
     """Run fl3h_render on a prepared frame; returns (image, stats)."""
     bpc = frame.bytes_per_channel
     dtype = np.uint16 if bpc == 2 else np.uint8
     img = np.zeros((height, width, nchan), dtype=dtype)
     stats = abi.stat_struct()
-    opts = fl3h_render_opts(1 if replay else 0, int(nwalkers))
+    opts = fl3h_render_opts(1 if replay else 0, int(nwalkers), int(deposit), int(chunk_samples))
     rc = lib().fl3h_render(ctypes.byref(frame), img.ctypes.data_as(ctypes.c_void_p), field, nchan, transp,
                            ctypes.byref(stats), ctypes.byref(opts))
     check(rc, "flam3_render")
@@ -49,7 +50,7 @@ def render_frame(frame, width, height, *, nchan=3, transp=0, field=0, replay=Fal
 
 
 def render(genomes, *, nchan=3, transp=0, bits=33, bpc=1, earlyclip=0, time=0.0, nthreads=1,
-           sub_batch_size=10000, seed="flam3", replay=False, nwalkers=0, field=0):
+           sub_batch_size=10000, seed="flam3", replay=False, nwalkers=0, field=0, deposit=0, chunk_samples=0):
     """Render Genome descriptions (one still, or an animation frame at
     ``time`` when several control points are given)."""
     if isinstance(genomes, Genome):
@@ -59,7 +60,8 @@ def render(genomes, *, nchan=3, transp=0, bits=33, bpc=1, earlyclip=0, time=0.0,
         f = make_frame(ctypes.addressof(arr), len(genomes), bits=bits, bpc=bpc, earlyclip=earlyclip,
                        time=time, nthreads=nthreads, sub_batch_size=sub_batch_size, seed=seed)
         return render_frame(f, genomes[0].width, genomes[0].height, nchan=nchan, transp=transp,
-                            field=field, replay=replay, nwalkers=nwalkers)
+                            field=field, replay=replay, nwalkers=nwalkers, deposit=deposit,
+                            chunk_samples=chunk_samples)
     finally:
         for gp in owned:
             lib().fl3h_genome_free(gp)

@@ -36,6 +36,10 @@ typedef struct {
                       replay one reference iteration thread (f->nthreads of
                       them), with the reference's ISAAC streams and sub-batches */
    int nwalkers;   /* production walkers per launch, 0 = library choice */
+   int deposit;    /* production histogram deposit: 0 library choice (tile-binned
+                      for large batches), 1 per-sample atomics, 2 tile-binned */
+   long long chunk_samples; /* binned deposit: samples per chunk (rounded up to
+                      a multiple of the walker count), 0 = library choice */
 } fl3h_render_opts;
 
 /* flam3_render with options; flam3_render(f, ...) == fl3h_render(f, ..., NULL). */
@@ -115,6 +119,20 @@ void fl3h_init_frame(flam3_frame *f, const char *isaac_seed);
  * fl3h_genome_free. */
 flam3_genome *fl3h_genome_new(int nstd, int has_final);
 void fl3h_genome_free(flam3_genome *g);
+
+/* Stage timing (no counterpart in the reference): with profiling enabled
+ * every kernel launch of the render path is bracketed by HIP events on the
+ * stream it runs on; totals accumulate until the next reset. */
+typedef struct {
+   double iterate_ms, density_ms, tonemap_ms;
+   long long iterate_launches, density_launches, tonemap_launches;
+   long long samples;      /* plotted samples launched */
+   long long cells;        /* histogram cells of the last density pass */
+   double bin_ms;          /* tile-binned deposit: sort + LDS accumulation */
+   long long bin_launches;
+} fl3h_stage_times;
+void fl3h_profile_reset(int enable);
+int fl3h_profile_get(fl3h_stage_times *out);
 
 const char *fl3h_last_error(void);
 

@@ -136,3 +136,58 @@ def test_progress_abort_returns_0_without_image(prod):
     finally:
         for gp in owned:
             G.free_struct(gp)
+
+
+BINNED_CASES = [
+    # (name, genome, render kwargs, walkers, chunk samples)
+    ("one_chunk", lambda: G.synthetic_small(seed=81, width=160, height=120, quality=30), {}, 2048, 0),
+    ("chunks_remainder", lambda: G.synthetic_1080p(seed=82, width=192, height=108, quality=25, supersample=2),
+     {}, 1024, 1024 * 37 + 5),
+    ("xaos_chunks", lambda: G.synthetic_8k(seed=83, width=128, height=72, quality=20, supersample=3),
+     {}, 512, 512 * 101),
+    ("earlyclip_rgba", lambda: G.synthetic_small(seed=84, width=96, height=80, quality=30),
+     dict(earlyclip=1, nchan=4, transp=1), 768, 768 * 19),
+]
+
+
+@pytest.mark.parametrize("name,mk,kw,walkers,chunk", BINNED_CASES, ids=[c[0] for c in BINNED_CASES])
+def test_binned_deposit_equals_atomic_deposit(prod, name, mk, kw, walkers, chunk):
+    """The tile-binned deposit (records, counting sort by tile, LDS tiles)
+    sums the same integer deltas as per-sample atomics, and chunked launches
+    continue each walker's trajectory, so with density estimation off (a
+    per-cell, order-free path to the image) the images are identical."""
+    g = mk()
+    g.estimator_radius = 0.0
+    a, sa = prod.render(g, seed="bin", nwalkers=walkers, deposit=1, **kw)
+    b, sb = prod.render(g, seed="bin", nwalkers=walkers, deposit=2, chunk_samples=chunk, **kw)
+    assert sa["num_iters"] == sb["num_iters"]
+    assert sa["badvals"] == sb["badvals"]
+    assert np.array_equal(a, b), np.abs(a.astype(int) - b.astype(int)).max()
+    assert a.mean() > 1.0
+
+
+def test_binned_deposit_with_density_estimation(prod):
+    g = G.synthetic_1080p(seed=85, width=192, height=108, quality=25, supersample=2)
+    a, _ = prod.render(g, seed="bin-de", nwalkers=1024, deposit=1)
+    b, _ = prod.render(g, seed="bin-de", nwalkers=1024, deposit=2, chunk_samples=1024 * 50)
+    d = np.abs(a.astype(np.int64) - b.astype(np.int64))
+    # DE sums floats in LDS in arrival order: one LSB at a rounding edge
+    assert d.max() <= 1 and (d > 0).mean() < 0.002
+
+
+def test_binned_deposit_linear_palette_close(prod):
+    """Linear palette: the record carries the blend fraction in 16-bit fixed
+    point, so a colour delta may differ by one before truncation."""
+    g = G.synthetic_small(seed=86, width=128, height=96, quality=30)
+    g.palette_mode = "linear"
+    g.estimator_radius = 0.0
+    a, _ = prod.render(g, seed="lin", nwalkers=1024, deposit=1)
+    b, _ = prod.render(g, seed="lin", nwalkers=1024, deposit=2, chunk_samples=1024 * 11)
+    d = np.abs(a.astype(np.int64) - b.astype(np.int64))
+    assert d.max() <= 2 and (d > 0).mean() < 0.01
+
+
+def test_binned_deposit_rejects_bits64(prod):
+    g = G.synthetic_small(seed=87, width=32, height=24, quality=2)
+    with pytest.raises(RuntimeError):
+        prod.render(g, seed="b64", bits=64, deposit=2)

@@ -185,7 +185,11 @@ def test_deposit_points(prod, oracle, bits, vis_mode, rotate, linear, scalar):
         np.testing.assert_allclose(got, want, rtol=1e-12, atol=0)
     else:
         assert np.array_equal(got[:, :, 4], want[:, :, 4])
-        assert np.array_equal(got, want)
+        diff = got.astype(np.int64) - want.astype(np.int64)
+        bad = np.argwhere(diff != 0)
+        assert len(bad) == 0, ("%d cells differ, max |d| %d, first %s got %s want %s"
+                               % (len(bad), np.abs(diff).max(), bad[:4].tolist(),
+                                  [int(got[tuple(b)]) for b in bad[:4]], [int(want[tuple(b)]) for b in bad[:4]]))
 
 
 def test_deposit_saturates_like_reference(prod, oracle):
