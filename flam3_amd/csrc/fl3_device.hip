@@ -24,6 +24,7 @@
 #include <string.h>
 #include <mutex>
 #include <type_traits>
+#include <vector>
 
 #include "fl3_pack.h"
 #include "fl3_core.h"
@@ -82,6 +83,7 @@ struct Ctx {
    hipStream_t stream = nullptr;
    DevBuf buckets, accum, job, table, image, isaac, points, misc, findex;
    DevBuf rec, sorted, wstate, bins;   // tile-binned deposit
+   DevBuf srcv;                        // density estimation per-source values
    unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
    int width = 0, height = 0, bits = 33;
 };
@@ -374,13 +376,21 @@ static int iterate_replay(const void *job, size_t job_bytes, uint32_t *isaac_hos
    return read_counters(badvals, nullptr);
 }
 
-static int upload_de(const fl3_de_filters *de, double **coefs, double **widths) {
+// DE tables on the device: coefficients (double), widths (double) and, for
+// the float accumulator, a float copy of the coefficients.
+static int upload_de(const fl3_de_filters *de, double **coefs, double **widths, float **coefs_f) {
    size_t nc = (size_t)de->nfilters * de->kernel_size, nw = de->nfilters;
-   if (g_ctx.table.ensure(sizeof(double) * (nc + nw))) return 1;
+   if (g_ctx.table.ensure(sizeof(double) * (nc + nw) + sizeof(float) * nc)) return 1;
    *coefs = (double *)g_ctx.table.p;
    *widths = *coefs + nc;
+   *coefs_f = (float *)(*widths + nw);
+   std::vector<float> cf(nc);
+   for (size_t i = 0; i < nc; i++) cf[i] = (float)de->filter_coefs[i];
    HIPCHK(hipMemcpyAsync(*coefs, de->filter_coefs, sizeof(double) * nc, hipMemcpyHostToDevice, g_ctx.stream));
    HIPCHK(hipMemcpyAsync(*widths, de->filter_widths, sizeof(double) * nw, hipMemcpyHostToDevice, g_ctx.stream));
+   HIPCHK(hipMemcpyAsync(*coefs_f, cf.data(), sizeof(float) * nc, hipMemcpyHostToDevice, g_ctx.stream));
+   // cf is a temporary: the copies finish before it goes
+   HIPCHK(hipStreamSynchronize(g_ctx.stream));
    return 0;
 }
 
@@ -389,11 +399,13 @@ extern "C" int fl3d_density(const fl3d_de_params *p, const fl3_de_filters *de) {
    DEArgs a;
    memset(&a, 0, sizeof(a));
    double *coefs, *widths;
-   if (upload_de(de, &coefs, &widths)) return 1;
+   float *coefs_f;
+   if (upload_de(de, &coefs, &widths, &coefs_f)) return 1;
    a.buckets = g_ctx.buckets.p;
    a.accum = g_ctx.accum.p;
    a.coefs = coefs;
    a.widths = widths;
+   a.coefs_t = g_ctx.bits == 64 ? (const void *)coefs : (const void *)coefs_f;
    a.width = g_ctx.width;
    a.height = g_ctx.height;
    a.oversample = p->oversample;
@@ -412,6 +424,8 @@ extern "C" int fl3d_density(const fl3d_de_params *p, const fl3_de_filters *de) {
    }
    if (g_ctx.findex.ensure(sizeof(uint16_t) * (size_t)g_ctx.width * g_ctx.height)) return 1;
    a.findex = (uint16_t *)g_ctx.findex.p;
+   if (g_ctx.srcv.ensure(accum_bytes(g_ctx.bits) * (size_t)g_ctx.width * g_ctx.height)) return 1;
+   a.srcv = g_ctx.srcv.p;
    const int pk = prof_begin(ST_DE);
    const int lrc = fl3k_launch_de_tiled(&a, g_ctx.stream, g_ctx.bits);
    prof_end(pk);

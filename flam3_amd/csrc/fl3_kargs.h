@@ -65,6 +65,8 @@ struct DEArgs {
    const double *coefs;
    const double *widths;
    uint16_t *findex;        // per cell filter index, 0xffff = no contribution
+   void *srcv;              // per cell AT[4]: colour x k1 log(1 + c3 k2) / c3
+   const void *coefs_t;     // the kernel coefficients in the accumulator type
    int width, height, oversample;
    int max_filtered_counts, max_filter_index, kernel_size;
    int half;                // largest kernel radius (cells)

@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void k_deposit_points(const unsigned char *job
 #define DE_TW 64
 #define DE_TH 64
 
-template <typename BT>
+template <typename BT, typename AT>
 __global__ __launch_bounds__(256) void k_de_prep(DEArgs a) {
    const int os = a.oversample;
    const int ss = os / 2;
@@ -199,53 +199,98 @@ __global__ __launch_bounds__(256) void k_de_prep(DEArgs a) {
             fi = 100 + (int)floor(pow(fsel - 100, a.curve));
          if (fi > a.max_filter_index) fi = a.max_filter_index;
          out = (uint16_t)fi;
+         // the per-source factor of rect.c:137, k1 log(1 + c3 k2) / c3,
+         // applied to the colour once; the kernel coefficient multiplies
+         // it per tap
+         const double sc = (a.k1 * log(1.0 + c3 * a.k2)) / c3;
+         AT *v = (AT *)a.srcv + cell * 4;
+         v[0] = (AT)((double)b[cell * 5 + 0] * sc);
+         v[1] = (AT)((double)b[cell * 5 + 1] * sc);
+         v[2] = (AT)((double)b[cell * 5 + 2] * sc);
+         v[3] = (AT)(c3 * sc);
       }
    }
    a.findex[cell] = out;
 }
 
-template <typename BT, typename AT>
+// Add the part of source (sx, sy)'s kernel that falls in the rectangle
+// [x0, x1] x [y0, y1] of the tile, positions p0, p0 + step, ...  The kernel
+// is radial: the coefficient of offset (dx, dy) is entry
+// hi (hi + 1) / 2 + lo of the source's row, hi = max(|dx|, |dy|),
+// lo = min(|dx|, |dy|) (the triangle walk of rect.c:123-170).
+template <typename AT>
+__device__ __forceinline__ void de_splat(AT *tile, const AT *crow, const AT *v, int sx, int sy, int x0, int x1,
+                                         int y0, int y1, int tx0, int ty0, int p0, int step) {
+   const int w = x1 - x0 + 1, n = w * (y1 - y0 + 1);
+   const AT v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3];
+   for (int p = p0; p < n; p += step) {
+      const int y = y0 + p / w, x = x0 + p % w;
+      const int dx = abs(x - sx), dy = abs(y - sy);
+      const int hi = max(dx, dy), lo = min(dx, dy);
+      const AT coef = crow[hi * (hi + 1) / 2 + lo];
+      if (coef == (AT)0) continue;
+      AT *t = tile + ((y - ty0) * DE_TW + (x - tx0)) * 4;
+      atomicAdd(t + 0, coef * v0);
+      atomicAdd(t + 1, coef * v1);
+      atomicAdd(t + 2, coef * v2);
+      atomicAdd(t + 3, coef * v3);
+   }
+}
+
+// One workgroup per 64x64 output tile, accumulated in LDS.  Each wave takes
+// 64 consecutive sources of the tile's halo; a source whose kernel covers at
+// most DE_LANE_TAPS cells of the tile is splatted by its own lane, larger
+// ones are splatted by the whole wave, one after another, so lanes do not
+// idle behind the widest kernel of their wave.
+#define DE_LANE_TAPS 24
+template <typename AT>
 __global__ __launch_bounds__(256) void k_de_tile(DEArgs a) {
    __shared__ AT tile[DE_TH * DE_TW * 4];
    __shared__ int radius[1024];
    const int W = a.width, H = a.height, R = a.half;
    const int tx0 = blockIdx.x * DE_TW, ty0 = blockIdx.y * DE_TH;
+   const int txe = min(tx0 + DE_TW, W) - 1, tye = min(ty0 + DE_TH, H) - 1;
    for (int e = threadIdx.x; e < DE_TH * DE_TW * 4; e += blockDim.x) tile[e] = (AT)0;
    for (int e = threadIdx.x; e <= a.max_filter_index && e < 1024; e += blockDim.x)
       radius[e] = (int)ceil(a.widths[e]) - 1;
    __syncthreads();
-   const BT *b = (const BT *)a.buckets;
+   const AT *coefs = (const AT *)a.coefs_t;
+   const AT *srcv = (const AT *)a.srcv;
    const int sx0 = max(0, tx0 - R), sx1 = min(W, tx0 + DE_TW + R);
    const int sy0 = max(0, ty0 - R), sy1 = min(H, ty0 + DE_TH + R);
    const int sw = sx1 - sx0, nsrc = sw * (sy1 - sy0);
-   for (int s = threadIdx.x; s < nsrc; s += blockDim.x) {
-      const int sx = sx0 + s % sw, sy = sy0 + s / sw;
-      const long cell = sx + (long)sy * W;
-      const unsigned fi = a.findex[cell];
-      if (fi == DE_SKIP) continue;
-      const int r = radius[fi];
-      const int x0 = max(sx - r, tx0), x1 = min(sx + r, min(tx0 + DE_TW, W) - 1);
-      const int y0 = max(sy - r, ty0), y1 = min(sy + r, min(ty0 + DE_TH, H) - 1);
-      if (x0 > x1 || y0 > y1) continue;
-      const double c0 = (double)b[cell * 5 + 0], c1 = (double)b[cell * 5 + 1];
-      const double c2 = (double)b[cell * 5 + 2], c3 = (double)b[cell * 5 + 3];
-      const double lsb = a.k1 * log(1.0 + c3 * a.k2);
-      const double *coefs = a.coefs + (long)fi * a.kernel_size;
-      for (int y = y0; y <= y1; y++) {
-         const int dy = abs(y - sy);
-         for (int x = x0; x <= x1; x++) {
-            const int dx = abs(x - sx);
-            const int hi = max(dx, dy), lo = min(dx, dy);
-            if (hi > r) continue;
-            const double coef = coefs[hi * (hi + 1) / 2 + lo];
-            if (coef == 0.0) continue;
-            const double ls = coef * lsb / c3;
-            AT *t = tile + ((y - ty0) * DE_TW + (x - tx0)) * 4;
-            atomicAdd(t + 0, (AT)(c0 * ls));
-            atomicAdd(t + 1, (AT)(c1 * ls));
-            atomicAdd(t + 2, (AT)(c2 * ls));
-            atomicAdd(t + 3, (AT)(c3 * ls));
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   for (int base = wave * 64; base < nsrc; base += blockDim.x) {
+      const int s = base + lane;
+      int sx = 0, sy = 0, fi = 0, x0 = 1, x1 = 0, y0 = 1, y1 = 0;
+      long cell = 0;
+      if (s < nsrc) {
+         sx = sx0 + s % sw;
+         sy = sy0 + s / sw;
+         cell = sx + (long)sy * W;
+         const unsigned f = a.findex[cell];
+         if (f != DE_SKIP) {
+            fi = (int)f;
+            const int r = radius[fi];
+            x0 = max(sx - r, tx0); x1 = min(sx + r, txe);
+            y0 = max(sy - r, ty0); y1 = min(sy + r, tye);
          }
+      }
+      const int taps = (x0 <= x1 && y0 <= y1) ? (x1 - x0 + 1) * (y1 - y0 + 1) : 0;
+      if (taps > 0 && taps <= DE_LANE_TAPS)
+         de_splat<AT>(tile, coefs + (long)fi * a.kernel_size, srcv + cell * 4, sx, sy, x0, x1, y0, y1, tx0, ty0, 0,
+                      1);
+      unsigned long long big = __ballot(taps > DE_LANE_TAPS);
+      while (big) {
+         const int l = __builtin_ctzll(big);
+         big &= big - 1;
+         const int bsx = __builtin_amdgcn_readlane(sx, l), bsy = __builtin_amdgcn_readlane(sy, l);
+         const int bfi = __builtin_amdgcn_readlane(fi, l);
+         const int bx0 = __builtin_amdgcn_readlane(x0, l), bx1 = __builtin_amdgcn_readlane(x1, l);
+         const int by0 = __builtin_amdgcn_readlane(y0, l), by1 = __builtin_amdgcn_readlane(y1, l);
+         const long bcell = bsx + (long)bsy * W;
+         de_splat<AT>(tile, coefs + (long)bfi * a.kernel_size, srcv + bcell * 4, bsx, bsy, bx0, bx1, by0, by1, tx0,
+                      ty0, lane, 64);
       }
    }
    __syncthreads();
@@ -401,11 +446,11 @@ extern "C" int fl3k_launch_de_tiled(const DEArgs *a, hipStream_t s, int bits) {
    const dim3 g1((a->width + 255) / 256, a->height), b1(256);
    const dim3 g2((a->width + DE_TW - 1) / DE_TW, (a->height + DE_TH - 1) / DE_TH), b2(256);
    if (bits == 64) {
-      hipLaunchKernelGGL((k_de_prep<double>), g1, b1, 0, s, *a);
-      hipLaunchKernelGGL((k_de_tile<double, double>), g2, b2, 0, s, *a);
+      hipLaunchKernelGGL((k_de_prep<double, double>), g1, b1, 0, s, *a);
+      hipLaunchKernelGGL((k_de_tile<double>), g2, b2, 0, s, *a);
    } else {
-      hipLaunchKernelGGL((k_de_prep<uint32_t>), g1, b1, 0, s, *a);
-      hipLaunchKernelGGL((k_de_tile<uint32_t, float>), g2, b2, 0, s, *a);
+      hipLaunchKernelGGL((k_de_prep<uint32_t, float>), g1, b1, 0, s, *a);
+      hipLaunchKernelGGL((k_de_tile<float>), g2, b2, 0, s, *a);
    }
    return hipGetLastError() != hipSuccess;
 }

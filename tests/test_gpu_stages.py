@@ -160,6 +160,12 @@ def _points(rng, n, vis_mode):
     pts[:, 0] = rng.uniform(-2.3, 2.3, n)
     pts[:, 1] = rng.uniform(-1.8, 1.8, n)
     pts[:, 2] = rng.uniform(-0.05, 1.05, n)
+    # A colour in (-1/256, 0) gives the linear palette a negative blend
+    # fraction (rect.c:469-481 truncates the index toward zero), hence a
+    # possibly negative delta, whose conversion to unsigned is undefined in
+    # C; those points are covered by test_deposit_negative_linear_fraction.
+    win = (pts[:, 2] > -1.0 / 256) & (pts[:, 2] < 0.0)
+    pts[win, 2] = -pts[win, 2]
     pts[:, 3] = 1.0 if vis_mode == "one" else rng.choice([0.0, 0.25, 0.7, 1.0], n)
     return pts
 
@@ -190,6 +196,32 @@ def test_deposit_points(prod, oracle, bits, vis_mode, rotate, linear, scalar):
         assert len(bad) == 0, ("%d cells differ, max |d| %d, first %s got %s want %s"
                                % (len(bad), np.abs(diff).max(), bad[:4].tolist(),
                                   [int(got[tuple(b)]) for b in bad[:4]], [int(want[tuple(b)]) for b in bad[:4]]))
+
+
+def test_deposit_negative_linear_fraction(prod, oracle):
+    """Linear palette, colour in (-1/256, 0): the blend fraction is negative
+    and a colour delta can be negative.  The reference converts it to
+    unsigned (undefined in C; on x86 the threaded deposit then saturates the
+    channel).  The GPU adds zero for a negative delta and counts the hit."""
+    rng = np.random.default_rng(17)
+    cam = _camera(rng, linear=True)
+    n = 50000
+    pts = np.zeros((n, 4))
+    pts[:, 0] = rng.uniform(-2.3, 2.3, n)
+    pts[:, 1] = rng.uniform(-1.8, 1.8, n)
+    pts[:, 2] = rng.uniform(-1.0 / 256, 0.0, n) * 0.999
+    pts[:, 3] = 1.0
+    want = np.zeros((48, 64, 5), dtype=np.uint32)
+    oracle.lib().or_deposit(ctypes.byref(cam), pts.ctypes.data_as(ctypes.c_void_p), n, 33, 0,
+                            want.ctypes.data_as(ctypes.c_void_p))
+    got = np.zeros_like(want)
+    assert prod.lib().fl3h_deposit_points(ctypes.byref(cam), pts.ctypes.data_as(ctypes.c_void_p), n, 33,
+                                          got.ctypes.data_as(ctypes.c_void_p)) == 0
+    assert np.array_equal(got[:, :, 4], want[:, :, 4])
+    # |fraction| < 1 keeps a blend below two palette values (2 x 255) per
+    # hit: nothing wrapped or saturated
+    assert (got[:, :, :4].astype(np.int64) <= 2 * got[:, :, 4:5].astype(np.int64)).all()
+    assert got[:, :, :4].sum() > 0
 
 
 def test_deposit_saturates_like_reference(prod, oracle):
