@@ -281,7 +281,33 @@ __device__ __forceinline__ int select_xform(const GenL<R> &G, int row, uint32_t 
 
 // One application of xform fn to p (variations.c:2129-2399).  Returns true
 // when the result is a bad value; q[0..1] then hold a random point.
-template <typename R, class RNG>
+// Variation tiers of the walker kernels.  Tier 0 holds the classic ids
+// 0-13 (linear .. julia); tier t in 1..FL3_NBLOCKS adds block t-1 of the
+// remaining ids; FL3_TIER_FULL holds all of them.
+#define FL3_CLASSIC 14
+#define FL3_BLOCK 17
+#define FL3_NBLOCKS 5
+#define FL3_TIER_FULL (FL3_NBLOCKS + 1)
+__host__ __device__ constexpr int fl3_tier_lo(int t) { return FL3_CLASSIC + (t - 1) * FL3_BLOCK; }
+__host__ __device__ constexpr int fl3_tier_hi(int t) {
+   return FL3_CLASSIC + t * FL3_BLOCK < FL3_NVAR ? FL3_CLASSIC + t * FL3_BLOCK : FL3_NVAR;
+}
+
+template <typename R, class RNG, int TIER>
+__device__ __forceinline__ void fl3_var_tier(int v, R w, const XfL<R> &X, Pre<R> &f, RNG &rng) {
+   if constexpr (TIER == FL3_TIER_FULL) {
+      fl3_var<R, RNG>(v, w, X, f, rng);
+   } else if constexpr (TIER == 0) {
+      fl3_var<R, RNG, 0, FL3_CLASSIC>(v, w, X, f, rng);
+   } else {
+      if (v < FL3_CLASSIC)
+         fl3_var<R, RNG, 0, FL3_CLASSIC>(v, w, X, f, rng);
+      else
+         fl3_var<R, RNG, fl3_tier_lo(TIER), fl3_tier_hi(TIER)>(v, w, X, f, rng);
+   }
+}
+
+template <typename R, class RNG, int TIER = FL3_TIER_FULL>
 __device__ bool apply_xform(const GenL<R> &G, int fn, const R *p, R *q, RNG &rng) {
    const XfL<R> &X = G.xf[fn];
    Pre<R> f;
@@ -320,7 +346,7 @@ __device__ bool apply_xform(const GenL<R> &G, int fn, const R *p, R *q, RNG &rng
       const R w = wrow[u];
       uint32_t o = xoff;
       asm volatile("" : "+v"(o));
-      fl3_var<R, RNG>(v, w, *(const XfL<R> *)(fl3_lds + o), f, rng);
+      fl3_var_tier<R, RNG, TIER>(v, w, *(const XfL<R> *)(fl3_lds + o), f, rng);
    }
    if (flags & FL3_HAS_POST) {
       q[0] = X.post[0] * f.p0 + X.post[2] * f.p1 + X.post[4];

@@ -82,7 +82,7 @@ struct Ctx {
    size_t lds_max = 65536;   // LDS bytes one workgroup may allocate
    hipStream_t stream = nullptr;
    DevBuf buckets, accum, job, table, image, isaac, points, misc, findex;
-   DevBuf rec, sorted, wstate, bins;   // tile-binned deposit
+   DevBuf rec, sorted, wstate, bins, blkhist;   // tile-binned deposit
    DevBuf srcv;                        // density estimation per-source values
    unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
    int width = 0, height = 0, bits = 33;
@@ -230,6 +230,22 @@ static long long choose_walkers(long long nsamples, int requested) {
    return (w + 255) / 256 * 256;
 }
 
+// Smallest walker-kernel tier holding every variation the genome uses.
+static int pick_tier(const fl3_job_hdr *h) {
+   int lo = FL3_NVAR, hi = -1;
+   for (int u = 0; u < h->nunion; u++) {
+      const int v = h->union_ids[u];
+      if (v >= FL3_CLASSIC) {
+         lo = v < lo ? v : lo;
+         hi = v > hi ? v : hi;
+      }
+   }
+   if (hi < 0) return 0;
+   for (int t = 1; t <= FL3_NBLOCKS; t++)
+      if (lo >= fl3_tier_lo(t) && hi < fl3_tier_hi(t)) return t;
+   return FL3_TIER_FULL;
+}
+
 // Tile shift of the binned deposit for the current histogram (cells per
 // tile = 1 << shift), or 0 when the histogram is too large for it.
 static int bin_shift() {
@@ -239,12 +255,25 @@ static int bin_shift() {
    return 0;
 }
 
+// 4-byte records (cell | palette index << 24) suffice when every plotted
+// point has visibility 1 and the palette is stepped: the deposit is then a
+// function of the cell and the palette index alone.
+static bool narrow_records_ok(const fl3_job_hdr *h, long long ncells) {
+   if (h->palette_mode != 0 || ncells >= (1LL << 24) - 1) return false;
+   const fl3_xform_pack *px = (const fl3_xform_pack *)((const unsigned char *)h + h->off_xf);
+   for (int e = 0; e < h->nstd; e++)
+      if (px[e].vis != 1.0) return false;
+   return true;
+}
+
 // Binned deposit: the walkers write records in chunks of `per` rows (the
 // walker state carries over between chunks, so the trajectories are those of
 // a single launch); each chunk is tile-sorted and accumulated.
-static int iterate_binned(IterArgs a, long long W, size_t lds, const fl3d_iter_params *p, int shift) {
+static int iterate_binned(IterArgs a, long long W, size_t lds, const fl3_job_hdr *h, const fl3d_iter_params *p,
+                          int shift, int tier) {
    const long long ncells = (long long)g_ctx.width * g_ctx.height;
    const int nbins = (int)((ncells + (1LL << shift) - 1) >> shift);
+   const bool narrow = narrow_records_ok(h, ncells);
    long long per = p->chunk > 0 ? (p->chunk + W - 1) / W : (FL3_BIN_CHUNK_SAMPLES / W);
    if (per < 1) per = 1;
    const long long rows_total = (p->nsamples + W - 1) / W;
@@ -253,46 +282,65 @@ static int iterate_binned(IterArgs a, long long W, size_t lds, const fl3d_iter_p
       set_err("binned deposit chunk exceeds 2^32 records");
       return 1;
    }
-   const size_t rec_bytes = sizeof(unsigned long long) * (size_t)W * per;
+   const long long nblk = W / 256;
+   const size_t rec_bytes = (narrow ? 4 : 8) * (size_t)W * per;
    if (g_ctx.rec.ensure(rec_bytes) || g_ctx.sorted.ensure(rec_bytes)) return 1;
    if (g_ctx.wstate.ensure(sizeof(uint32_t) * 10 * (size_t)W)) return 1;
-   const size_t bin_words = 2 * (size_t)nbins + 2 * ((size_t)nbins + 1) + 4;
+   if (g_ctx.blkhist.ensure(sizeof(unsigned) * (size_t)nblk * nbins)) return 1;
+   const size_t bin_words = 3 * (size_t)nbins + 8;
    if (g_ctx.bins.ensure(sizeof(unsigned) * bin_words)) return 1;
    unsigned *bw = (unsigned *)g_ctx.bins.p;
    BinArgs b;
    memset(&b, 0, sizeof(b));
    b.job = a.job;
    b.buckets = a.buckets;
-   b.rec = (const unsigned long long *)g_ctx.rec.p;
-   b.sorted = (unsigned long long *)g_ctx.sorted.p;
+   b.rec = g_ctx.rec.p;
+   b.sorted = g_ctx.sorted.p;
+   b.blk_hist = (unsigned *)g_ctx.blkhist.p;
+   b.nblk = (int)nblk;
+   b.walkers = W;
    b.bin_total = bw;
-   b.bin_fill = bw + nbins;
-   b.bin_start = bw + 2 * nbins;
-   b.item_start = bw + 3 * nbins + 1;
-   b.item_counter = bw + 4 * nbins + 2;
+   b.bin_start = bw + nbins;
+   b.item_start = bw + 2 * nbins + 1;
+   b.item_counter = bw + 3 * nbins + 2;
    b.shift = shift;
    b.nbins = nbins;
-   b.chunk = FL3_BIN_ITEM_RECORDS;
+   // a work item's per-cell sums must fit 32 bits (and the packed pairs must
+   // not carry): at most 2^20 records, fewer if a palette entry exceeds 255
+   {
+      const double *dm = (const double *)((const unsigned char *)h + h->off_dmap);
+      double maxd = 255.0;
+      for (int e = 0; e < 1024; e++) maxd = dm[e] > maxd ? dm[e] : maxd;
+      const double lim = 4294967295.0 / (maxd + 1.0);
+      b.chunk = lim < (double)FL3_BIN_ITEM_RECORDS ? (int)lim : FL3_BIN_ITEM_RECORDS;
+      if (b.chunk < 1) b.chunk = 1;
+   }
    b.width = g_ctx.width;
    b.height = g_ctx.height;
+   a.rec_mode = narrow ? 2 : 1;
    a.rec = (unsigned long long *)g_ctx.rec.p;
    a.wstate = (uint32_t *)g_ctx.wstate.p;
+   a.blk_hist = (unsigned *)g_ctx.blkhist.p;
+   a.bin_shift = shift;
+   a.nbins = nbins;
+   a.hist_off = (unsigned)((lds + 15) & ~(size_t)15);
+   const size_t lds_rec = a.hist_off + sizeof(unsigned) * (size_t)nbins;
    for (long long done = 0; done < p->nsamples;) {
       const long long c = p->nsamples - done < W * per ? p->nsamples - done : W * per;
       a.nsamples = c;
       a.rows_out = (c + W - 1) / W;
       a.resume = done > 0;
       int pk = prof_begin(ST_ITER);
-      int lrc = fl3k_launch_iter_fast(&a, (unsigned)(W / 256), lds, g_ctx.stream, g_ctx.bits, 1);
+      int lrc = fl3k_launch_iter_fast(&a, (unsigned)nblk, lds_rec, g_ctx.stream, g_ctx.bits, 1, tier);
       prof_end(pk);
       if (lrc) {
          set_err("k_iter_fast launch failed");
          return 1;
       }
-      HIPCHK(hipMemsetAsync(bw, 0, sizeof(unsigned) * bin_words, g_ctx.stream));
-      b.n = W * a.rows_out;
+      HIPCHK(hipMemsetAsync(b.item_counter, 0, sizeof(unsigned), g_ctx.stream));
+      b.rows = a.rows_out;
       pk = prof_begin(ST_BIN);
-      lrc = fl3k_launch_bin(&b, g_ctx.ncu, g_ctx.stream);
+      lrc = fl3k_launch_bin(&b, g_ctx.ncu, narrow ? 1 : 0, g_ctx.stream);
       prof_end(pk);
       if (lrc) {
          set_err("binned deposit launch failed");
@@ -323,6 +371,7 @@ extern "C" int fl3d_iterate(const void *job, size_t job_bytes, const fl3d_iter_p
       const long long W = choose_walkers(p->nsamples, p->nwalkers);
       const size_t lds = lds_bytes_for<float>(h);
       const int shift = bin_shift();
+      const int tier = pick_tier(h);
       const bool binnable = g_ctx.bits != 64 && h->nxf <= 256 && shift > 0;
       if (p->deposit == 2 && !binnable) {
          set_err("tile-binned deposit needs bits 32/33, at most 256 xforms and at most %s%ld tiles", "",
@@ -332,10 +381,11 @@ extern "C" int fl3d_iterate(const void *job, size_t job_bytes, const fl3d_iter_p
       const bool binned = binnable && (p->deposit == 2 || (p->deposit == 0 && p->nsamples >= FL3_BIN_MIN_SAMPLES));
       g_prof.samples += p->nsamples;
       if (binned) {
-         if (iterate_binned(a, W, lds, p, shift)) return 1;
+         if (iterate_binned(a, W, lds, h, p, shift, tier)) return 1;
       } else {
          const int pk = prof_begin(ST_ITER);
-         const int lrc = fl3k_launch_iter_fast(&a, (unsigned)(W / 256), lds, g_ctx.stream, g_ctx.bits, p->exact);
+         const int lrc =
+             fl3k_launch_iter_fast(&a, (unsigned)(W / 256), lds, g_ctx.stream, g_ctx.bits, p->exact, tier);
          prof_end(pk);
          if (lrc) {
             set_err("k_iter_fast launch failed");
@@ -643,6 +693,49 @@ extern "C" int fl3h_xform_preview(flam3_genome *cp, int xi, double range, int nu
       if (hipMemcpyAsync(st, g_ctx.isaac.p, sizeof(st), hipMemcpyDeviceToHost, g_ctx.stream) != hipSuccess) { set_err("device copy failed"); break; }
       if (hipStreamSynchronize(g_ctx.stream) != hipSuccess) { set_err("k_xform_preview failed"); break; }
       unpack_isaac(st, rc);
+      rcode = 0;
+   } while (0);
+   free(job);
+   return rcode;
+}
+
+extern "C" int fl3h_apply_xform_fp32(flam3_genome *cp, int xi, const double *points, long long npoints,
+                                     double *out, int tier) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   if (ctx_init()) return 1;
+   if (xi < 0 || xi >= cp->num_xforms || npoints < 0 || tier < -1 || tier > FL3_TIER_FULL) {
+      set_err("fl3h_apply_xform_fp32: bad arguments");
+      return 1;
+   }
+   const double oldweight = cp->xform[xi].density;
+   cp->xform[xi].density = 1.0;
+   if (fl3_prepare_precalc_flags(cp)) {
+      cp->xform[xi].density = oldweight;
+      set_err("fl3h_apply_xform_fp32: precalc failed");
+      return 1;
+   }
+   fl3_xform_precalc(&cp->xform[xi]);
+   size_t jb = 0;
+   void *job = fl3_pack_job(cp, NULL, NULL, &jb);
+   cp->xform[xi].density = oldweight;
+   if (!job) return 1;
+   const fl3_job_hdr *h = (const fl3_job_hdr *)job;
+   if (tier < 0) tier = pick_tier(h);
+   const size_t pb = sizeof(double) * 4 * (size_t)npoints;
+   int rcode = 1;
+   do {
+      if (npoints == 0) { rcode = 0; break; }
+      if (upload_job(job, jb)) break;
+      if (g_ctx.points.ensure(2 * pb)) break;
+      double *din = (double *)g_ctx.points.p, *dout = din + 4 * npoints;
+      if (hipMemcpyAsync(din, points, pb, hipMemcpyHostToDevice, g_ctx.stream) != hipSuccess) { set_err("device copy failed"); break; }
+      if (fl3k_launch_apply_fp32((const unsigned char *)g_ctx.job.p, xi, din, npoints, dout, 0x5eedULL,
+                                 lds_bytes_for<float>(h), g_ctx.stream, tier)) {
+         set_err("k_apply_fp32 launch failed");
+         break;
+      }
+      if (hipMemcpyAsync(out, dout, pb, hipMemcpyDeviceToHost, g_ctx.stream) != hipSuccess) { set_err("device copy failed"); break; }
+      if (hipStreamSynchronize(g_ctx.stream) != hipSuccess) { set_err("k_apply_fp32 failed"); break; }
       rcode = 0;
    } while (0);
    free(job);

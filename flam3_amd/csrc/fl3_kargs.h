@@ -19,8 +19,12 @@ struct IterArgs {
    int nwalkers;
    // binned deposit: one 8-byte record per plotted sample, at i*W + walker;
    // rows_out rows are written (walkers with fewer samples pad with no-ops)
-   unsigned long long *rec;
+   int rec_mode;              // 0 deposit with atomics, 1 8-byte records, 2 4-byte records
+   unsigned long long *rec;   // 8-byte records, or 4-byte ones in the same buffer
    long long rows_out;
+   unsigned *blk_hist;        // [workgroup][tile] record counts
+   int bin_shift, nbins;
+   unsigned hist_off;         // LDS byte offset of the workgroup's tile counts
    // walker state kept between launches of one batch (10 words per walker,
    // structure of arrays): resume = continue from it without fuse steps
    uint32_t *wstate;
@@ -40,10 +44,13 @@ struct IterArgs {
 struct BinArgs {
    const unsigned char *job;       // for palette / visibility tables
    void *buckets;
-   const unsigned long long *rec;  // records in walker order
-   unsigned long long *sorted;     // records grouped by tile
-   long long n;                    // records
-   unsigned *bin_total, *bin_fill, *bin_start, *item_start, *item_counter;
+   const void *rec;                // records in walker order (4 or 8 bytes)
+   void *sorted;                   // records grouped by tile
+   unsigned *blk_hist;             // [walker workgroup][tile] counts
+   int nblk;                       // walker workgroups (256 walkers each)
+   long long walkers;              // row stride of rec
+   long long rows;                 // record rows in this chunk
+   unsigned *bin_total, *bin_start, *item_start, *item_counter;
    int shift;                      // cells per tile = 1 << shift
    int nbins;                      // tiles
    int chunk;                      // records per accumulation work item

@@ -180,7 +180,10 @@ __global__ __launch_bounds__(256) void k_de_prep(DEArgs a) {
    const long cell = i + (long)j * W;
    uint16_t out = DE_SKIP;
    const BT *b = (const BT *)a.buckets;
-   if (i >= os - 1 && i < W - (os - 1) && j >= os - 1) {
+   // rows (os - 1) .. H - os + 1 (rect.c:979-1000 spans H - 2 (os - 1) + 1
+   // rows from os - 1); for os 1 that includes row H, past the histogram,
+   // which counts as empty here
+   if (i >= os - 1 && i < W - (os - 1) && j >= os - 1 && j <= H - os + 1) {
       const double c4 = (double)b[cell * 5 + 4], c3 = (double)b[cell * 5 + 3];
       if (c4 != 0 && c3 != 0) {
          double fsel = 0.0;

@@ -1,120 +1,58 @@
 // fl3_kernels_fast.hip - FP32 production walkers (compiled with FP
 // contraction on).
-#include "fl3_core.h"
+#include "fl3_walker.h"
 #include "fl3_launch.h"
 
 
-// Production walkers.  Walker w plots nsamples/W (+1 for the first
-// nsamples%W walkers) points after `fuse` unplotted steps from a random
-// start, exactly the per-sub-batch procedure of rect.c:393-400 with one long
-// sub-batch per walker.
-template <int BITS, bool EXACT, bool RECORDS>
-__global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
-   unsigned char *lds = fl3_lds;
-   GenL<float> G;
-   load_genome<float>(a.job, lds, G);
-   __syncthreads();
-   const long long W = (long long)gridDim.x * blockDim.x;
-   const long long wid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-   const long long n = a.nsamples / W + (wid < a.nsamples % W ? 1 : 0);
-   WalkerRng rng;
-   float p[4], q[4];
-   int lastxf = 0, consec = 0;
-   uint32_t *ws = a.wstate;
-   if (a.resume) {
-      for (int k = 0; k < 4; k++) p[k] = __uint_as_float(ws[k * W + wid]);
-      rng.s0 = ws[4 * W + wid];
-      rng.s1 = ws[5 * W + wid];
-      rng.s2 = ws[6 * W + wid];
-      rng.s3 = ws[7 * W + wid];
-      lastxf = (int)ws[8 * W + wid];
-      consec = (int)ws[9 * W + wid];
-   } else {
-      rng.seed(a.key0, a.key1, (uint64_t)wid);
-      p[0] = u11<float>(rng);
-      p[1] = u11<float>(rng);
-      p[2] = u01<float>(rng);
-      p[3] = u01<float>(rng);
-   }
-   uint32_t bad = 0;
-   const bool use_final = G.final_index >= 0;
-   for (long long i = a.resume ? 0 : -a.fuse; i < n;) {
-      const uint32_t r = rng.next();
-      const int fn = select_xform(G, G.chaos_enable ? lastxf : 0, r & 16383u);
-      if (apply_xform<float>(G, fn, p, q, rng)) {
-         consec++;
-         bad++;
-         if (consec < 5) {
-            p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
-            continue;
-         }
-         consec = 0;
-      } else
-         consec = 0;
-      lastxf = fn + 1;
-      p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
-      if (use_final) {
-         if (G.final_opacity == 1.0f || u01<float>(rng) < G.final_opacity) {
-            apply_xform<float>(G, G.final_index, p, q, rng);
-            q[3] = p[3];
-         }
-      }
-      if (i >= 0) {
-         if (RECORDS)
-            a.rec[i * W + wid] = make_record<float>(G, q, fn);
-         else
-            deposit<float, BITS, EXACT>(G, q, a.buckets, a.sat);
-      }
-      i++;
-   }
-   if (RECORDS)
-      for (long long i = n; i < a.rows_out; i++) a.rec[i * W + wid] = FL3_REC_NONE;
-   if (ws) {
-      for (int k = 0; k < 4; k++) ws[k * W + wid] = __float_as_uint(p[k]);
-      ws[4 * W + wid] = rng.s0;
-      ws[5 * W + wid] = rng.s1;
-      ws[6 * W + wid] = rng.s2;
-      ws[7 * W + wid] = rng.s3;
-      ws[8 * W + wid] = (uint32_t)lastxf;
-      ws[9 * W + wid] = (uint32_t)consec;
-   }
-   add_bad(a.badvals, bad);
+extern "C" int fl3k_launch_iter_fast(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int bits,
+                                     int exact, int tier) {
+   if (tier >= 0 && tier < FL3_TIER_FULL && (a->rec_mode || (bits != 64 && !exact)))
+      return fl3k_launch_iter_tier(a, grid, lds, s, tier);
+   if (a->rec_mode == FL3_REC_NARROW)
+      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_NARROW, FL3_TIER_FULL>), dim3(grid), dim3(256), lds, s, *a);
+   else if (a->rec_mode == FL3_REC_WIDE)
+      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_WIDE, FL3_TIER_FULL>), dim3(grid), dim3(256), lds, s, *a);
+   else if (bits == 64)
+      hipLaunchKernelGGL((k_iter_fast<64, true, FL3_REC_DIRECT, FL3_TIER_FULL>), dim3(grid), dim3(256), lds, s, *a);
+   else if (exact)
+      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_DIRECT, FL3_TIER_FULL>), dim3(grid), dim3(256), lds, s, *a);
+   else
+      hipLaunchKernelGGL((k_iter_fast<33, false, FL3_REC_DIRECT, FL3_TIER_FULL>), dim3(grid), dim3(256), lds, s,
+                         *a);
+   return hipGetLastError() != hipSuccess;
 }
 
-extern "C" int fl3k_launch_iter_fast(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int bits,
-                                     int exact) {
-   if (a->rec)
-      hipLaunchKernelGGL((k_iter_fast<33, true, true>), dim3(grid), dim3(256), lds, s, *a);
-   else if (bits == 64)
-      hipLaunchKernelGGL((k_iter_fast<64, true, false>), dim3(grid), dim3(256), lds, s, *a);
-   else if (exact)
-      hipLaunchKernelGGL((k_iter_fast<33, true, false>), dim3(grid), dim3(256), lds, s, *a);
-   else
-      hipLaunchKernelGGL((k_iter_fast<33, false, false>), dim3(grid), dim3(256), lds, s, *a);
+extern "C" int fl3k_launch_apply_fp32(const unsigned char *job, int xi, const double *pts, long long n, double *out,
+                                      unsigned long long key, size_t lds, hipStream_t s, int tier) {
+   if (tier >= 0 && tier < FL3_TIER_FULL) return fl3k_launch_apply_fp32_tier(job, xi, pts, n, out, key, lds, s, tier);
+   const unsigned grid = (unsigned)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096);
+   hipLaunchKernelGGL(k_apply_fp32<FL3_TIER_FULL>, dim3(grid), dim3(256), lds, s, job, xi, pts, n, out, key);
    return hipGetLastError() != hipSuccess;
 }
 
 // ------------------------------------------------------------------ binned deposit
 //
-// The walker kernel above writes one record per plotted sample instead of
-// five global atomics.  The records are then grouped by tile of 1 << shift
-// cells with a counting sort (count, scan, scatter), and each tile's records
-// are accumulated in LDS and added to the histogram once.  Integer sums
-// commute, so the histogram is the one the atomic deposit would produce;
-// the tile flush applies the reference's saturation exactly.
+// The walker kernel writes one record per plotted sample instead of five
+// global atomics, and counts its workgroup's records per tile of 1 << shift
+// cells.  The records are then grouped by tile (a counting sort: per-tile
+// sums and per-workgroup offsets, tile offsets, scatter), and each tile's
+// records are accumulated in LDS and added to the histogram once.  Integer
+// sums commute, so the histogram is the one the atomic deposit produces; the
+// tile flush applies the reference's saturation exactly.
 
-__global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
-   unsigned *hist = (unsigned *)fl3_lds;
-   for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) hist[b] = 0;
-   __syncthreads();
-   for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < a.n;
-        r += (long long)gridDim.x * blockDim.x) {
-      const unsigned c = (unsigned)a.rec[r];
-      if (c != FL3_REC_NONE) atomicAdd(&hist[c >> a.shift], 1u);
+// One thread per tile: total over walker workgroups, and blk_hist turned
+// into each workgroup's exclusive offset within the tile.
+__global__ __launch_bounds__(256) void k_bin_sum(BinArgs a) {
+   const int b = blockIdx.x * blockDim.x + threadIdx.x;
+   if (b >= a.nbins) return;
+   unsigned run = 0;
+   for (int k = 0; k < a.nblk; k++) {
+      unsigned *h = a.blk_hist + (size_t)k * a.nbins + b;
+      const unsigned c = *h;
+      *h = run;
+      run += c;
    }
-   __syncthreads();
-   for (int b = threadIdx.x; b < a.nbins; b += blockDim.x)
-      if (hist[b]) atomicAdd(&a.bin_total[b], hist[b]);
+   a.bin_total[b] = run;
 }
 
 // One workgroup: exclusive scans of the tile counts and of the work items
@@ -151,42 +89,62 @@ __global__ __launch_bounds__(1024) void k_bin_scan(BinArgs a) {
    }
 }
 
+template <typename RT>
+__device__ __forceinline__ unsigned rec_cell(RT v) {
+   return sizeof(RT) == 4 ? (unsigned)v & 0xffffffu : (unsigned)v;
+}
+template <typename RT>
+__device__ __forceinline__ bool rec_none(RT v) {
+   return (unsigned)v == FL3_REC_NONE;
+}
+
+// Workgroup k moves the records of walkers [256k, 256k + 256) (one column
+// block of the record matrix) to its reserved range of each tile.
+template <typename RT>
 __global__ __launch_bounds__(256) void k_bin_scatter(BinArgs a) {
-   unsigned *hist = (unsigned *)fl3_lds, *base = hist + a.nbins;
-   for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) hist[b] = 0;
-   __syncthreads();
-   for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < a.n;
-        r += (long long)gridDim.x * blockDim.x) {
-      const unsigned c = (unsigned)a.rec[r];
-      if (c != FL3_REC_NONE) atomicAdd(&hist[c >> a.shift], 1u);
-   }
-   __syncthreads();
+   unsigned *base = (unsigned *)fl3_lds, *cnt = base + a.nbins;
+   const unsigned *off = a.blk_hist + (size_t)blockIdx.x * a.nbins;
    for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) {
-      const unsigned h = hist[b];
-      base[b] = h ? a.bin_start[b] + atomicAdd(&a.bin_fill[b], h) : 0u;
-      hist[b] = 0;
+      base[b] = a.bin_start[b] + off[b];
+      cnt[b] = 0;
    }
    __syncthreads();
-   for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < a.n;
-        r += (long long)gridDim.x * blockDim.x) {
-      const unsigned long long v = a.rec[r];
-      const unsigned c = (unsigned)v;
-      if (c == FL3_REC_NONE) continue;
-      const unsigned b = c >> a.shift;
-      a.sorted[base[b] + atomicAdd(&hist[b], 1u)] = v;
+   const RT *rec = (const RT *)a.rec;
+   RT *sorted = (RT *)a.sorted;
+   const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   long long i = 0;
+   for (; i + 4 <= a.rows; i += 4) {
+      RT v[4];
+      for (int k = 0; k < 4; k++) v[k] = rec[(i + k) * a.walkers + col];
+      for (int k = 0; k < 4; k++) {
+         if (rec_none(v[k])) continue;
+         const unsigned b = rec_cell(v[k]) >> a.shift;
+         sorted[base[b] + atomicAdd(&cnt[b], 1u)] = v[k];
+      }
+   }
+   for (; i < a.rows; i++) {
+      const RT v = rec[i * a.walkers + col];
+      if (rec_none(v)) continue;
+      const unsigned b = rec_cell(v) >> a.shift;
+      sorted[base[b] + atomicAdd(&cnt[b], 1u)] = v;
    }
 }
 
 // Persistent workgroups take work items (tile, chunk of its records) from a
-// counter.  The tile's 5 channels live in LDS; a tile that has one work item
-// is added to the histogram without atomics.
+// counter.  The tile's channels live in LDS as two 64-bit pairs (R|G, B|A)
+// and the hit count: a work item adds at most 2^20 deltas of at most 255 to
+// a cell, so no 32-bit half carries into the other.  A tile that has one
+// work item is added to the histogram without atomics.
+template <typename RT>
 __global__ __launch_bounds__(256) void k_bin_accum(BinArgs a) {
    const fl3_job_hdr *h = (const fl3_job_hdr *)a.job;
    const int tile_cells = 1 << a.shift;
-   uint32_t *tile = (uint32_t *)fl3_lds;                       // tile_cells x 5
-   uint32_t *dmapu = tile + 5 * tile_cells;               // 1024
-   float *dmapf = (float *)(dmapu + 1024);                // 1024
-   float *vis = dmapf + 1024;                             // nxf
+   unsigned long long *rg = (unsigned long long *)fl3_lds;   // tile_cells
+   unsigned long long *ba = rg + tile_cells;                  // tile_cells
+   uint32_t *hits = (uint32_t *)(ba + tile_cells);            // tile_cells
+   uint32_t *dmapu = hits + tile_cells;                       // 1024
+   float *dmapf = (float *)(dmapu + 1024);                    // 1024
+   float *vis = dmapf + 1024;                                 // 256
    __shared__ int item_sh;
    const double *pd = (const double *)(a.job + h->off_dmap);
    const fl3_xform_pack *px = (const fl3_xform_pack *)(a.job + h->off_xf);
@@ -194,13 +152,18 @@ __global__ __launch_bounds__(256) void k_bin_accum(BinArgs a) {
       dmapu[e] = (uint32_t)pd[e];
       dmapf[e] = (float)pd[e];
    }
-   for (int e = threadIdx.x; e < h->nxf; e += blockDim.x) vis[e] = (float)px[e].vis;
+   for (int e = threadIdx.x; e < h->nxf && e < 256; e += blockDim.x) vis[e] = (float)px[e].vis;
    const int linear = h->palette_mode == 1;
    const long long total_items = a.item_start[a.nbins];
+   const RT *sorted = (const RT *)a.sorted;
    for (;;) {
       __syncthreads();
       if (threadIdx.x == 0) item_sh = (int)atomicAdd(a.item_counter, 1u);
-      for (int e = threadIdx.x; e < 5 * tile_cells; e += blockDim.x) tile[e] = 0;
+      for (int e = threadIdx.x; e < tile_cells; e += blockDim.x) {
+         rg[e] = 0;
+         ba[e] = 0;
+         hits[e] = 0;
+      }
       __syncthreads();
       const long long item = item_sh;
       if (item >= total_items) break;
@@ -216,38 +179,53 @@ __global__ __launch_bounds__(256) void k_bin_accum(BinArgs a) {
       const long long r1 = min((long long)a.bin_start[b + 1], r0 + a.chunk);
       const bool single = (a.item_start[b + 1] - a.item_start[b]) == 1;
       const unsigned cell0 = (unsigned)b << a.shift;
-      for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-         const unsigned long long v = a.sorted[r];
-         const unsigned local = (unsigned)v - cell0;
-         const int ci = (int)((v >> 32) & 0xff), fn = (int)((v >> 40) & 0xff);
-         const float vv = vis[fn];
-         uint32_t d[5];
-         if (linear) {
-            const float f = (float)(v >> 48) * (1.0f / 65535.0f);
-            for (int c = 0; c < 4; c++) {
-               const float col = dmapf[ci * 4 + c] * (1.0f - f) + dmapf[ci * 4 + 4 + c] * f;
-               d[c] = (uint32_t)(vv == 1.0f ? col : vv * col);
-            }
-         } else if (vv == 1.0f) {
-            for (int c = 0; c < 4; c++) d[c] = dmapu[ci * 4 + c];
-         } else {
-            for (int c = 0; c < 4; c++) d[c] = (uint32_t)(vv * dmapf[ci * 4 + c]);
+      for (long long rb = r0; rb < r1; rb += 4 * blockDim.x) {
+         RT v[4];
+         for (int u = 0; u < 4; u++) {
+            const long long r = rb + u * blockDim.x + threadIdx.x;
+            v[u] = r < r1 ? sorted[r] : (RT)FL3_REC_NONE;
          }
-         d[4] = vv == 1.0f ? 255u : (uint32_t)(vv * 255.0f);
-         uint32_t *t = tile + local * 5;
-         for (int c = 0; c < 5; c++)
-            if (d[c]) atomicAdd(t + c, d[c]);
+         for (int u = 0; u < 4; u++) {
+            if (rec_none(v[u])) continue;
+            const unsigned local = rec_cell(v[u]) - cell0;
+            uint32_t d[5];
+            if (sizeof(RT) == 4) {
+               const int ci = (int)((unsigned)v[u] >> 24);
+               for (int c = 0; c < 4; c++) d[c] = dmapu[ci * 4 + c];
+               d[4] = 255u;
+            } else {
+               const unsigned long long w = (unsigned long long)v[u];
+               const int ci = (int)((w >> 32) & 0xff), fn = (int)((w >> 40) & 0xff);
+               const float vv = vis[fn];
+               if (linear) {
+                  const float f = (float)(w >> 48) * (1.0f / 65535.0f);
+                  for (int c = 0; c < 4; c++) {
+                     const float col = dmapf[ci * 4 + c] * (1.0f - f) + dmapf[ci * 4 + 4 + c] * f;
+                     d[c] = (uint32_t)(vv == 1.0f ? col : vv * col);
+                  }
+               } else if (vv == 1.0f) {
+                  for (int c = 0; c < 4; c++) d[c] = dmapu[ci * 4 + c];
+               } else {
+                  for (int c = 0; c < 4; c++) d[c] = (uint32_t)(vv * dmapf[ci * 4 + c]);
+               }
+               d[4] = vv == 1.0f ? 255u : (uint32_t)(vv * 255.0f);
+            }
+            atomicAdd(&rg[local], (unsigned long long)d[0] | ((unsigned long long)d[1] << 32));
+            atomicAdd(&ba[local], (unsigned long long)d[2] | ((unsigned long long)d[3] << 32));
+            atomicAdd(&hits[local], d[4]);
+         }
       }
       __syncthreads();
       uint32_t *bk = (uint32_t *)a.buckets + (size_t)cell0 * 5;
       const long long ncells = (long long)a.width * a.height;
       const int nvalid = (int)min((long long)tile_cells, ncells - (long long)cell0);
       for (int e = threadIdx.x; e < nvalid * 5; e += blockDim.x) {
-         const uint32_t v = tile[e];
+         const int cl = e / 5, ch = e % 5;
+         const uint32_t v = ch == 4 ? hits[cl] : (uint32_t)((ch < 2 ? rg[cl] : ba[cl]) >> (32 * (ch & 1)));
          if (!v) continue;
          if (single) {
-            const unsigned long long s = (unsigned long long)bk[e] + v;
-            bk[e] = s > 0xffffffffull ? 0xffffffffu : (uint32_t)s;
+            const unsigned long long s2 = (unsigned long long)bk[e] + v;
+            bk[e] = s2 > 0xffffffffull ? 0xffffffffu : (uint32_t)s2;
          } else
             sat_add(bk + e, v);
       }
@@ -255,16 +233,27 @@ __global__ __launch_bounds__(256) void k_bin_accum(BinArgs a) {
 }
 
 extern "C" size_t fl3k_bin_lds_bytes(int shift) {
-   return sizeof(uint32_t) * 5 * ((size_t)1 << shift) + 8 * 1024 + 4 * 256 + 64;
+   return (size_t)20 * ((size_t)1 << shift) + 8 * 1024 + 4 * 256 + 64;
 }
 
-extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, hipStream_t s) {
-   const unsigned g = (unsigned)min(1024LL, max(1LL, (a->n + 256LL * 64 - 1) / (256LL * 64)));
-   hipLaunchKernelGGL(k_bin_count, dim3(g), dim3(256), sizeof(unsigned) * a->nbins, s, *a);
+extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int narrow, hipStream_t s) {
+   hipLaunchKernelGGL(k_bin_sum, dim3((a->nbins + 255) / 256), dim3(256), 0, s, *a);
    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, *a);
-   hipLaunchKernelGGL(k_bin_scatter, dim3(g), dim3(256), 2 * sizeof(unsigned) * a->nbins, s, *a);
+   const size_t lds_s = 2 * sizeof(unsigned) * a->nbins;
    const size_t lds = fl3k_bin_lds_bytes(a->shift);
-   if (lds > 65536) (void)hipFuncSetAttribute((const void *)k_bin_accum, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-   hipLaunchKernelGGL(k_bin_accum, dim3(ncu * (lds > 65536 ? 1 : 3)), dim3(256), lds, s, *a);
+   const unsigned gacc = (unsigned)ncu * (lds > 65536 ? 1 : 3);
+   if (narrow) {
+      hipLaunchKernelGGL(k_bin_scatter<unsigned>, dim3(a->nblk), dim3(256), lds_s, s, *a);
+      if (lds > 65536)
+         (void)hipFuncSetAttribute((const void *)k_bin_accum<unsigned>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds);
+      hipLaunchKernelGGL(k_bin_accum<unsigned>, dim3(gacc), dim3(256), lds, s, *a);
+   } else {
+      hipLaunchKernelGGL(k_bin_scatter<unsigned long long>, dim3(a->nblk), dim3(256), lds_s, s, *a);
+      if (lds > 65536)
+         (void)hipFuncSetAttribute((const void *)k_bin_accum<unsigned long long>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(k_bin_accum<unsigned long long>, dim3(gacc), dim3(256), lds, s, *a);
+   }
    return hipGetLastError() != hipSuccess;
 }

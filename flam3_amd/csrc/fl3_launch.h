@@ -9,8 +9,15 @@
 #include "fl3_kargs.h"
 
 extern "C" {
-int fl3k_launch_iter_fast(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int bits, int exact);
-int fl3k_launch_bin(const BinArgs *a, int ncu, hipStream_t s);
+int fl3k_launch_iter_fast(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int bits, int exact,
+                          int tier);
+/* tier in [0, FL3_TIER_FULL): kernels restricted to that variation tier */
+int fl3k_launch_iter_tier(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int tier);
+int fl3k_launch_apply_fp32(const unsigned char *job, int xi, const double *pts, long long n, double *out,
+                           unsigned long long key, size_t lds, hipStream_t s, int tier);
+int fl3k_launch_apply_fp32_tier(const unsigned char *job, int xi, const double *pts, long long n, double *out,
+                                unsigned long long key, size_t lds, hipStream_t s, int tier);
+int fl3k_launch_bin(const BinArgs *a, int ncu, int narrow, hipStream_t s);
 size_t fl3k_bin_lds_bytes(int shift);
 int fl3k_launch_replay(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int bits);
 int fl3k_launch_points(const PointsArgs *a, unsigned grid, size_t lds, hipStream_t s);

@@ -1,0 +1,132 @@
+// fl3_walker.h - the FP32 production walker kernel, instantiated per
+// variation tier by fl3_kernels_fast.hip (full set) and fl3_kernels_tier.hip
+// (the classic set and the classic set plus one block).
+#ifndef FL3_WALKER_H
+#define FL3_WALKER_H
+#include "fl3_core.h"
+
+// Record formats of the binned deposit (RECORDS template argument).
+#define FL3_REC_DIRECT 0   // no records: deposit with atomics
+#define FL3_REC_WIDE 1     // 8-byte records (make_record)
+#define FL3_REC_NARROW 2   // 4-byte records: cell | palette index << 24
+
+// Production walkers.  Walker w plots nsamples/W (+1 for the first
+// nsamples%W walkers) points after `fuse` unplotted steps from a random
+// start, exactly the per-sub-batch procedure of rect.c:393-400 with one long
+// sub-batch per walker.  In record mode each workgroup also counts its
+// records per tile in LDS and stores the counts at blk_hist[block][tile].
+template <int BITS, bool EXACT, int RECORDS, int TIER>
+__global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
+   unsigned char *lds = fl3_lds;
+   GenL<float> G;
+   load_genome<float>(a.job, lds, G);
+   unsigned *bh = (unsigned *)(lds + a.hist_off);
+   if (RECORDS)
+      for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) bh[b] = 0;
+   __syncthreads();
+   const long long W = (long long)gridDim.x * blockDim.x;
+   const long long wid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   const long long n = a.nsamples / W + (wid < a.nsamples % W ? 1 : 0);
+   WalkerRng rng;
+   float p[4], q[4];
+   int lastxf = 0, consec = 0;
+   uint32_t *ws = a.wstate;
+   if (a.resume) {
+      for (int k = 0; k < 4; k++) p[k] = __uint_as_float(ws[k * W + wid]);
+      rng.s0 = ws[4 * W + wid];
+      rng.s1 = ws[5 * W + wid];
+      rng.s2 = ws[6 * W + wid];
+      rng.s3 = ws[7 * W + wid];
+      lastxf = (int)ws[8 * W + wid];
+      consec = (int)ws[9 * W + wid];
+   } else {
+      rng.seed(a.key0, a.key1, (uint64_t)wid);
+      p[0] = u11<float>(rng);
+      p[1] = u11<float>(rng);
+      p[2] = u01<float>(rng);
+      p[3] = u01<float>(rng);
+   }
+   uint32_t bad = 0;
+   const bool use_final = G.final_index >= 0;
+   for (long long i = a.resume ? 0 : -a.fuse; i < n;) {
+      const uint32_t r = rng.next();
+      const int fn = select_xform(G, G.chaos_enable ? lastxf : 0, r & 16383u);
+      if (apply_xform<float, WalkerRng, TIER>(G, fn, p, q, rng)) {
+         consec++;
+         bad++;
+         if (consec < 5) {
+            p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
+            continue;
+         }
+         consec = 0;
+      } else
+         consec = 0;
+      lastxf = fn + 1;
+      p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
+      if (use_final) {
+         if (G.final_opacity == 1.0f || u01<float>(rng) < G.final_opacity) {
+            apply_xform<float, WalkerRng, TIER>(G, G.final_index, p, q, rng);
+            q[3] = p[3];
+         }
+      }
+      if (i >= 0) {
+         if (RECORDS == FL3_REC_WIDE) {
+            const unsigned long long v = make_record<float>(G, q, fn);
+            a.rec[i * W + wid] = v;
+            if ((unsigned)v != FL3_REC_NONE) atomicAdd(&bh[(unsigned)v >> a.bin_shift], 1u);
+         } else if (RECORDS == FL3_REC_NARROW) {
+            const unsigned long long v = make_record<float>(G, q, 0);
+            const unsigned cell = (unsigned)v;
+            ((unsigned *)a.rec)[i * W + wid] = cell == FL3_REC_NONE ? FL3_REC_NONE
+                                                                     : (cell | ((unsigned)(v >> 32) & 0xffu) << 24);
+            if (cell != FL3_REC_NONE) atomicAdd(&bh[cell >> a.bin_shift], 1u);
+         } else
+            deposit<float, BITS, EXACT>(G, q, a.buckets, a.sat);
+      }
+      i++;
+   }
+   if (RECORDS == FL3_REC_WIDE)
+      for (long long i = n; i < a.rows_out; i++) a.rec[i * W + wid] = FL3_REC_NONE;
+   if (RECORDS == FL3_REC_NARROW)
+      for (long long i = n; i < a.rows_out; i++) ((unsigned *)a.rec)[i * W + wid] = FL3_REC_NONE;
+   if (RECORDS) {
+      __syncthreads();
+      for (int b = threadIdx.x; b < a.nbins; b += blockDim.x)
+         a.blk_hist[(size_t)blockIdx.x * a.nbins + b] = bh[b];
+   }
+   if (ws) {
+      for (int k = 0; k < 4; k++) ws[k * W + wid] = __float_as_uint(p[k]);
+      ws[4 * W + wid] = rng.s0;
+      ws[5 * W + wid] = rng.s1;
+      ws[6 * W + wid] = rng.s2;
+      ws[7 * W + wid] = rng.s3;
+      ws[8 * W + wid] = (uint32_t)lastxf;
+      ws[9 * W + wid] = (uint32_t)consec;
+   }
+   add_bad(a.badvals, bad);
+}
+
+// One FP32 application of xform xi to each of n points (x, y, colour) in the
+// walkers' arithmetic: out = (x, y, colour, 1 if the result was a bad value).
+// Random variations draw from a walker stream keyed by (key, point index).
+template <int TIER>
+__global__ __launch_bounds__(256) void k_apply_fp32(const unsigned char *job, int xi, const double *pts,
+                                                    long long n, double *out, unsigned long long key) {
+   unsigned char *lds = fl3_lds;
+   GenL<float> G;
+   load_genome<float>(job, lds, G);
+   __syncthreads();
+   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+        i += (long long)gridDim.x * blockDim.x) {
+      WalkerRng rng;
+      rng.seed(key, ~key, (uint64_t)i);
+      float p[4] = {(float)pts[4 * i], (float)pts[4 * i + 1], (float)pts[4 * i + 2], (float)pts[4 * i + 3]}, q[4];
+      const bool bad = apply_xform<float, WalkerRng, TIER>(G, xi, p, q, rng);
+      out[4 * i] = q[0];
+      out[4 * i + 1] = q[1];
+      out[4 * i + 2] = q[2];
+      out[4 * i + 3] = bad ? 1.0 : 0.0;
+   }
+}
+
+#endif

@@ -63,6 +63,15 @@ int fl3h_iterate(flam3_genome *cp, int n, int fuse, double *samples,
 int fl3h_xform_preview(flam3_genome *cp, int xi, double range, int numvals, int depth,
                        double *result, randctx *rc);
 
+/* One FP32 application of xform xi (apply_xform, variations.c:2129-2399) to
+ * each point (x, y, colour, unused) in the production walkers' arithmetic and
+ * variation tier (tier -1: the tier the genome's variations select; 0-5 a
+ * restricted tier; 6 all variations).  out[4i..4i+3] = x, y, colour, 1.0 if
+ * the result was a bad value.  No counterpart in the reference: it exposes
+ * the FP32 walkers' xform step to tests. */
+int fl3h_apply_xform_fp32(flam3_genome *cp, int xi, const double *points, long long npoints, double *out,
+                          int tier);
+
 /* Camera and palette of one temporal sample, as rect.c:778-827 sets them up. */
 typedef struct {
    double bounds[4];       /* x0 y0 x1 y1, inclusive, gutter included */

@@ -129,6 +129,93 @@ def test_variation_fp64(prod, oracle, vi):
     assert list(rc.randrsl) == list(rco.randrsl) and rc.randcnt == rco.randcnt
 
 
+# Variations that draw random numbers (variations.c: the ones that call
+# flam3_random_isaac_*): FP32 walkers use their own streams, so these are
+# compared by the moments of many draws instead of point by point.
+RANDOM_VARS = {"julia", "noise", "julian", "juliascope", "blur", "gaussian_blur", "radial_blur", "pie", "arch",
+               "square", "rays", "blade", "twintrian", "super_shape", "flower", "conic", "parabola", "boarders",
+               "cpow", "pre_blur", "wedge_julia"}
+
+
+def _apply_oracle(oracle, g, pts, seed):
+    import pyref
+    out = np.zeros_like(pts)
+    with pyref.Parsed(G.flame_document([g])) as p:
+        pyref.ref().prepare_precalc_flags(p.ptr)
+        pyref.ref().xform_precalc(p.ptr, 0)
+        rco = oracle.seeded_rc(seed)
+        q = np.zeros(4)
+        for i in range(len(pts)):
+            pt = np.ascontiguousarray(pts[i])
+            oracle.lib().or_apply_xform(p.ptr, 0, pt.ctypes.data_as(ctypes.c_void_p),
+                                        q.ctypes.data_as(ctypes.c_void_p), ctypes.byref(rco))
+            out[i] = q
+    return out
+
+
+def _apply_fp32(prod, g, pts, tier=-1):
+    out = np.zeros_like(pts)
+    gp = g.to_struct()
+    try:
+        assert prod.lib().fl3h_apply_xform_fp32(gp, 0, pts.ctypes.data_as(ctypes.c_void_p), len(pts),
+                                                out.ctypes.data_as(ctypes.c_void_p), tier) == 0, prod.last_error()
+    finally:
+        G.free_struct(gp)
+    return out
+
+
+@pytest.mark.parametrize("vi", range(99))
+def test_variation_fp32_walker(prod, oracle, vi):
+    """The production walkers' FP32 xform step, in the kernel tier the
+    genome selects, against the FP64 oracle.  Deterministic variations: on a
+    13x13 grid at least 95% of the points within 2e-3 relative (FP32 with
+    the hardware sin/cos/exp2/log2 and approximate division; points next to
+    a pole or a discontinuity may move by more).  Random variations:
+    the same distribution of 4000 draws from one start point (two-sample
+    Kolmogorov-Smirnov, p > 1e-4)."""
+    g = _variation_genome(vi)
+    name = abi.VARIATION_NAMES[vi]
+    if name in RANDOM_VARS:
+        pts = np.tile(np.array([[0.3, -0.4, 0.5, 0.0]]), (4000, 1))
+        got = _apply_fp32(prod, g, pts)
+        want = _apply_oracle(oracle, g, pts, "fp32-%d" % vi)
+        ok = (got[:, 3] == 0) & np.isfinite(want[:, :2]).all(axis=1)
+        assert ok.mean() > 0.9, name
+        from scipy.stats import ks_2samp
+        for k in range(2):
+            # quantise to a grid far above FP32 rounding, so that outcomes
+            # with a few discrete values compare as equal atoms
+            a, b = got[ok, k], want[ok, k]
+            step = 1e-4 * max(1.0, float(np.abs(b).max()))
+            a, b = np.round(a / step), np.round(b / step)
+            assert ks_2samp(a, b).pvalue > 1e-4, (name, k, np.percentile(a, [10, 50, 90]),
+                                                  np.percentile(b, [10, 50, 90]))
+        return
+    nv = 6
+    xs = np.linspace(-2.0, 2.0, 2 * nv + 1)
+    pts = np.array([[x, y, 0.5, 0.0] for x in xs for y in xs])
+    got = _apply_fp32(prod, g, pts)
+    want = _apply_oracle(oracle, g, pts, "fp32-%d" % vi)
+    ok = (got[:, 3] == 0) & np.isfinite(want[:, :2]).all(axis=1) & (np.abs(want[:, :2]) < 1e9).all(axis=1)
+    rel = (np.abs(got[:, :2] - want[:, :2]) / np.maximum(1.0, np.abs(want[:, :2]))).max(axis=1)
+    assert ok.mean() >= 0.9, (name, ok.mean())
+    assert (rel[ok] <= 2e-3).mean() >= 0.95, (name, np.sort(rel[ok])[-10:])
+    assert np.allclose(got[ok, 2], want[ok, 2], rtol=1e-6, atol=1e-6)
+
+
+def test_variation_tiers_agree_with_full_kernel(prod):
+    """A tier kernel computes what the all-variation kernel computes."""
+    rng = np.random.default_rng(5)
+    pts = np.column_stack([rng.uniform(-2, 2, 500), rng.uniform(-2, 2, 500), np.full(500, 0.5), np.zeros(500)])
+    for vi in (3, 16, 42, 55, 70, 92):
+        g = _variation_genome(vi)
+        a = _apply_fp32(prod, g, pts, tier=-1)
+        b = _apply_fp32(prod, g, pts, tier=6)
+        ok = (a[:, 3] == 0) & (b[:, 3] == 0)
+        rel = np.abs(a[ok, :2] - b[ok, :2]) / np.maximum(1.0, np.abs(b[ok, :2]))
+        assert (rel <= 1e-5).mean() >= 0.99, (vi, rel.max())
+
+
 # ---------------------------------------------------------------- deposit
 
 def _camera(rng, width=64, height=48, rotate=False, linear=False, scalar=1.0):
