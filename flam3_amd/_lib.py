@@ -66,6 +66,10 @@ def lib():
     L.fl3h_xform_preview.restype = ci
     L.fl3h_apply_xform_fp32.argtypes = [P(abi.flam3_genome), ci, vp, ctypes.c_longlong, vp, ci]
     L.fl3h_apply_xform_fp32.restype = ci
+    L.fl3h_interpolate.argtypes = [P(abi.flam3_genome), ci, cd, cd, P(abi.flam3_genome)]
+    L.fl3h_interpolate.restype = ci
+    L.fl3h_genome_clear.argtypes = [P(abi.flam3_genome)]
+    L.fl3h_genome_clear.restype = None
     L.fl3h_deposit_points.argtypes = [P(fl3h_camera), vp, ctypes.c_longlong, ci, vp]
     L.fl3h_deposit_points.restype = ci
     L.fl3h_density_estimate.argtypes = [vp, ci, ci, ci, ci, cd, cd, cd, ci, ci, ci, ci, vp, vp, vp]

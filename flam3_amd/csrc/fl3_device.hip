@@ -219,12 +219,12 @@ static int read_counters(double *badvals, int *saturated) {
    return 0;
 }
 
-// Walkers per launch: enough resident waves to hide the atomic latency,
-// never fewer than ~64 samples per walker.
+// Walkers per launch: 8 workgroups of 256 per CU, never fewer than ~256
+// samples per walker (each walker pays its fuse steps once).
 static long long choose_walkers(long long nsamples, int requested) {
    if (requested > 0) return requested;
    long long maxw = (long long)g_ctx.ncu * 8 * 256;
-   long long w = nsamples / 64;
+   long long w = nsamples / 256;
    if (w > maxw) w = maxw;
    if (w < 256) w = 256;
    return (w + 255) / 256 * 256;
@@ -470,6 +470,10 @@ extern "C" int fl3d_density(const fl3d_de_params *p, const fl3_de_filters *de) {
    a.half = half;
    if (de->max_filter_index >= 1024) {
       set_err("density estimation: more than 1024 kernels");
+      return 1;
+   }
+   if (g_ctx.width >= 65536 || g_ctx.height >= 65536) {
+      set_err("density estimation: histogram side above 65535 cells");
       return 1;
    }
    if (g_ctx.findex.ensure(sizeof(uint16_t) * (size_t)g_ctx.width * g_ctx.height)) return 1;

@@ -17,3 +17,10 @@ int fl3_interpolate(const flam3_genome *cps, int ncps, double t, double stagger,
    if (ncps == 1) return fl3_genome_copy(dst, &cps[0]);
    return fl3_interpolate_multi(cps, ncps, t, stagger, dst);
 }
+
+int fl3h_interpolate(const flam3_genome *genomes, int ngenomes, double time, double stagger,
+                     flam3_genome *result) {
+   return fl3_interpolate(genomes, ngenomes, time, stagger, result);
+}
+
+void fl3h_genome_clear(flam3_genome *g) { fl3_genome_free(g); }

@@ -40,19 +40,51 @@ extern "C" int fl3k_launch_apply_fp32(const unsigned char *job, int xi, const do
 // sums commute, so the histogram is the one the atomic deposit produces; the
 // tile flush applies the reference's saturation exactly.
 
-// One thread per tile: total over walker workgroups, and blk_hist turned
-// into each workgroup's exclusive offset within the tile.
-__global__ __launch_bounds__(256) void k_bin_sum(BinArgs a) {
-   const int b = blockIdx.x * blockDim.x + threadIdx.x;
+// Tile totals over the walker workgroups, and blk_hist turned into each
+// workgroup's exclusive offset within its tile.  A workgroup covers 64
+// tiles; its 16 wavefronts split the walker workgroups into 16 ranges (rows
+// of 64 consecutive tiles are read coalesced), sum their ranges, exchange
+// the range sums in LDS, then write the running offsets.
+__global__ __launch_bounds__(1024) void k_bin_sum(BinArgs a) {
+   __shared__ unsigned part[16][64];
+   const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+   const int b = blockIdx.x * 64 + lane;
+   const int per = (a.nblk + 15) / 16;
+   const int k0 = grp * per, k1 = min(a.nblk, k0 + per);
+   unsigned s = 0;
+   if (b < a.nbins) {
+      int k = k0;
+      for (; k + 8 <= k1; k += 8) {
+         unsigned v[8];
+         for (int u = 0; u < 8; u++) v[u] = a.blk_hist[(size_t)(k + u) * a.nbins + b];
+         for (int u = 0; u < 8; u++) s += v[u];
+      }
+      for (; k < k1; k++) s += a.blk_hist[(size_t)k * a.nbins + b];
+   }
+   part[grp][lane] = s;
+   __syncthreads();
    if (b >= a.nbins) return;
-   unsigned run = 0;
-   for (int k = 0; k < a.nblk; k++) {
+   unsigned run = 0, total = 0;
+   for (int g = 0; g < 16; g++) {
+      if (g < grp) run += part[g][lane];
+      total += part[g][lane];
+   }
+   if (grp == 0) a.bin_total[b] = total;
+   int k = k0;
+   for (; k + 8 <= k1; k += 8) {
+      unsigned v[8];
+      for (int u = 0; u < 8; u++) v[u] = a.blk_hist[(size_t)(k + u) * a.nbins + b];
+      for (int u = 0; u < 8; u++) {
+         a.blk_hist[(size_t)(k + u) * a.nbins + b] = run;
+         run += v[u];
+      }
+   }
+   for (; k < k1; k++) {
       unsigned *h = a.blk_hist + (size_t)k * a.nbins + b;
       const unsigned c = *h;
       *h = run;
       run += c;
    }
-   a.bin_total[b] = run;
 }
 
 // One workgroup: exclusive scans of the tile counts and of the work items
@@ -237,7 +269,7 @@ extern "C" size_t fl3k_bin_lds_bytes(int shift) {
 }
 
 extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int narrow, hipStream_t s) {
-   hipLaunchKernelGGL(k_bin_sum, dim3((a->nbins + 255) / 256), dim3(256), 0, s, *a);
+   hipLaunchKernelGGL(k_bin_sum, dim3((a->nbins + 63) / 64), dim3(1024), 0, s, *a);
    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, *a);
    const size_t lds_s = 2 * sizeof(unsigned) * a->nbins;
    const size_t lds = fl3k_bin_lds_bytes(a->shift);

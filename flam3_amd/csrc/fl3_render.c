@@ -21,6 +21,7 @@
 #define WHITE_LEVEL 255.0
 #define PREFILTER_WHITE 255
 #define FUSE_NORMAL 15   /* rect.c:36 */
+#define FUSE_WALKER 100
 #define FUSE_EARLYCLIP 100
 
 static int env_int(const char *name, int dflt) {
@@ -302,7 +303,11 @@ int fl3h_render(flam3_frame *spec, void *out, int field, int nchan, int transp, 
                   keys[2 * ts + 1] = (w[2] | (w[3] << 32)) ^ ((uint64_t)tidx * 0x9E3779B97F4A7C15ULL);
                }
                memset(&ip, 0, sizeof(ip));
-               ip.fuse = spec->earlyclip ? FUSE_EARLYCLIP : FUSE_NORMAL;
+               /* The reference fuses 15 steps (100 with earlyclip) before
+                * each 10000-sample sub-batch; a production walker fuses once
+                * before a few hundred to a few thousand samples, so it fuses
+                * 100 steps to keep the start transient out of the image. */
+               ip.fuse = FUSE_WALKER;
                ip.nsamples = (int64_t)nthreads * (int64_t)(batch_size / (double)nthreads);
                ip.key[0] = keys[2 * ts];
                ip.key[1] = keys[2 * ts + 1];

@@ -41,6 +41,7 @@ class Xform:
     variations: Dict[str, float] = field(default_factory=dict)
     params: Dict[str, float] = field(default_factory=dict)
     chaos: Optional[List[float]] = None
+    animate: float = 1.0
 
 
 @dataclass
@@ -70,6 +71,9 @@ class Genome:
     palette: Optional[np.ndarray] = None      # 256 x 3, 0..255
     time: float = 0.0
     interpolation_type: str = "log"
+    interpolation: str = "linear"             # between control points: linear / smooth
+    palette_interpolation: str = "hsv_circular"
+    hsv_rgb_palette_blend: float = 0.0
     temporal_filter_type: str = "box"
     temporal_filter_width: float = 1.0
     temporal_filter_exp: float = 0.0
@@ -91,7 +95,9 @@ class Genome:
             ("highlight_power", r(self.highlight_power)), ("estimator_radius", r(self.estimator_radius)),
             ("estimator_minimum", r(self.estimator_minimum)), ("estimator_curve", r(self.estimator_curve)),
             ("gamma_threshold", r(self.gamma_threshold)), ("palette_mode", self.palette_mode),
-            ("interpolation_type", self.interpolation_type),
+            ("interpolation_type", self.interpolation_type), ("interpolation", self.interpolation),
+            ("palette_interpolation", self.palette_interpolation),
+            ("hsv_rgb_palette_blend", r(self.hsv_rgb_palette_blend)),
             ("temporal_filter_type", self.temporal_filter_type),
             ("temporal_filter_width", r(self.temporal_filter_width)),
             ("temporal_filter_exp", r(self.temporal_filter_exp)),
@@ -118,6 +124,8 @@ class Genome:
         a.append('color_speed="%s"' % r(x.color_speed))
         if tag == "xform":
             a.append('opacity="%s"' % r(x.opacity))
+        if x.animate != 1.0:
+            a.append('animate="%s"' % r(x.animate))
         for name, w in x.variations.items():
             a.append('%s="%s"' % (name, r(float(w))))
         for name, v in x.params.items():
@@ -162,6 +170,9 @@ class Genome:
         g.gam_lin_thresh = self.gamma_threshold
         g.palette_mode = 1 if self.palette_mode == "linear" else 0
         g.interpolation_type = {"linear": 0, "log": 1, "old": 2, "older": 3}[self.interpolation_type]
+        g.interpolation = {"linear": 0, "smooth": 1}[self.interpolation]
+        g.palette_interpolation = {"hsv": 0, "sweep": 1, "rgb": 2, "hsv_circular": 3}[self.palette_interpolation]
+        g.hsv_rgb_palette_blend = self.hsv_rgb_palette_blend
         g.temporal_filter_type = {"box": 0, "gaussian": 1, "exp": 2}[self.temporal_filter_type]
         g.temporal_filter_width = self.temporal_filter_width
         g.temporal_filter_exp = self.temporal_filter_exp
@@ -191,6 +202,8 @@ class Genome:
             d.opacity = x.opacity
         d.color = x.color
         d.color_speed = x.color_speed
+        if x.animate != 1.0:
+            d.animate = x.animate
         for name, w in x.variations.items():
             d.var[VAR_INDEX[name]] = float(w)
         for name, v in x.params.items():

@@ -129,6 +129,17 @@ void fl3h_init_frame(flam3_frame *f, const char *isaac_seed);
 flam3_genome *fl3h_genome_new(int nstd, int has_final);
 void fl3h_genome_free(flam3_genome *g);
 
+/* Replaces flam3_interpolate (flam3.h:562, flam3.c:717-802 with
+ * interpolation.c:373-1031): the genome at `time` between the control
+ * points genomes[0..ngenomes), sorted by time, written into *result (whose
+ * xforms, if any, are released first; zero-initialise a new result).
+ * Returns 0, or 1 when ngenomes < 1 or memory runs out. */
+int fl3h_interpolate(const flam3_genome *genomes, int ngenomes, double time, double stagger,
+                     flam3_genome *result);
+/* Release the xforms and xaos rows of a genome filled by fl3h_interpolate
+ * (the struct itself stays the caller's). */
+void fl3h_genome_clear(flam3_genome *g);
+
 /* Stage timing (no counterpart in the reference): with profiling enabled
  * every kernel launch of the render path is bracketed by HIP events on the
  * stream it runs on; totals accumulate until the next reset. */

@@ -70,6 +70,8 @@ def ref(kind="strict"):
     L.ref_free_genomes.argtypes = [P(abi.flam3_genome), ci]
     L.ref_interpolate.argtypes = [P(abi.flam3_genome), ci, cd]
     L.ref_interpolate.restype = P(abi.flam3_genome)
+    L.ref_interpolate_stagger.argtypes = [P(abi.flam3_genome), ci, cd, cd]
+    L.ref_interpolate_stagger.restype = P(abi.flam3_genome)
     L.flam3_render_memory_required.argtypes = [P(abi.flam3_frame)]
     L.flam3_render_memory_required.restype = cd
     _libs[kind] = L
@@ -130,7 +132,8 @@ def make_frame(gptr, n, *, bits=33, bpc=1, earlyclip=0, time=0.0, nthreads=1, su
 def render(xml, *, nchan=3, transp=0, bits=33, bpc=1, earlyclip=0, time=0.0, nthreads=1,
            sub_batch_size=10000, seed="flam3", kind="strict", field=0):
     """Reference flam3_render of the genomes in `xml`; returns (image, stats)."""
-    os.environ.setdefault("flam3_palettes", PALETTES)
+    if os.path.exists(PALETTES):  # only palette-index genomes need it; the test genomes carry colours
+        os.environ.setdefault("flam3_palettes", PALETTES)
     with Parsed(xml, kind) as p:
         g0 = p[0]
         f = make_frame(p.ptr, p.n, bits=bits, bpc=bpc, earlyclip=earlyclip, time=time, nthreads=nthreads,

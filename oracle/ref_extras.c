@@ -51,6 +51,12 @@ flam3_genome *ref_interpolate(flam3_genome *cps, int ncps, double t) {
    return r;
 }
 
+flam3_genome *ref_interpolate_stagger(flam3_genome *cps, int ncps, double t, double stagger) {
+   flam3_genome *r = (flam3_genome *)calloc(1, sizeof(flam3_genome));
+   flam3_interpolate(cps, ncps, t, stagger, r);
+   return r;
+}
+
 size_t ref_sizeof_genome(void) { return sizeof(flam3_genome); }
 size_t ref_sizeof_xform(void) { return sizeof(flam3_xform); }
 size_t ref_sizeof_frame(void) { return sizeof(flam3_frame); }

@@ -38,3 +38,10 @@ def prod():
     import flam3_amd
     flam3_amd.lib()
     return flam3_amd
+
+
+@pytest.fixture(scope="session")
+def prod_lib():
+    """The product library for its host-only entry points (no GPU needed)."""
+    import flam3_amd
+    return flam3_amd.lib()
