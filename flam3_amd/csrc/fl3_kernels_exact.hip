@@ -166,6 +166,7 @@ __global__ __launch_bounds__(256) void k_deposit_points(const unsigned char *job
 #define DE_SKIP 0xffffu
 #define DE_TW 64
 #define DE_TH 64
+#define DE_PLANE (DE_TW * DE_TH)  // the LDS tile holds one plane per channel
 
 template <typename BT, typename AT>
 __global__ __launch_bounds__(256) void k_de_prep(DEArgs a) {
@@ -232,11 +233,11 @@ __device__ __forceinline__ void de_splat(AT *tile, const AT *crow, const AT *v, 
       const int hi = max(dx, dy), lo = min(dx, dy);
       const AT coef = crow[hi * (hi + 1) / 2 + lo];
       if (coef == (AT)0) continue;
-      AT *t = tile + ((y - ty0) * DE_TW + (x - tx0)) * 4;
-      atomicAdd(t + 0, coef * v0);
-      atomicAdd(t + 1, coef * v1);
-      atomicAdd(t + 2, coef * v2);
-      atomicAdd(t + 3, coef * v3);
+      AT *t = tile + (y - ty0) * DE_TW + (x - tx0);
+      atomicAdd(t, coef * v0);
+      atomicAdd(t + DE_PLANE, coef * v1);
+      atomicAdd(t + 2 * DE_PLANE, coef * v2);
+      atomicAdd(t + 3 * DE_PLANE, coef * v3);
    }
 }
 
@@ -302,8 +303,7 @@ __global__ __launch_bounds__(256) void k_de_tile(DEArgs a) {
       const int x = tx0 + e % DE_TW, y = ty0 + e / DE_TW;
       if (x >= W || y >= H) continue;
       AT *d = acc + (x + (long)y * W) * 4;
-      const AT *t = tile + e * 4;
-      for (int k = 0; k < 4; k++) d[k] += t[k];
+      for (int k = 0; k < 4; k++) d[k] += tile[k * DE_PLANE + e];
    }
 }
 
@@ -406,11 +406,11 @@ __global__ __launch_bounds__(DF_THREADS) void k_de_flat(DEArgs a) {
             const int hi2 = max(dx, dy), lo2 = min(dx, dy);
             const AT coef = crow[hi2 * (hi2 + 1) / 2 + lo2];
             if (coef != (AT)0) {
-               AT *tp = tile + ((y - ty0) * DE_TW + (x - tx0)) * 4;
-               atomicAdd(tp + 0, coef * v0);
-               atomicAdd(tp + 1, coef * v1);
-               atomicAdd(tp + 2, coef * v2);
-               atomicAdd(tp + 3, coef * v3);
+               AT *tp = tile + (y - ty0) * DE_TW + (x - tx0);
+               atomicAdd(tp, coef * v0);
+               atomicAdd(tp + DE_PLANE, coef * v1);
+               atomicAdd(tp + 2 * DE_PLANE, coef * v2);
+               atomicAdd(tp + 3 * DE_PLANE, coef * v3);
             }
             if (++x > cx1) {
                x = cx0;
@@ -425,8 +425,7 @@ __global__ __launch_bounds__(DF_THREADS) void k_de_flat(DEArgs a) {
       const int x = tx0 + e % DE_TW, y = ty0 + e / DE_TW;
       if (x >= W || y >= H) continue;
       AT *d = acc + (x + (long)y * W) * 4;
-      const AT *t = tile + e * 4;
-      for (int c = 0; c < 4; c++) d[c] += t[c];
+      for (int c = 0; c < 4; c++) d[c] += tile[c * DE_PLANE + e];
    }
 }
 
