@@ -41,6 +41,7 @@ int fl3d_iterate(const void *job, size_t job_bytes, const fl3d_iter_params *p,
 typedef struct {
    int oversample;
    double k1, k2, curve;
+   int exact;           /* contributions formed and summed as the reference does */
 } fl3d_de_params;
 
 /* Density estimation into the accumulator (rect.c:57-251). */

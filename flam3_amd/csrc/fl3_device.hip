@@ -83,7 +83,7 @@ struct Ctx {
    hipStream_t stream = nullptr;
    DevBuf buckets, accum, job, table, image, isaac, points, misc, findex;
    DevBuf rec, sorted, wstate, bins, blkhist;   // tile-binned deposit
-   DevBuf srcv;                        // density estimation per-source values
+   DevBuf srcv, srcl;                  // density estimation per-source values
    unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
    int width = 0, height = 0, bits = 33;
 };
@@ -479,7 +479,10 @@ extern "C" int fl3d_density(const fl3d_de_params *p, const fl3_de_filters *de) {
    if (g_ctx.findex.ensure(sizeof(uint16_t) * (size_t)g_ctx.width * g_ctx.height)) return 1;
    a.findex = (uint16_t *)g_ctx.findex.p;
    if (g_ctx.srcv.ensure(accum_bytes(g_ctx.bits) * (size_t)g_ctx.width * g_ctx.height)) return 1;
+   if (g_ctx.srcl.ensure(sizeof(double) * (size_t)g_ctx.width * g_ctx.height)) return 1;
    a.srcv = g_ctx.srcv.p;
+   a.srcl = (double *)g_ctx.srcl.p;
+   a.exact = p->exact || g_ctx.bits == 64;
    const int pk = prof_begin(ST_DE);
    const int lrc = fl3k_launch_de_tiled(&a, g_ctx.stream, g_ctx.bits);
    prof_end(pk);
@@ -807,6 +810,7 @@ extern "C" int fl3h_density_estimate(const void *buckets, int width, int height,
    p.k1 = k1;
    p.k2 = k2;
    p.curve = curve;
+   p.exact = 1;
    if (max_filter_index > 0) {
       if (fl3d_density(&p, &de)) return 1;
    } else {

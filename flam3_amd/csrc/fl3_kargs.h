@@ -74,6 +74,8 @@ struct DEArgs {
    uint16_t *findex;        // per cell filter index, 0xffff = no contribution
    void *srcv;              // per cell AT[4]: colour x k1 log(1 + c3 k2) / c3
    const void *coefs_t;     // the kernel coefficients in the accumulator type
+   double *srcl;            // per cell k1 log(1 + c3 k2) (the reference's per-tap numerator)
+   int exact;               // sum each contribution as the reference does (double, in source order)
    int width, height, oversample;
    int max_filtered_counts, max_filter_index, kernel_size;
    int half;                // largest kernel radius (cells)

@@ -168,6 +168,11 @@ __global__ __launch_bounds__(256) void k_deposit_points(const unsigned char *job
 #define DE_TH 64
 #define DE_PLANE (DE_TW * DE_TH)  // the LDS tile holds one plane per channel
 
+// Channel i of the histogram as a double: uint32 cells with a float
+// accumulator (bits 33), double cells with a double one (bits 64).
+__device__ __forceinline__ double bucket_value(const void *b, long i, float) { return (double)((const uint32_t *)b)[i]; }
+__device__ __forceinline__ double bucket_value(const void *b, long i, double) { return ((const double *)b)[i]; }
+
 template <typename BT, typename AT>
 __global__ __launch_bounds__(256) void k_de_prep(DEArgs a) {
    const int os = a.oversample;
@@ -206,7 +211,9 @@ __global__ __launch_bounds__(256) void k_de_prep(DEArgs a) {
          // the per-source factor of rect.c:137, k1 log(1 + c3 k2) / c3,
          // applied to the colour once; the kernel coefficient multiplies
          // it per tap
-         const double sc = (a.k1 * log(1.0 + c3 * a.k2)) / c3;
+         const double lnum = a.k1 * log(1.0 + c3 * a.k2);
+         a.srcl[cell] = lnum;
+         const double sc = lnum / c3;
          AT *v = (AT *)a.srcv + cell * 4;
          v[0] = (AT)((double)b[cell * 5 + 0] * sc);
          v[1] = (AT)((double)b[cell * 5 + 1] * sc);
@@ -429,6 +436,106 @@ __global__ __launch_bounds__(DF_THREADS) void k_de_flat(DEArgs a) {
    }
 }
 
+// Gather form: each thread owns one output cell and adds, in registers,
+// the contribution of every source whose kernel reaches it, sources taken
+// in row-major order -- the order in which the reference's single DE thread
+// reaches a cell (rect.c:80-173) -- so the sum is deterministic and, in the
+// exact mode, formed operation by operation as the reference forms it:
+// ls = coef * k1 log(1 + c3 k2) / c3, c *= ls, accumulator += c (float
+// accumulator: rounded once per contribution).  A workgroup covers 16x16
+// cells (four 8x8 blocks, one per wavefront); it walks its halo in chunks of
+// 256 cells, compacts the sources that reach it into LDS in order, and each
+// wavefront skips the ones that miss its block.  No atomics.
+#define DG_REGION 16
+#define DG_CHUNK 256
+template <typename AT, bool EXACT>
+__global__ __launch_bounds__(DG_CHUNK) void k_de_gather(DEArgs a) {
+   __shared__ int radius[1024];
+   __shared__ unsigned s_pos[DG_CHUNK];
+   __shared__ int s_fi[DG_CHUNK];
+   __shared__ double s_val[DG_CHUNK][5];
+   __shared__ unsigned wave_cnt[DG_CHUNK / 64];
+   const int W = a.width, H = a.height, R = a.half;
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   const int rx0 = blockIdx.x * DG_REGION, ry0 = blockIdx.y * DG_REGION;
+   const int rx1 = min(rx0 + DG_REGION, W) - 1, ry1 = min(ry0 + DG_REGION, H) - 1;
+   const int mx0 = rx0 + 8 * (wave & 1), my0 = ry0 + 8 * (wave >> 1);
+   const int x = mx0 + (lane & 7), y = my0 + (lane >> 3);
+   const bool live = x < W && y < H;
+   AT *accp = (AT *)a.accum + (live ? (x + (long)y * W) * 4 : 0);
+   AT acc[4] = {(AT)0, (AT)0, (AT)0, (AT)0};
+   if (live)
+      for (int c = 0; c < 4; c++) acc[c] = accp[c];
+   for (int e = tid; e <= a.max_filter_index && e < 1024; e += DG_CHUNK) radius[e] = (int)ceil(a.widths[e]) - 1;
+   __syncthreads();
+   const int hx0 = max(0, rx0 - R), hx1 = min(W, rx1 + 1 + R);
+   const int hy0 = max(0, ry0 - R), hy1 = min(H, ry1 + 1 + R);
+   const int hw = hx1 - hx0, nh = hw * (hy1 - hy0);
+   const AT *srcv = (const AT *)a.srcv;
+   for (int base = 0; base < nh; base += DG_CHUNK) {
+      const int s = base + tid;
+      int flag = 0, sx = 0, sy = 0, fi = 0;
+      if (s < nh) {
+         sx = hx0 + s % hw;
+         sy = hy0 + s / hw;
+         const unsigned f = a.findex[sx + (long)sy * W];
+         if (f != DE_SKIP) {
+            fi = (int)f;
+            const int r = radius[fi];
+            const int gx = max(max(rx0 - sx, sx - rx1), 0), gy = max(max(ry0 - sy, sy - ry1), 0);
+            flag = gx <= r && gy <= r;
+         }
+      }
+      const unsigned long long m = __ballot(flag);
+      const unsigned rank = __popcll(m & ((1ull << lane) - 1ull));
+      if (lane == 0) wave_cnt[wave] = (unsigned)__popcll(m);
+      __syncthreads();
+      unsigned off = 0, n = 0;
+      for (int w = 0; w < DG_CHUNK / 64; w++) {
+         if (w < wave) off += wave_cnt[w];
+         n += wave_cnt[w];
+      }
+      if (flag) {
+         const unsigned idx = off + rank;
+         const long cell = sx + (long)sy * W;
+         s_pos[idx] = (unsigned)sx | ((unsigned)sy << 16);
+         s_fi[idx] = fi;
+         if (EXACT) {
+            for (int c = 0; c < 4; c++) s_val[idx][c] = bucket_value(a.buckets, cell * 5 + c, (AT)0);
+            s_val[idx][4] = a.srcl[cell];
+         } else {
+            for (int c = 0; c < 4; c++) s_val[idx][c] = (double)srcv[cell * 4 + c];
+         }
+      }
+      __syncthreads();
+      for (unsigned k = 0; k < n; k++) {
+         const unsigned pos = s_pos[k];
+         const int csx = (int)(pos & 0xffffu), csy = (int)(pos >> 16);
+         const int cfi = s_fi[k];
+         const int r = radius[cfi];
+         if (csx + r < mx0 || csx - r > mx0 + 7 || csy + r < my0 || csy - r > my0 + 7) continue;
+         const int dx = abs(x - csx), dy = abs(y - csy);
+         const int hi = max(dx, dy), lo = min(dx, dy);
+         if (!live || hi > r) continue;
+         const long ci = (long)cfi * a.kernel_size + hi * (hi + 1) / 2 + lo;
+         if (EXACT) {
+            const double coef = a.coefs[ci];
+            if (coef == 0.0) continue;
+            const double c3 = s_val[k][3];
+            const double ls = coef * s_val[k][4] / c3;
+            for (int c = 0; c < 4; c++) acc[c] = (AT)((double)acc[c] + s_val[k][c] * ls);
+         } else {
+            const AT coef = ((const AT *)a.coefs_t)[ci];
+            if (coef == (AT)0) continue;
+            for (int c = 0; c < 4; c++) acc[c] += coef * (AT)s_val[k][c];
+         }
+      }
+      __syncthreads();
+   }
+   if (live)
+      for (int c = 0; c < 4; c++) accp[c] = acc[c];
+}
+
 // Log scaling of every cell without DE (rect.c:947-974).
 template <typename BT, typename AT>
 __global__ __launch_bounds__(256) void k_logscale(const BT *b, AT *acc, long ncells, double k1, double k2) {
@@ -570,12 +677,16 @@ extern "C" int fl3k_launch_deposit(const unsigned char *job, const double *pts, 
 extern "C" int fl3k_launch_de_tiled(const DEArgs *a, hipStream_t s, int bits) {
    const dim3 g1((a->width + 255) / 256, a->height), b1(256);
    const dim3 g2((a->width + DE_TW - 1) / DE_TW, (a->height + DE_TH - 1) / DE_TH), b2(256);
+   const dim3 g3((a->width + DG_REGION - 1) / DG_REGION, (a->height + DG_REGION - 1) / DG_REGION);
    if (bits == 64) {
       hipLaunchKernelGGL((k_de_prep<double, double>), g1, b1, 0, s, *a);
-      hipLaunchKernelGGL((k_de_flat<double>), g2, dim3(DF_THREADS), 0, s, *a);
+      hipLaunchKernelGGL((k_de_gather<double, true>), g3, dim3(DG_CHUNK), 0, s, *a);
    } else {
       hipLaunchKernelGGL((k_de_prep<uint32_t, float>), g1, b1, 0, s, *a);
-      hipLaunchKernelGGL((k_de_flat<float>), g2, dim3(DF_THREADS), 0, s, *a);
+      if (a->exact)
+         hipLaunchKernelGGL((k_de_gather<float, true>), g3, dim3(DG_CHUNK), 0, s, *a);
+      else
+         hipLaunchKernelGGL((k_de_gather<float, false>), g3, dim3(DG_CHUNK), 0, s, *a);
    }
    return hipGetLastError() != hipSuccess;
 }

@@ -353,6 +353,7 @@ int fl3h_render(flam3_frame *spec, void *out, int field, int nchan, int transp, 
          dp.k1 = k1;
          dp.k2 = k2;
          dp.curve = cp.estimator_curve;
+         dp.exact = replay || spec->bits == 64;
          if (fl3d_density(&dp, &de)) {
             fl3_free_de_filters(&de);
             goto devfail;
