@@ -55,6 +55,9 @@ def parse_args():
     p.add_argument("--quality", type=float, default=500.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-quality", type=float, default=60.0)
+    p.add_argument("--copy-probe", action="store_true",
+                   help="after the timed steps, copy 1 GiB device to device with torch (a plain copy kernel of "
+                        "known size, to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     return p.parse_args()
 
 
@@ -144,6 +147,13 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     r = gpu_bench(args)
+    if args.copy_probe:
+        import torch
+        src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: 1073741824 bytes device to device")
     st = r["stage"]
     samples_per_step = r["samples"] / args.steps
     ms_per_step = 1000.0 * r["seconds"] / args.steps

@@ -85,6 +85,8 @@ struct Ctx {
    DevBuf rec, sorted, wstate, bins, blkhist;   // tile-binned deposit
    DevBuf srcv, srcl;                  // density estimation per-source values
    unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
+   void *staging = nullptr;                   // pinned host buffer for the image download
+   size_t staging_cap = 0;
    int width = 0, height = 0, bits = 33;
 };
 
@@ -550,8 +552,24 @@ extern "C" int fl3d_tonemap(const fl3d_tm_params *p, const double *filter, void 
          set_err("k_tonemap launch failed");
          return 1;
       }
-      HIPCHK(hipMemcpy2DAsync(out_host, out_stride, g_ctx.image.p, row, row, p->image_height,
-                              hipMemcpyDeviceToHost, g_ctx.stream));
+      // one copy into pinned staging, then rows into the caller's buffer
+      // (a strided copy straight to pageable memory goes row by row)
+      if (g_ctx.staging_cap < img) {
+         if (g_ctx.staging) (void)hipHostFree(g_ctx.staging);
+         g_ctx.staging = nullptr;
+         g_ctx.staging_cap = 0;
+         HIPCHK(hipHostMalloc(&g_ctx.staging, img, hipHostMallocDefault));
+         g_ctx.staging_cap = img;
+      }
+      HIPCHK(hipMemcpyAsync(g_ctx.staging, g_ctx.image.p, img, hipMemcpyDeviceToHost, g_ctx.stream));
+      HIPCHK(hipStreamSynchronize(g_ctx.stream));
+      const unsigned char *src = (const unsigned char *)g_ctx.staging;
+      unsigned char *dst = (unsigned char *)out_host;
+      if (out_stride == row)
+         memcpy(dst, src, img);
+      else
+         for (int j = 0; j < p->image_height; j++) memcpy(dst + (size_t)j * out_stride, src + (size_t)j * row, row);
+      return 0;
    }
    HIPCHK(hipStreamSynchronize(g_ctx.stream));
    return 0;
