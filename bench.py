@@ -57,7 +57,7 @@ def parse_args():
     p.add_argument("--cpu-quality", type=float, default=60.0)
     p.add_argument("--copy-probe", action="store_true",
                    help="after the timed steps, copy 1 GiB device to device with torch (a plain copy kernel of "
-                        "known size, to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
+                        "known size: 1 GiB read, 1 GiB written, to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return p.parse_args()
 
 
@@ -151,7 +151,7 @@ def main():
         import torch
         src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.add(src, 1, out=dst)  # an elementwise kernel (a plain copy may go to the DMA engine)
         torch.cuda.synchronize()
         log("copy probe: 1073741824 bytes device to device")
     st = r["stage"]

@@ -36,6 +36,7 @@ struct IterArgs {
 // item (bounds a tile cell's LDS partial sum by 2^20 * 255 < 2^32) and the
 // batch size from which the library prefers binning to atomics.
 #define FL3_MAX_BINS 8192
+
 #define FL3_BIN_CHUNK_SAMPLES (1LL << 28)
 #define FL3_BIN_ITEM_RECORDS (1 << 20)
 #define FL3_BIN_MIN_SAMPLES (1LL << 24)
