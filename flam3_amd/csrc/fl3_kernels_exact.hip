@@ -153,20 +153,14 @@ __global__ __launch_bounds__(256) void k_deposit_points(const unsigned char *job
 // ------------------------------------------------------------------ density estimation
 
 
-// ---- tiled density estimation ------------------------------------------
+// ---- density estimation ---------------------------------------------------
 //
 // The reference scatters each source cell's kernel into the accumulator
-// (rect.c:126-173).  Here pass 1 picks every source's kernel (the hit-count
-// neighbourhood sum and index selection of rect.c:92-117) and pass 2 gives
-// each workgroup one output tile held in LDS: it visits every source within
-// the largest kernel radius of the tile, adds the part of that source's
-// kernel that falls inside the tile with LDS atomics, and writes the tile
-// back once.  No global atomics; each accumulator cell is read and written
-// once per batch.
+// (rect.c:126-173).  Here pass 1 (k_de_prep) picks every source's kernel (the
+// hit-count neighbourhood sum and index selection of rect.c:92-117) and pass
+// 2 (k_de_gather) gathers, for each output cell, the contributions of the
+// sources that reach it.
 #define DE_SKIP 0xffffu
-#define DE_TW 64
-#define DE_TH 64
-#define DE_PLANE (DE_TW * DE_TH)  // the LDS tile holds one plane per channel
 
 // Channel i of the histogram as a double: uint32 cells with a float
 // accumulator (bits 33), double cells with a double one (bits 64).
@@ -222,218 +216,6 @@ __global__ __launch_bounds__(256) void k_de_prep(DEArgs a) {
       }
    }
    a.findex[cell] = out;
-}
-
-// Add the part of source (sx, sy)'s kernel that falls in the rectangle
-// [x0, x1] x [y0, y1] of the tile, positions p0, p0 + step, ...  The kernel
-// is radial: the coefficient of offset (dx, dy) is entry
-// hi (hi + 1) / 2 + lo of the source's row, hi = max(|dx|, |dy|),
-// lo = min(|dx|, |dy|) (the triangle walk of rect.c:123-170).
-template <typename AT>
-__device__ __forceinline__ void de_splat(AT *tile, const AT *crow, const AT *v, int sx, int sy, int x0, int x1,
-                                         int y0, int y1, int tx0, int ty0, int p0, int step) {
-   const int w = x1 - x0 + 1, n = w * (y1 - y0 + 1);
-   const AT v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3];
-   for (int p = p0; p < n; p += step) {
-      const int y = y0 + p / w, x = x0 + p % w;
-      const int dx = abs(x - sx), dy = abs(y - sy);
-      const int hi = max(dx, dy), lo = min(dx, dy);
-      const AT coef = crow[hi * (hi + 1) / 2 + lo];
-      if (coef == (AT)0) continue;
-      AT *t = tile + (y - ty0) * DE_TW + (x - tx0);
-      atomicAdd(t, coef * v0);
-      atomicAdd(t + DE_PLANE, coef * v1);
-      atomicAdd(t + 2 * DE_PLANE, coef * v2);
-      atomicAdd(t + 3 * DE_PLANE, coef * v3);
-   }
-}
-
-// One workgroup per 64x64 output tile, accumulated in LDS.  Each wave takes
-// 64 consecutive sources of the tile's halo; a source whose kernel covers at
-// most DE_LANE_TAPS cells of the tile is splatted by its own lane, larger
-// ones are splatted by the whole wave, one after another, so lanes do not
-// idle behind the widest kernel of their wave.
-#define DE_LANE_TAPS 24
-template <typename AT>
-__global__ __launch_bounds__(256) void k_de_tile(DEArgs a) {
-   __shared__ AT tile[DE_TH * DE_TW * 4];
-   __shared__ int radius[1024];
-   const int W = a.width, H = a.height, R = a.half;
-   const int tx0 = blockIdx.x * DE_TW, ty0 = blockIdx.y * DE_TH;
-   const int txe = min(tx0 + DE_TW, W) - 1, tye = min(ty0 + DE_TH, H) - 1;
-   for (int e = threadIdx.x; e < DE_TH * DE_TW * 4; e += blockDim.x) tile[e] = (AT)0;
-   for (int e = threadIdx.x; e <= a.max_filter_index && e < 1024; e += blockDim.x)
-      radius[e] = (int)ceil(a.widths[e]) - 1;
-   __syncthreads();
-   const AT *coefs = (const AT *)a.coefs_t;
-   const AT *srcv = (const AT *)a.srcv;
-   const int sx0 = max(0, tx0 - R), sx1 = min(W, tx0 + DE_TW + R);
-   const int sy0 = max(0, ty0 - R), sy1 = min(H, ty0 + DE_TH + R);
-   const int sw = sx1 - sx0, nsrc = sw * (sy1 - sy0);
-   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-   for (int base = wave * 64; base < nsrc; base += blockDim.x) {
-      const int s = base + lane;
-      int sx = 0, sy = 0, fi = 0, x0 = 1, x1 = 0, y0 = 1, y1 = 0;
-      long cell = 0;
-      if (s < nsrc) {
-         sx = sx0 + s % sw;
-         sy = sy0 + s / sw;
-         cell = sx + (long)sy * W;
-         const unsigned f = a.findex[cell];
-         if (f != DE_SKIP) {
-            fi = (int)f;
-            const int r = radius[fi];
-            x0 = max(sx - r, tx0); x1 = min(sx + r, txe);
-            y0 = max(sy - r, ty0); y1 = min(sy + r, tye);
-         }
-      }
-      const int taps = (x0 <= x1 && y0 <= y1) ? (x1 - x0 + 1) * (y1 - y0 + 1) : 0;
-      if (taps > 0 && taps <= DE_LANE_TAPS)
-         de_splat<AT>(tile, coefs + (long)fi * a.kernel_size, srcv + cell * 4, sx, sy, x0, x1, y0, y1, tx0, ty0, 0,
-                      1);
-      unsigned long long big = __ballot(taps > DE_LANE_TAPS);
-      while (big) {
-         const int l = __builtin_ctzll(big);
-         big &= big - 1;
-         const int bsx = __builtin_amdgcn_readlane(sx, l), bsy = __builtin_amdgcn_readlane(sy, l);
-         const int bfi = __builtin_amdgcn_readlane(fi, l);
-         const int bx0 = __builtin_amdgcn_readlane(x0, l), bx1 = __builtin_amdgcn_readlane(x1, l);
-         const int by0 = __builtin_amdgcn_readlane(y0, l), by1 = __builtin_amdgcn_readlane(y1, l);
-         const long bcell = bsx + (long)bsy * W;
-         de_splat<AT>(tile, coefs + (long)bfi * a.kernel_size, srcv + bcell * 4, bsx, bsy, bx0, bx1, by0, by1, tx0,
-                      ty0, lane, 64);
-      }
-   }
-   __syncthreads();
-   AT *acc = (AT *)a.accum;
-   for (int e = threadIdx.x; e < DE_TH * DE_TW; e += blockDim.x) {
-      const int x = tx0 + e % DE_TW, y = ty0 + e / DE_TW;
-      if (x >= W || y >= H) continue;
-      AT *d = acc + (x + (long)y * W) * 4;
-      for (int k = 0; k < 4; k++) d[k] += tile[k * DE_PLANE + e];
-   }
-}
-
-// Load-balanced variant: per batch of 512 halo cells, the workgroup lists
-// the sources whose kernel reaches the tile and the number of tile cells
-// each covers (a block scan), then every thread takes an equal contiguous
-// share of the batch's taps and walks them source by source.  No thread
-// waits for another's wide kernel, and a tap's cell comes from stepping, not
-// dividing.
-#define DF_THREADS 512
-template <typename AT>
-__global__ __launch_bounds__(DF_THREADS) void k_de_flat(DEArgs a) {
-   __shared__ AT tile[DE_TH * DE_TW * 4];
-   __shared__ int radius[1024];
-   __shared__ unsigned pre[DF_THREADS + 1];
-   __shared__ unsigned sinfo[DF_THREADS][4];
-   __shared__ unsigned wave_tot[DF_THREADS / 64];
-   const int W = a.width, H = a.height, R = a.half;
-   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-   const int tx0 = blockIdx.x * DE_TW, ty0 = blockIdx.y * DE_TH;
-   const int txe = min(tx0 + DE_TW, W) - 1, tye = min(ty0 + DE_TH, H) - 1;
-   for (int e = tid; e < DE_TH * DE_TW * 4; e += DF_THREADS) tile[e] = (AT)0;
-   for (int e = tid; e <= a.max_filter_index && e < 1024; e += DF_THREADS) radius[e] = (int)ceil(a.widths[e]) - 1;
-   if (tid == 0) pre[0] = 0;
-   __syncthreads();
-   const AT *coefs = (const AT *)a.coefs_t;
-   const AT *srcv = (const AT *)a.srcv;
-   const int sx0 = max(0, tx0 - R), sx1 = min(W, tx0 + DE_TW + R);
-   const int sy0 = max(0, ty0 - R), sy1 = min(H, ty0 + DE_TH + R);
-   const int sw = sx1 - sx0, nsrc = sw * (sy1 - sy0);
-   for (int base = 0; base < nsrc; base += DF_THREADS) {
-      // this thread's halo cell: its kernel's rectangle inside the tile
-      const int s = base + tid;
-      unsigned taps = 0;
-      int sx = 0, sy = 0, fi = 0, x0 = 0, x1 = 0, y0 = 0, y1 = 0;
-      if (s < nsrc) {
-         sx = sx0 + s % sw;
-         sy = sy0 + s / sw;
-         const unsigned f = a.findex[sx + (long)sy * W];
-         if (f != DE_SKIP) {
-            fi = (int)f;
-            const int r = radius[fi];
-            x0 = max(sx - r, tx0); x1 = min(sx + r, txe);
-            y0 = max(sy - r, ty0); y1 = min(sy + r, tye);
-            if (x0 <= x1 && y0 <= y1) taps = (unsigned)((x1 - x0 + 1) * (y1 - y0 + 1));
-         }
-      }
-      // inclusive scan of the tap counts over the workgroup
-      unsigned v = taps;
-      for (int off = 1; off < 64; off <<= 1) {
-         const unsigned n = __shfl_up(v, off, 64);
-         if (lane >= off) v += n;
-      }
-      if (lane == 63) wave_tot[wave] = v;
-      // coordinates < 65536 (the host checks the histogram size)
-      sinfo[tid][0] = (unsigned)sx | ((unsigned)sy << 16);
-      sinfo[tid][1] = (unsigned)fi;
-      sinfo[tid][2] = (unsigned)x0 | ((unsigned)x1 << 16);
-      sinfo[tid][3] = (unsigned)y0 | ((unsigned)y1 << 16);
-      __syncthreads();
-      for (int w = 0; w < wave; w++) v += wave_tot[w];
-      pre[tid + 1] = v;
-      __syncthreads();
-      const unsigned S = pre[DF_THREADS];
-      const unsigned t0 = (unsigned)((unsigned long long)S * tid / DF_THREADS);
-      const unsigned t1 = (unsigned)((unsigned long long)S * (tid + 1) / DF_THREADS);
-      if (t0 < t1) {
-         int lo = 0, hi = DF_THREADS - 1;
-         while (lo < hi) {  // last source whose first tap is <= t0
-            const int mid = (lo + hi + 1) >> 1;
-            if (pre[mid] <= t0) lo = mid;
-            else hi = mid - 1;
-         }
-         int j = lo;
-         int csx, csy, cx0, cx1, cy0, cy1;
-         const AT *crow;
-         AT v0, v1, v2, v3;
-         auto load = [&](int jj) {
-            const unsigned p0 = sinfo[jj][0], p2 = sinfo[jj][2], p3 = sinfo[jj][3];
-            csx = (int)(p0 & 0xffffu); csy = (int)(p0 >> 16);
-            cx0 = (int)(p2 & 0xffffu); cx1 = (int)(p2 >> 16);
-            cy0 = (int)(p3 & 0xffffu); cy1 = (int)(p3 >> 16);
-            crow = coefs + (long)sinfo[jj][1] * a.kernel_size;
-            const AT *vv = srcv + (csx + (long)csy * W) * 4;
-            v0 = vv[0]; v1 = vv[1]; v2 = vv[2]; v3 = vv[3];
-         };
-         load(j);
-         const int wdt = cx1 - cx0 + 1;
-         const unsigned k = t0 - pre[j];
-         int y = cy0 + (int)(k / wdt), x = cx0 + (int)(k % wdt);
-         for (unsigned t = t0; t < t1; t++) {
-            if (t == pre[j + 1]) {
-               j++;
-               while (pre[j + 1] == pre[j]) j++;
-               load(j);
-               x = cx0;
-               y = cy0;
-            }
-            const int dx = abs(x - csx), dy = abs(y - csy);
-            const int hi2 = max(dx, dy), lo2 = min(dx, dy);
-            const AT coef = crow[hi2 * (hi2 + 1) / 2 + lo2];
-            if (coef != (AT)0) {
-               AT *tp = tile + (y - ty0) * DE_TW + (x - tx0);
-               atomicAdd(tp, coef * v0);
-               atomicAdd(tp + DE_PLANE, coef * v1);
-               atomicAdd(tp + 2 * DE_PLANE, coef * v2);
-               atomicAdd(tp + 3 * DE_PLANE, coef * v3);
-            }
-            if (++x > cx1) {
-               x = cx0;
-               y++;
-            }
-         }
-      }
-      __syncthreads();
-   }
-   AT *acc = (AT *)a.accum;
-   for (int e = tid; e < DE_TH * DE_TW; e += DF_THREADS) {
-      const int x = tx0 + e % DE_TW, y = ty0 + e / DE_TW;
-      if (x >= W || y >= H) continue;
-      AT *d = acc + (x + (long)y * W) * 4;
-      for (int c = 0; c < 4; c++) d[c] += tile[c * DE_PLANE + e];
-   }
 }
 
 // Gather form: each thread owns one output cell and adds, in registers,
@@ -676,7 +458,6 @@ extern "C" int fl3k_launch_deposit(const unsigned char *job, const double *pts, 
 
 extern "C" int fl3k_launch_de_tiled(const DEArgs *a, hipStream_t s, int bits) {
    const dim3 g1((a->width + 255) / 256, a->height), b1(256);
-   const dim3 g2((a->width + DE_TW - 1) / DE_TW, (a->height + DE_TH - 1) / DE_TH), b2(256);
    const dim3 g3((a->width + DG_REGION - 1) / DG_REGION, (a->height + DG_REGION - 1) / DG_REGION);
    if (bits == 64) {
       hipLaunchKernelGGL((k_de_prep<double, double>), g1, b1, 0, s, *a);

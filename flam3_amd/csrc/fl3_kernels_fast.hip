@@ -40,6 +40,18 @@ extern "C" int fl3k_launch_apply_fp32(const unsigned char *job, int xi, const do
 // sums commute, so the histogram is the one the atomic deposit produces; the
 // tile flush applies the reference's saturation exactly.
 
+// Launch shape of the tile pipeline: threads per accumulation workgroup and
+// records each thread loads before working on them (accumulation, scatter).
+#ifndef FL3_ACC_THREADS
+#define FL3_ACC_THREADS 512
+#endif
+#ifndef FL3_ACC_UNROLL
+#define FL3_ACC_UNROLL 8
+#endif
+#ifndef FL3_SCATTER_UNROLL
+#define FL3_SCATTER_UNROLL 8
+#endif
+
 // Tile totals over the walker workgroups, and blk_hist turned into each
 // workgroup's exclusive offset within its tile.  A workgroup covers 64
 // tiles; its 16 wavefronts split the walker workgroups into 16 ranges (rows
@@ -145,10 +157,10 @@ __global__ __launch_bounds__(256) void k_bin_scatter(BinArgs a) {
    RT *sorted = (RT *)a.sorted;
    const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    long long i = 0;
-   for (; i + 4 <= a.rows; i += 4) {
-      RT v[4];
-      for (int k = 0; k < 4; k++) v[k] = rec[(i + k) * a.walkers + col];
-      for (int k = 0; k < 4; k++) {
+   for (; i + FL3_SCATTER_UNROLL <= a.rows; i += FL3_SCATTER_UNROLL) {
+      RT v[FL3_SCATTER_UNROLL];
+      for (int k = 0; k < FL3_SCATTER_UNROLL; k++) v[k] = rec[(i + k) * a.walkers + col];
+      for (int k = 0; k < FL3_SCATTER_UNROLL; k++) {
          if (rec_none(v[k])) continue;
          const unsigned b = rec_cell(v[k]) >> a.shift;
          sorted[base[b] + atomicAdd(&cnt[b], 1u)] = v[k];
@@ -168,7 +180,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(BinArgs a) {
 // a cell, so no 32-bit half carries into the other.  A tile that has one
 // work item is added to the histogram without atomics.
 template <typename RT>
-__global__ __launch_bounds__(256) void k_bin_accum(BinArgs a) {
+__global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum(BinArgs a) {
    const fl3_job_hdr *h = (const fl3_job_hdr *)a.job;
    const int tile_cells = 1 << a.shift;
    unsigned long long *rg = (unsigned long long *)fl3_lds;   // tile_cells
@@ -211,13 +223,13 @@ __global__ __launch_bounds__(256) void k_bin_accum(BinArgs a) {
       const long long r1 = min((long long)a.bin_start[b + 1], r0 + a.chunk);
       const bool single = (a.item_start[b + 1] - a.item_start[b]) == 1;
       const unsigned cell0 = (unsigned)b << a.shift;
-      for (long long rb = r0; rb < r1; rb += 4 * blockDim.x) {
-         RT v[4];
-         for (int u = 0; u < 4; u++) {
+      for (long long rb = r0; rb < r1; rb += FL3_ACC_UNROLL * blockDim.x) {
+         RT v[FL3_ACC_UNROLL];
+         for (int u = 0; u < FL3_ACC_UNROLL; u++) {
             const long long r = rb + u * blockDim.x + threadIdx.x;
             v[u] = r < r1 ? sorted[r] : (RT)FL3_REC_NONE;
          }
-         for (int u = 0; u < 4; u++) {
+         for (int u = 0; u < FL3_ACC_UNROLL; u++) {
             if (rec_none(v[u])) continue;
             const unsigned local = rec_cell(v[u]) - cell0;
             uint32_t d[5];
@@ -279,13 +291,13 @@ extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int narrow, hipStream_
       if (lds > 65536)
          (void)hipFuncSetAttribute((const void *)k_bin_accum<unsigned>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds);
-      hipLaunchKernelGGL(k_bin_accum<unsigned>, dim3(gacc), dim3(256), lds, s, *a);
+      hipLaunchKernelGGL(k_bin_accum<unsigned>, dim3(gacc), dim3(FL3_ACC_THREADS), lds, s, *a);
    } else {
       hipLaunchKernelGGL(k_bin_scatter<unsigned long long>, dim3(a->nblk), dim3(256), lds_s, s, *a);
       if (lds > 65536)
          (void)hipFuncSetAttribute((const void *)k_bin_accum<unsigned long long>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      hipLaunchKernelGGL(k_bin_accum<unsigned long long>, dim3(gacc), dim3(256), lds, s, *a);
+      hipLaunchKernelGGL(k_bin_accum<unsigned long long>, dim3(gacc), dim3(FL3_ACC_THREADS), lds, s, *a);
    }
    return hipGetLastError() != hipSuccess;
 }
