@@ -43,13 +43,13 @@ extern "C" int fl3k_launch_apply_fp32(const unsigned char *job, int xi, const do
 // Launch shape of the tile pipeline: threads per accumulation workgroup and
 // records each thread loads before working on them (accumulation, scatter).
 #ifndef FL3_ACC_THREADS
-#define FL3_ACC_THREADS 512
+#define FL3_ACC_THREADS 1024
 #endif
 #ifndef FL3_ACC_UNROLL
-#define FL3_ACC_UNROLL 8
+#define FL3_ACC_UNROLL 16
 #endif
 #ifndef FL3_SCATTER_UNROLL
-#define FL3_SCATTER_UNROLL 8
+#define FL3_SCATTER_UNROLL 16
 #endif
 
 // Tile totals over the walker workgroups, and blk_hist turned into each

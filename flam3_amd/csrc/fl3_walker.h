@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
       p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
       if (use_final) {
          if (G.final_opacity == 1.0f || u01<float>(rng) < G.final_opacity) {
-            apply_xform<float, WalkerRng, TIER>(G, G.final_index, p, q, rng);
+            apply_xform<float, WalkerRng, TIER, true>(G, G.final_index, p, q, rng);
             q[3] = p[3];
          }
       }
