@@ -252,7 +252,8 @@ static int pick_tier(const fl3_job_hdr *h) {
 // tile = 1 << shift), or 0 when the histogram is too large for it.
 static int bin_shift() {
    const long long ncells = (long long)g_ctx.width * g_ctx.height;
-   for (int sh = 11; sh <= 12; sh++)
+   // 4096-cell tiles when LDS allows (fewer, larger scatter runs), else 2048
+   for (int sh = 12; sh >= 11; sh--)
       if (((ncells + (1LL << sh) - 1) >> sh) <= FL3_MAX_BINS && fl3k_bin_lds_bytes(sh) <= g_ctx.lds_max) return sh;
    return 0;
 }
