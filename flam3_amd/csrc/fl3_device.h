@@ -37,6 +37,10 @@ typedef struct {
  * fast mode (the caller then repeats the batch with exact = 1). */
 int fl3d_iterate(const void *job, size_t job_bytes, const fl3d_iter_params *p,
                  double *badvals, int *saturated);
+/* The same for njobs jobs in one launch (the temporal samples of a batch):
+ * p->nsamples samples each, job j's walkers keyed by keys[2j], keys[2j+1]. */
+int fl3d_iterate_batch(const void *const *jobs, const size_t *job_bytes, int njobs, const fl3d_iter_params *p,
+                       const uint64_t *keys, double *badvals, int *saturated);
 
 typedef struct {
    int oversample;

@@ -82,7 +82,7 @@ struct Ctx {
    size_t lds_max = 65536;   // LDS bytes one workgroup may allocate
    hipStream_t stream = nullptr;
    DevBuf buckets, accum, job, table, image, isaac, points, misc, findex;
-   DevBuf rec, sorted, wstate, bins, blkhist;   // tile-binned deposit
+   DevBuf rec, sorted, wstate, bins, blkhist, jobtab;   // tile-binned deposit
    DevBuf srcv, srcl;                  // density estimation per-source values
    unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
    void *staging = nullptr;                   // pinned host buffer for the image download
@@ -269,17 +269,46 @@ static bool narrow_records_ok(const fl3_job_hdr *h, long long ncells) {
    return true;
 }
 
+// The jobs of one launch (one per temporal sample of a batch) as the
+// device sees them.
+struct JobSet {
+   int n = 1;
+   const fl3_job_hdr *const *h = nullptr;  // host headers
+   const unsigned char *dev_one = nullptr; // n == 1: the job
+   const unsigned char *dev_all = nullptr; // n > 1: concatenated jobs
+   const long long *dev_off = nullptr;     // n > 1: byte offsets
+   const unsigned long long *dev_keys = nullptr;
+   size_t lds = 0;                         // largest genome LDS image
+   int tier = 0, nxf_max = 0;
+   bool same_tables = true;                // one palette and visibility set for all jobs
+   bool linear = false;
+};
+
+static bool same_tables(const fl3_job_hdr *a, const fl3_job_hdr *b) {
+   if (a->nxf != b->nxf || a->nstd != b->nstd || a->palette_mode != b->palette_mode) return false;
+   if (memcmp((const unsigned char *)a + a->off_dmap, (const unsigned char *)b + b->off_dmap, sizeof(double) * 1024))
+      return false;
+   const fl3_xform_pack *pa = (const fl3_xform_pack *)((const unsigned char *)a + a->off_xf);
+   const fl3_xform_pack *pb = (const fl3_xform_pack *)((const unsigned char *)b + b->off_xf);
+   for (int e = 0; e < a->nxf; e++)
+      if (pa[e].vis != pb[e].vis) return false;
+   return true;
+}
+
 // Binned deposit: the walkers write records in chunks of `per` rows (the
 // walker state carries over between chunks, so the trajectories are those of
 // a single launch); each chunk is tile-sorted and accumulated.
-static int iterate_binned(IterArgs a, long long W, size_t lds, const fl3_job_hdr *h, const fl3d_iter_params *p,
-                          int shift, int tier) {
+static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &js, const fl3d_iter_params *p,
+                          int shift) {
    const long long ncells = (long long)g_ctx.width * g_ctx.height;
    const int nbins = (int)((ncells + (1LL << shift) - 1) >> shift);
-   const bool narrow = narrow_records_ok(h, ncells);
+   const bool narrow = js.same_tables && narrow_records_ok(js.h[0], ncells);
+   // one palette for all jobs: narrow or wide records; else the job index
+   // rides in the record (step palettes only: it takes the fraction's bits)
+   const int format = narrow ? 1 : (js.same_tables ? 0 : 2);
    long long per = p->chunk > 0 ? (p->chunk + W - 1) / W : (FL3_BIN_CHUNK_SAMPLES / W);
    if (per < 1) per = 1;
-   const long long rows_total = (p->nsamples + W - 1) / W;
+   const long long rows_total = (p->nsamples + Wj - 1) / Wj;
    if (per > rows_total) per = rows_total;
    if ((double)W * per >= 4.0e9) {
       set_err("binned deposit chunk exceeds 2^32 records");
@@ -295,7 +324,7 @@ static int iterate_binned(IterArgs a, long long W, size_t lds, const fl3_job_hdr
    unsigned *bw = (unsigned *)g_ctx.bins.p;
    BinArgs b;
    memset(&b, 0, sizeof(b));
-   b.job = a.job;
+   b.job = js.n > 1 ? js.dev_all : js.dev_one;  // palette tables of job 0 (identical for all when narrow)
    b.buckets = a.buckets;
    b.rec = g_ctx.rec.p;
    b.sorted = g_ctx.sorted.p;
@@ -308,33 +337,60 @@ static int iterate_binned(IterArgs a, long long W, size_t lds, const fl3_job_hdr
    b.item_counter = bw + 3 * nbins + 2;
    b.shift = shift;
    b.nbins = nbins;
-   // a work item's per-cell sums must fit 32 bits (and the packed pairs must
-   // not carry): at most 2^20 records, fewer if a palette entry exceeds 255
-   {
-      const double *dm = (const double *)((const unsigned char *)h + h->off_dmap);
-      double maxd = 255.0;
-      for (int e = 0; e < 1024; e++) maxd = dm[e] > maxd ? dm[e] : maxd;
-      const double lim = 4294967295.0 / (maxd + 1.0);
-      b.chunk = lim < (double)FL3_BIN_ITEM_RECORDS ? (int)lim : FL3_BIN_ITEM_RECORDS;
-      if (b.chunk < 1) b.chunk = 1;
-   }
    b.width = g_ctx.width;
    b.height = g_ctx.height;
-   a.rec_mode = narrow ? 2 : 1;
+   // a work item's per-cell sums must fit 32 bits (and the packed pairs must
+   // not carry): at most 2^20 records, fewer if a palette entry exceeds 255
+   double maxd = 255.0;
+   for (int j = 0; j < js.n; j++) {
+      const double *dm = (const double *)((const unsigned char *)js.h[j] + js.h[j]->off_dmap);
+      for (int e = 0; e < 1024; e++) maxd = dm[e] > maxd ? dm[e] : maxd;
+   }
+   const double lim = 4294967295.0 / (maxd + 1.0);
+   b.chunk = lim < (double)FL3_BIN_ITEM_RECORDS ? (int)lim : FL3_BIN_ITEM_RECORDS;
+   if (b.chunk < 1) b.chunk = 1;
+   if (format == 2) {
+      // palette and visibility of every job for the accumulation
+      const int vs = js.nxf_max > 0 ? js.nxf_max : 1;
+      std::vector<uint32_t> du((size_t)js.n * 1024);
+      std::vector<float> df((size_t)js.n * 1024), vis((size_t)js.n * vs, 1.0f);
+      for (int j = 0; j < js.n; j++) {
+         const unsigned char *hb = (const unsigned char *)js.h[j];
+         const double *dm = (const double *)(hb + js.h[j]->off_dmap);
+         const fl3_xform_pack *px = (const fl3_xform_pack *)(hb + js.h[j]->off_xf);
+         for (int e = 0; e < 1024; e++) {
+            du[(size_t)j * 1024 + e] = (uint32_t)dm[e];
+            df[(size_t)j * 1024 + e] = (float)dm[e];
+         }
+         for (int e = 0; e < js.h[j]->nxf; e++) vis[(size_t)j * vs + e] = (float)px[e].vis;
+      }
+      const size_t nb = du.size() * 4, fb = df.size() * 4, vb = vis.size() * 4;
+      if (g_ctx.jobtab.ensure(nb + fb + vb)) return 1;
+      unsigned char *t = (unsigned char *)g_ctx.jobtab.p;
+      HIPCHK(hipMemcpyAsync(t, du.data(), nb, hipMemcpyHostToDevice, g_ctx.stream));
+      HIPCHK(hipMemcpyAsync(t + nb, df.data(), fb, hipMemcpyHostToDevice, g_ctx.stream));
+      HIPCHK(hipMemcpyAsync(t + nb + fb, vis.data(), vb, hipMemcpyHostToDevice, g_ctx.stream));
+      HIPCHK(hipStreamSynchronize(g_ctx.stream));  // the host vectors go out of scope
+      b.dmapu_all = (const uint32_t *)t;
+      b.dmapf_all = (const float *)(t + nb);
+      b.vis_all = (const float *)(t + nb + fb);
+      b.vis_stride = vs;
+   }
+   a.rec_mode = narrow ? 2 : (format == 2 ? 3 : 1);
    a.rec = (unsigned long long *)g_ctx.rec.p;
    a.wstate = (uint32_t *)g_ctx.wstate.p;
    a.blk_hist = (unsigned *)g_ctx.blkhist.p;
    a.bin_shift = shift;
    a.nbins = nbins;
-   a.hist_off = (unsigned)((lds + 15) & ~(size_t)15);
+   a.hist_off = (unsigned)((js.lds + 15) & ~(size_t)15);
    const size_t lds_rec = a.hist_off + sizeof(unsigned) * (size_t)nbins;
    for (long long done = 0; done < p->nsamples;) {
-      const long long c = p->nsamples - done < W * per ? p->nsamples - done : W * per;
+      const long long c = p->nsamples - done < Wj * per ? p->nsamples - done : Wj * per;
       a.nsamples = c;
-      a.rows_out = (c + W - 1) / W;
+      a.rows_out = (c + Wj - 1) / Wj;
       a.resume = done > 0;
       int pk = prof_begin(ST_ITER);
-      int lrc = fl3k_launch_iter_fast(&a, (unsigned)nblk, lds_rec, g_ctx.stream, g_ctx.bits, 1, tier);
+      int lrc = fl3k_launch_iter_fast(&a, (unsigned)nblk, lds_rec, g_ctx.stream, g_ctx.bits, 1, js.tier);
       prof_end(pk);
       if (lrc) {
          set_err("k_iter_fast launch failed");
@@ -343,7 +399,7 @@ static int iterate_binned(IterArgs a, long long W, size_t lds, const fl3_job_hdr
       HIPCHK(hipMemsetAsync(b.item_counter, 0, sizeof(unsigned), g_ctx.stream));
       b.rows = a.rows_out;
       pk = prof_begin(ST_BIN);
-      lrc = fl3k_launch_bin(&b, g_ctx.ncu, narrow ? 1 : 0, g_ctx.stream);
+      lrc = fl3k_launch_bin(&b, g_ctx.ncu, format, g_ctx.stream);
       prof_end(pk);
       if (lrc) {
          set_err("binned deposit launch failed");
@@ -354,41 +410,90 @@ static int iterate_binned(IterArgs a, long long W, size_t lds, const fl3_job_hdr
    return 0;
 }
 
-extern "C" int fl3d_iterate(const void *job, size_t job_bytes, const fl3d_iter_params *p,
-                            double *badvals, int *saturated) {
+// One launch over njobs jobs (the temporal samples of a batch), p->nsamples
+// plotted samples each, job j's walkers keyed by keys[2j], keys[2j + 1].
+extern "C" int fl3d_iterate_batch(const void *const *jobs, const size_t *job_bytes, int njobs,
+                                  const fl3d_iter_params *p, const uint64_t *keys, double *badvals,
+                                  int *saturated) {
    std::lock_guard<std::recursive_mutex> lk(g_mu);
-   const fl3_job_hdr *h = (const fl3_job_hdr *)job;
-   if (upload_job(job, job_bytes)) return 1;
+   if (njobs < 1) {
+      set_err("fl3d_iterate_batch: no jobs");
+      return 1;
+   }
+   std::vector<const fl3_job_hdr *> hs((size_t)njobs);
+   JobSet js;
+   js.n = njobs;
+   for (int j = 0; j < njobs; j++) {
+      hs[j] = (const fl3_job_hdr *)jobs[j];
+      const size_t l = lds_bytes_for<float>(hs[j]);
+      js.lds = l > js.lds ? l : js.lds;
+      const int t = pick_tier(hs[j]);
+      js.tier = t > js.tier ? t : js.tier;
+      js.nxf_max = hs[j]->nxf > js.nxf_max ? hs[j]->nxf : js.nxf_max;
+      if (hs[j]->palette_mode != 0) js.linear = true;
+      if (j > 0 && !same_tables(hs[0], hs[j])) js.same_tables = false;
+   }
+   js.h = hs.data();
+   std::vector<long long> off((size_t)njobs);
+   if (njobs == 1) {
+      if (upload_job(jobs[0], job_bytes[0])) return 1;
+      js.dev_one = (const unsigned char *)g_ctx.job.p;
+   } else {
+      size_t total = 0;
+      for (int j = 0; j < njobs; j++) {
+         off[j] = (long long)total;
+         total += (job_bytes[j] + 255) & ~(size_t)255;
+      }
+      const size_t tab = sizeof(long long) * njobs + sizeof(uint64_t) * 2 * njobs;
+      std::vector<unsigned char> host(total + tab);
+      for (int j = 0; j < njobs; j++) memcpy(host.data() + off[j], jobs[j], job_bytes[j]);
+      memcpy(host.data() + total, off.data(), sizeof(long long) * njobs);
+      memcpy(host.data() + total + sizeof(long long) * njobs, keys, sizeof(uint64_t) * 2 * njobs);
+      if (g_ctx.job.ensure(total + tab)) return 1;
+      HIPCHK(hipMemcpyAsync(g_ctx.job.p, host.data(), total + tab, hipMemcpyHostToDevice, g_ctx.stream));
+      HIPCHK(hipStreamSynchronize(g_ctx.stream));  // host is a local buffer
+      js.dev_all = (const unsigned char *)g_ctx.job.p;
+      js.dev_off = (const long long *)(js.dev_all + total);
+      js.dev_keys = (const unsigned long long *)(js.dev_all + total + sizeof(long long) * njobs);
+   }
    HIPCHK(hipMemsetAsync(g_ctx.counters, 0, 16, g_ctx.stream));
    if (p->nsamples > 0) {
       IterArgs a;
       memset(&a, 0, sizeof(a));
-      a.job = (const unsigned char *)g_ctx.job.p;
+      a.job = js.dev_one;
+      a.jobs = js.dev_all;
+      a.job_off = js.dev_off;
+      a.keys = js.dev_keys;
+      a.njobs = njobs;
       a.buckets = g_ctx.buckets.p;
       a.nsamples = p->nsamples;
       a.fuse = p->fuse;
-      a.key0 = p->key[0];
-      a.key1 = p->key[1];
+      a.key0 = keys[0];
+      a.key1 = keys[1];
       a.badvals = g_ctx.counters;
       a.sat = (uint32_t *)(g_ctx.counters + 1);
-      const long long W = choose_walkers(p->nsamples, p->nwalkers);
-      const size_t lds = lds_bytes_for<float>(h);
+      // walkers: a multiple of 256 per job
+      long long W = choose_walkers(p->nsamples * njobs, p->nwalkers);
+      long long Wj = (W / njobs) / 256 * 256;
+      if (Wj < 256) Wj = 256;
+      W = Wj * njobs;
       const int shift = bin_shift();
-      const int tier = pick_tier(h);
-      const bool binnable = g_ctx.bits != 64 && h->nxf <= 256 && shift > 0;
+      const bool binnable = g_ctx.bits != 64 && js.nxf_max <= 256 && shift > 0 && !(njobs > 1 && js.linear &&
+                                                                                    !js.same_tables);
       if (p->deposit == 2 && !binnable) {
          set_err("tile-binned deposit needs bits 32/33, at most 256 xforms and at most %s%ld tiles", "",
                  (long)FL3_MAX_BINS);
          return 1;
       }
-      const bool binned = binnable && (p->deposit == 2 || (p->deposit == 0 && p->nsamples >= FL3_BIN_MIN_SAMPLES));
-      g_prof.samples += p->nsamples;
+      const bool binned =
+          binnable && (p->deposit == 2 || (p->deposit == 0 && p->nsamples * njobs >= FL3_BIN_MIN_SAMPLES));
+      g_prof.samples += p->nsamples * njobs;
       if (binned) {
-         if (iterate_binned(a, W, lds, h, p, shift, tier)) return 1;
+         if (iterate_binned(a, W, Wj, js, p, shift)) return 1;
       } else {
          const int pk = prof_begin(ST_ITER);
          const int lrc =
-             fl3k_launch_iter_fast(&a, (unsigned)(W / 256), lds, g_ctx.stream, g_ctx.bits, p->exact, tier);
+             fl3k_launch_iter_fast(&a, (unsigned)(W / 256), js.lds, g_ctx.stream, g_ctx.bits, p->exact, js.tier);
          prof_end(pk);
          if (lrc) {
             set_err("k_iter_fast launch failed");
@@ -397,6 +502,14 @@ extern "C" int fl3d_iterate(const void *job, size_t job_bytes, const fl3d_iter_p
       }
    }
    return read_counters(badvals, saturated);
+}
+
+extern "C" int fl3d_iterate(const void *job, size_t job_bytes, const fl3d_iter_params *p, double *badvals,
+                            int *saturated) {
+   const void *jobs[1] = {job};
+   const size_t bytes[1] = {job_bytes};
+   const uint64_t keys[2] = {p->key[0], p->key[1]};
+   return fl3d_iterate_batch(jobs, bytes, 1, p, keys, badvals, saturated);
 }
 
 // Replay launch: isaac holds 36 words per walker (host-seeded); updated.

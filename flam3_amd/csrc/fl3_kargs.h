@@ -6,6 +6,13 @@
 
 struct IterArgs {
    const unsigned char *job;
+   // several jobs (the temporal samples of a batch) in one launch: workgroup
+   // b iterates job b % njobs with walker key keys[2 * job .. +1]; nsamples
+   // is then per job
+   const unsigned char *jobs;
+   const long long *job_off;
+   const unsigned long long *keys;
+   int njobs;
    void *buckets;
    long long nsamples;      // fast: total plotted samples in the launch
    int fuse;
@@ -19,7 +26,8 @@ struct IterArgs {
    int nwalkers;
    // binned deposit: one 8-byte record per plotted sample, at i*W + walker;
    // rows_out rows are written (walkers with fewer samples pad with no-ops)
-   int rec_mode;              // 0 deposit with atomics, 1 8-byte records, 2 4-byte records
+   int rec_mode;              // 0 deposit with atomics, 1 8-byte records, 2 4-byte records,
+                              // 3 8-byte records carrying the job instead of the palette fraction
    unsigned long long *rec;   // 8-byte records, or 4-byte ones in the same buffer
    long long rows_out;
    unsigned *blk_hist;        // [workgroup][tile] record counts
@@ -56,6 +64,11 @@ struct BinArgs {
    int nbins;                      // tiles
    int chunk;                      // records per accumulation work item
    int width, height;
+   // job records: palette (floor and float) and xform visibility per job
+   const uint32_t *dmapu_all;      // [job][1024]
+   const float *dmapf_all;         // [job][1024]
+   const float *vis_all;           // [job][vis_stride]
+   int vis_stride;
 };
 
 struct PointsArgs {

@@ -8,7 +8,9 @@ extern "C" int fl3k_launch_iter_fast(const IterArgs *a, unsigned grid, size_t ld
                                      int exact, int tier) {
    if (tier >= 0 && tier < FL3_TIER_FULL && (a->rec_mode || (bits != 64 && !exact)))
       return fl3k_launch_iter_tier(a, grid, lds, s, tier);
-   if (a->rec_mode == FL3_REC_NARROW)
+   if (a->rec_mode == FL3_REC_JOB)
+      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_JOB, FL3_TIER_FULL>), dim3(grid), dim3(256), lds, s, *a);
+   else if (a->rec_mode == FL3_REC_NARROW)
       hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_NARROW, FL3_TIER_FULL>), dim3(grid), dim3(256), lds, s, *a);
    else if (a->rec_mode == FL3_REC_WIDE)
       hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_WIDE, FL3_TIER_FULL>), dim3(grid), dim3(256), lds, s, *a);
@@ -179,7 +181,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(BinArgs a) {
 // and the hit count: a work item adds at most 2^20 deltas of at most 255 to
 // a cell, so no 32-bit half carries into the other.  A tile that has one
 // work item is added to the histogram without atomics.
-template <typename RT>
+template <typename RT, bool JOBREC>
 __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum(BinArgs a) {
    const fl3_job_hdr *h = (const fl3_job_hdr *)a.job;
    const int tile_cells = 1 << a.shift;
@@ -237,6 +239,20 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum(BinArgs a) {
                const int ci = (int)((unsigned)v[u] >> 24);
                for (int c = 0; c < 4; c++) d[c] = dmapu[ci * 4 + c];
                d[4] = 255u;
+            } else if (JOBREC) {
+               // the record's job selects palette and visibility (stepped palette)
+               const unsigned long long w = (unsigned long long)v[u];
+               const int ci = (int)((w >> 32) & 0xff), fn = (int)((w >> 40) & 0xff), jb = (int)(w >> 48);
+               const float vv = a.vis_all[jb * a.vis_stride + fn];
+               const uint32_t *du = a.dmapu_all + jb * 1024 + ci * 4;
+               if (vv == 1.0f) {
+                  for (int c = 0; c < 4; c++) d[c] = du[c];
+                  d[4] = 255u;
+               } else {
+                  const float *df = a.dmapf_all + jb * 1024 + ci * 4;
+                  for (int c = 0; c < 4; c++) d[c] = (uint32_t)(vv * df[c]);
+                  d[4] = (uint32_t)(vv * 255.0f);
+               }
             } else {
                const unsigned long long w = (unsigned long long)v[u];
                const int ci = (int)((w >> 32) & 0xff), fn = (int)((w >> 40) & 0xff);
@@ -280,24 +296,32 @@ extern "C" size_t fl3k_bin_lds_bytes(int shift) {
    return (size_t)20 * ((size_t)1 << shift) + 8 * 1024 + 4 * 256 + 64;
 }
 
-extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int narrow, hipStream_t s) {
+extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_t s) {
    hipLaunchKernelGGL(k_bin_sum, dim3((a->nbins + 63) / 64), dim3(1024), 0, s, *a);
    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, *a);
    const size_t lds_s = 2 * sizeof(unsigned) * a->nbins;
    const size_t lds = fl3k_bin_lds_bytes(a->shift);
    const unsigned gacc = (unsigned)ncu * (lds > 65536 ? 1 : 3);
-   if (narrow) {
+   const int big = lds > 65536;
+   if (format == 1) {
       hipLaunchKernelGGL(k_bin_scatter<unsigned>, dim3(a->nblk), dim3(256), lds_s, s, *a);
-      if (lds > 65536)
-         (void)hipFuncSetAttribute((const void *)k_bin_accum<unsigned>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds);
-      hipLaunchKernelGGL(k_bin_accum<unsigned>, dim3(gacc), dim3(FL3_ACC_THREADS), lds, s, *a);
+      if (big)
+         (void)hipFuncSetAttribute((const void *)k_bin_accum<unsigned, false>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL((k_bin_accum<unsigned, false>), dim3(gacc), dim3(FL3_ACC_THREADS), lds, s, *a);
    } else {
       hipLaunchKernelGGL(k_bin_scatter<unsigned long long>, dim3(a->nblk), dim3(256), lds_s, s, *a);
-      if (lds > 65536)
-         (void)hipFuncSetAttribute((const void *)k_bin_accum<unsigned long long>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      hipLaunchKernelGGL(k_bin_accum<unsigned long long>, dim3(gacc), dim3(FL3_ACC_THREADS), lds, s, *a);
+      if (format == 2) {
+         if (big)
+            (void)hipFuncSetAttribute((const void *)k_bin_accum<unsigned long long, true>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+         hipLaunchKernelGGL((k_bin_accum<unsigned long long, true>), dim3(gacc), dim3(FL3_ACC_THREADS), lds, s, *a);
+      } else {
+         if (big)
+            (void)hipFuncSetAttribute((const void *)k_bin_accum<unsigned long long, false>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+         hipLaunchKernelGGL((k_bin_accum<unsigned long long, false>), dim3(gacc), dim3(FL3_ACC_THREADS), lds, s, *a);
+      }
    }
    return hipGetLastError() != hipSuccess;
 }

@@ -7,7 +7,9 @@
 
 template <int TIER>
 static void launch_tier(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s) {
-   if (a->rec_mode == FL3_REC_NARROW)
+   if (a->rec_mode == FL3_REC_JOB)
+      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_JOB, TIER>), dim3(grid), dim3(256), lds, s, *a);
+   else if (a->rec_mode == FL3_REC_NARROW)
       hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_NARROW, TIER>), dim3(grid), dim3(256), lds, s, *a);
    else if (a->rec_mode == FL3_REC_WIDE)
       hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_WIDE, TIER>), dim3(grid), dim3(256), lds, s, *a);

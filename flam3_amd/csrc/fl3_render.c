@@ -24,6 +24,37 @@
 #define FUSE_WALKER 100
 #define FUSE_EARLYCLIP 100
 
+/* Job records carry the job index in 16 bits. */
+#define MAX_BATCH_JOBS 4096
+
+/* One launch over the *n temporal samples collected so far (nsamples each,
+ * keys[2t], keys[2t+1] for the t-th); frees their jobs and empties the set. */
+static int flush_batch(void **jobs, size_t *bytes, int *n, int64_t nsamples, const uint64_t *keys, int exact,
+                       const fl3h_render_opts *opts, double *bad, int *sat) {
+   fl3d_iter_params ip;
+   int rc, t;
+   memset(&ip, 0, sizeof(ip));
+   /* The reference fuses 15 steps (100 with earlyclip) before each
+    * 10000-sample sub-batch; a production walker fuses once before a few
+    * hundred to a few thousand samples, so it fuses 100 steps to keep the
+    * start transient out of the image. */
+   ip.fuse = FUSE_WALKER;
+   ip.nsamples = nsamples;
+   ip.key[0] = keys[0];
+   ip.key[1] = keys[1];
+   ip.exact = exact;
+   ip.nwalkers = opts ? opts->nwalkers : 0;
+   ip.deposit = opts ? opts->deposit : 0;
+   ip.chunk = opts ? opts->chunk_samples : 0;
+   rc = fl3d_iterate_batch((const void *const *)jobs, bytes, *n, &ip, keys, bad, sat);
+   for (t = 0; t < *n; t++) {
+      free(jobs[t]);
+      jobs[t] = NULL;
+   }
+   *n = 0;
+   return rc;
+}
+
 static int env_int(const char *name, int dflt) {
    const char *v = getenv(name);
    return v ? atoi(v) : dflt;
@@ -131,6 +162,10 @@ int fl3h_render(flam3_frame *spec, void *out, int field, int nchan, int transp, 
    const int bpc = spec->bytes_per_channel;
    uint32_t *walker_states = NULL;
    uint64_t *keys = NULL;
+   void **bjobs = NULL;   /* temporal samples waiting for their launch */
+   size_t *bbytes = NULL;
+   int bn = 0, bts0 = 0;
+   int64_t bns = 0;
    time_t tstart = time(NULL);
 
    memset(&cp, 0, sizeof(cp));
@@ -197,7 +232,9 @@ int fl3h_render(flam3_frame *spec, void *out, int field, int nchan, int transp, 
    cam = (fl3h_camera *)calloc(1, sizeof(fl3h_camera));
    walker_states = (uint32_t *)malloc(sizeof(uint32_t) * 36 * (size_t)nthreads);
    keys = (uint64_t *)calloc((size_t)2 * nts, sizeof(uint64_t));
-   if (!cam || !walker_states || !keys) goto fail;
+   bjobs = (void **)calloc((size_t)(nts < MAX_BATCH_JOBS ? nts : MAX_BATCH_JOBS), sizeof(void *));
+   bbytes = (size_t *)calloc((size_t)(nts < MAX_BATCH_JOBS ? nts : MAX_BATCH_JOBS), sizeof(size_t));
+   if (!cam || !walker_states || !keys || !bjobs || !bbytes) goto fail;
 
    for (batch_num = 0; batch_num < nbatches && !aborted; batch_num++) {
       fl3_de_filters de;
@@ -288,7 +325,7 @@ int fl3h_render(flam3_frame *spec, void *out, int field, int nchan, int transp, 
                   goto devfail;
                }
             } else {
-               fl3d_iter_params ip;
+               int64_t nsamp;
                if (pass == 0) {
                   /* The frame stream advances by 16 words per reference thread,
                    * as rect.c:858-865 consumes it; the first four seed the
@@ -302,24 +339,24 @@ int fl3h_render(flam3_frame *spec, void *out, int field, int nchan, int transp, 
                   keys[2 * ts] = w[0] | (w[1] << 32);
                   keys[2 * ts + 1] = (w[2] | (w[3] << 32)) ^ ((uint64_t)tidx * 0x9E3779B97F4A7C15ULL);
                }
-               memset(&ip, 0, sizeof(ip));
-               /* The reference fuses 15 steps (100 with earlyclip) before
-                * each 10000-sample sub-batch; a production walker fuses once
-                * before a few hundred to a few thousand samples, so it fuses
-                * 100 steps to keep the start transient out of the image. */
-               ip.fuse = FUSE_WALKER;
-               ip.nsamples = (int64_t)nthreads * (int64_t)(batch_size / (double)nthreads);
-               ip.key[0] = keys[2 * ts];
-               ip.key[1] = keys[2 * ts + 1];
-               ip.exact = exact;
-               ip.nwalkers = opts ? opts->nwalkers : 0;
-               ip.deposit = opts ? opts->deposit : 0;
-               ip.chunk = opts ? opts->chunk_samples : 0;
-               if (fl3d_iterate(job, job_bytes, &ip, &bad, &sat)) {
-                  free(job);
-                  goto devfail;
+               /* Temporal samples with the same sample count go to the GPU
+                * together, one launch for the whole run of them. */
+               nsamp = (int64_t)nthreads * (int64_t)(batch_size / (double)nthreads);
+               if (bn > 0 && (nsamp != bns || bn == MAX_BATCH_JOBS)) {
+                  if (flush_batch(bjobs, bbytes, &bn, bns, keys + 2 * bts0, exact, opts, &bad, &sat)) goto devfail;
+                  badvals += bad;
+                  batch_bad += bad;
+                  if (sat) saturated = 1;
+                  bad = 0.0;
                }
-               if (sat) saturated = 1;
+               if (bn == 0) {
+                  bts0 = ts;
+                  bns = nsamp;
+               }
+               bjobs[bn] = job;
+               bbytes[bn] = job_bytes;
+               bn++;
+               job = NULL;
             }
             free(job);
             badvals += bad;
@@ -332,6 +369,14 @@ int fl3h_render(flam3_frame *spec, void *out, int field, int nchan, int transp, 
                background[2] += cp.background[2];
                vib_gam_n++;
             }
+         }
+         if (!aborted && bn > 0) {
+            double bad = 0.0;
+            int sat = 0;
+            if (flush_batch(bjobs, bbytes, &bn, bns, keys + 2 * bts0, exact, opts, &bad, &sat)) goto devfail;
+            badvals += bad;
+            batch_bad += bad;
+            if (sat) saturated = 1;
          }
          if (aborted) break;
       }
@@ -415,6 +460,9 @@ cleanup:
    free(cam);
    free(walker_states);
    free(keys);
+   for (i = 0; i < bn; i++) free(bjobs[i]);
+   free(bjobs);
+   free(bbytes);
    fl3_genome_free(&cp);
    return rv;
 }

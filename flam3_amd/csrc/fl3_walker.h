@@ -9,6 +9,7 @@
 #define FL3_REC_DIRECT 0   // no records: deposit with atomics
 #define FL3_REC_WIDE 1     // 8-byte records (make_record)
 #define FL3_REC_NARROW 2   // 4-byte records: cell | palette index << 24
+#define FL3_REC_JOB 3      // 8-byte records: cell, palette index, xform, job
 
 // Production walkers.  Walker w plots nsamples/W (+1 for the first
 // nsamples%W walkers) points after `fuse` unplotted steps from a random
@@ -19,14 +20,20 @@ template <int BITS, bool EXACT, int RECORDS, int TIER>
 __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
    unsigned char *lds = fl3_lds;
    GenL<float> G;
-   load_genome<float>(a.job, lds, G);
+   const int nj = a.njobs > 1 ? a.njobs : 1;
+   const int jb = nj > 1 ? (int)(blockIdx.x % nj) : 0;
+   load_genome<float>(nj > 1 ? a.jobs + a.job_off[jb] : a.job, lds, G);
    unsigned *bh = (unsigned *)(lds + a.hist_off);
    if (RECORDS)
       for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) bh[b] = 0;
    __syncthreads();
+   // W, wid: the record matrix and walker-state layout over all jobs;
+   // Wj, wj: this walker's job-local count and index
    const long long W = (long long)gridDim.x * blockDim.x;
    const long long wid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-   const long long n = a.nsamples / W + (wid < a.nsamples % W ? 1 : 0);
+   const long long Wj = (long long)(gridDim.x / nj) * blockDim.x;
+   const long long wj = (long long)(blockIdx.x / nj) * blockDim.x + threadIdx.x;
+   const long long n = a.nsamples / Wj + (wj < a.nsamples % Wj ? 1 : 0);
    WalkerRng rng;
    float p[4], q[4];
    int lastxf = 0, consec = 0;
@@ -40,7 +47,10 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
       lastxf = (int)ws[8 * W + wid];
       consec = (int)ws[9 * W + wid];
    } else {
-      rng.seed(a.key0, a.key1, (uint64_t)wid);
+      if (nj > 1)
+         rng.seed(a.keys[2 * jb], a.keys[2 * jb + 1], (uint64_t)wj);
+      else
+         rng.seed(a.key0, a.key1, (uint64_t)wid);
       p[0] = u11<float>(rng);
       p[1] = u11<float>(rng);
       p[2] = u01<float>(rng);
@@ -74,6 +84,12 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
             const unsigned long long v = make_record<float>(G, q, fn);
             a.rec[i * W + wid] = v;
             if ((unsigned)v != FL3_REC_NONE) atomicAdd(&bh[(unsigned)v >> a.bin_shift], 1u);
+         } else if (RECORDS == FL3_REC_JOB) {
+            const unsigned long long v = make_record<float>(G, q, fn);
+            a.rec[i * W + wid] = (unsigned)v == FL3_REC_NONE
+                                     ? (unsigned long long)FL3_REC_NONE
+                                     : ((v & 0xffffffffffffull) | ((unsigned long long)jb << 48));
+            if ((unsigned)v != FL3_REC_NONE) atomicAdd(&bh[(unsigned)v >> a.bin_shift], 1u);
          } else if (RECORDS == FL3_REC_NARROW) {
             const unsigned long long v = make_record<float>(G, q, 0);
             const unsigned cell = (unsigned)v;
@@ -85,7 +101,7 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
       }
       i++;
    }
-   if (RECORDS == FL3_REC_WIDE)
+   if (RECORDS == FL3_REC_WIDE || RECORDS == FL3_REC_JOB)
       for (long long i = n; i < a.rows_out; i++) a.rec[i * W + wid] = FL3_REC_NONE;
    if (RECORDS == FL3_REC_NARROW)
       for (long long i = n; i < a.rows_out; i++) ((unsigned *)a.rec)[i * W + wid] = FL3_REC_NONE;
