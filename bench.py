@@ -11,7 +11,7 @@ per second over the timed steps.
 The roofline entry is for the dominant kernel, the walker kernel
 k_iter_fast, timed with HIP events on its stream inside the timed region.
 It has no compulsory memory traffic (genome in LDS, walker state in
-registers, one 8-byte record written per sample), so it is bound by FP32
+registers, one 4-byte record written per sample), so it is bound by FP32
 vector arithmetic: achieved = samples per launch x the genome's counted FP32
 operations per sample (flam3_amd.opcount) / average launch time, against
 MI355X's FP32 peak of 157.3 TFLOP/s.  The other stages are reported with
@@ -34,13 +34,29 @@ sys.path.insert(0, ROOT)
 METRIC = "chaos-game samples/s + 1080p q500 frame time; preview-request p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# Algorithmic bytes of the binned deposit per plotted sample: the 8-byte
-# record written by the walkers, read by the tile count, read and written by
-# the scatter, read by the tile accumulation.
-BIN_BYTES_PER_SAMPLE = 40.0
+# Algorithmic bytes of the binned deposit per plotted sample of this genome
+# (4-byte records): read and written by the scatter, read by the tile
+# accumulation.
+BIN_BYTES_PER_SAMPLE = 12.0
 # Density estimation per histogram cell (bits 33): 20-byte bucket read,
 # 16-byte accumulator read and written.
 DE_BYTES_PER_CELL = 20.0 + 32.0
+
+
+# HBM bytes per walker launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE,
+# calibrated by k_copy_probe (profiles/k_iter_fast_1080p_q500_hbm_bytes.txt).
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_iter_fast_1080p_q500_hbm_bytes.txt")
+
+
+def measured_traffic():
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -56,8 +72,8 @@ def parse_args():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-quality", type=float, default=60.0)
     p.add_argument("--copy-probe", action="store_true",
-                   help="after the timed steps, copy 1 GiB device to device with torch (a plain copy kernel of "
-                        "known size: 1 GiB read, 1 GiB written, to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+                   help="after the timed steps, run k_copy_probe over 1 GiB (a plain copy kernel of known traffic: "
+                        "1 GiB read, 1 GiB written) to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE")
     return p.parse_args()
 
 
@@ -148,12 +164,12 @@ def main():
         sys.exit(2)
     r = gpu_bench(args)
     if args.copy_probe:
-        import torch
-        src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        torch.add(src, 1, out=dst)  # an elementwise kernel (a plain copy may go to the DMA engine)
-        torch.cuda.synchronize()
-        log("copy probe: 1073741824 bytes device to device")
+        import flam3_amd as F
+        L = F.lib()
+        L.fl3h_copy_probe.argtypes = [ctypes.c_longlong]
+        if L.fl3h_copy_probe(1 << 30) != 0:
+            raise RuntimeError("copy probe failed: " + F.last_error())
+        log("copy probe: k_copy_probe, 1073741824 bytes read and written")
     st = r["stage"]
     samples_per_step = r["samples"] / args.steps
     ms_per_step = 1000.0 * r["seconds"] / args.steps
@@ -173,9 +189,10 @@ def main():
                                % args.quality, "samples_per_frame": samples_per_step,
                    "frame_time_ms": ms_per_step},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": measured_traffic() if args.quality == 500.0 else None,
                      "kernel": "k_iter_fast", "kernel_ms": it_ms, "samples_per_launch": per_launch,
                      "ops_per_sample": ops,
+                     "algorithmic_bytes_per_launch": 4.0 * per_launch,
                      "note": "FP32 vector-ALU bound (no matrix work on this path); peak is MI355X FP32, "
                              "157.3 TFLOP/s for vector and dense matrix alike"},
         "stages": {

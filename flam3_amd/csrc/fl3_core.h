@@ -219,22 +219,27 @@ struct GenL {
    R b0, b1, b2, b3, rot00, rot01, rot10, rot11, rc0, rc1, ws0, wb0s0, hs1, hb1s1, final_opacity;
 };
 
+// LDS image of a genome: xforms, weights, selection breakpoints, union ids,
+// variation lists, then (only for kernels that deposit colours themselves)
+// the palette in three forms.
 template <typename R>
-static size_t lds_bytes_for(const fl3_job_hdr *h) {
+static size_t lds_bytes_for(const fl3_job_hdr *h, bool palette = true) {
    size_t n = 0;
    n += sizeof(XfL<R>) * h->nxf;
    n += sizeof(R) * h->nxf * (size_t)(h->nunion > 0 ? h->nunion : 1);
    n += sizeof(int) * (size_t)h->nrows * h->nstd;
    n += sizeof(int) * FL3_NVAR;
-   n = (n + 15) & ~(size_t)15;
-   n += sizeof(R) * 1024 + sizeof(uint32_t) * 1024 + sizeof(double) * 1024;
    n += (size_t)h->nxf * (size_t)(h->nunion > 0 ? h->nunion : 1) + (size_t)h->nxf;
-   return ((n + 15) & ~(size_t)15) + 64;
+   n = (n + 15) & ~(size_t)15;
+   if (palette) n += sizeof(R) * 1024 + sizeof(uint32_t) * 1024 + sizeof(double) * 1024;
+   return n + 64;
 }
 
 // Copy the job into LDS, converting to R.  All threads participate.
+// Without `palette` the palette is left out (the LDS image is then
+// lds_bytes_for(h, false) long).
 template <typename R>
-__device__ void load_genome(const unsigned char *job, unsigned char *lds, GenL<R> &G) {
+__device__ void load_genome(const unsigned char *job, unsigned char *lds, GenL<R> &G, bool palette = true) {
    const fl3_job_hdr *h = (const fl3_job_hdr *)job;
    G.h = h;
    G.nxf = h->nxf; G.nstd = h->nstd; G.final_index = h->final_index;
@@ -251,6 +256,8 @@ __device__ void load_genome(const unsigned char *job, unsigned char *lds, GenL<R
    R *w = (R *)(lds + off); off += sizeof(R) * G.nxf * (size_t)(G.nunion > 0 ? G.nunion : 1);
    int *st = (int *)(lds + off); off += sizeof(int) * (size_t)h->nrows * G.nstd;
    int *uid = (int *)(lds + off); off += sizeof(int) * FL3_NVAR;
+   uint8_t *vl = (uint8_t *)(lds + off); off += (size_t)G.nxf * (G.nunion > 0 ? G.nunion : 1);
+   uint8_t *vc = (uint8_t *)(lds + off); off += (size_t)G.nxf;
    off = (off + 15) & ~(size_t)15;
    R *dmap = (R *)(lds + off); off += sizeof(R) * 1024;
    uint32_t *dmapu = (uint32_t *)(lds + off); off += sizeof(uint32_t) * 1024;
@@ -281,14 +288,13 @@ __device__ void load_genome(const unsigned char *job, unsigned char *lds, GenL<R
    for (int e = tid; e < G.nxf * G.nunion; e += nt) w[e] = (R)pw[e];
    for (int e = tid; e < h->nrows * G.nstd; e += nt) st[e] = ps[e];
    for (int e = tid; e < FL3_NVAR; e += nt) uid[e] = h->union_ids[e];
-   for (int e = tid; e < 1024; e += nt) {
-      double v = pd[e];
-      dmap[e] = (R)v;
-      dmapu[e] = (uint32_t)v;
-      dmapd[e] = v;
-   }
-   uint8_t *vl = (uint8_t *)(dmapd + 1024);
-   uint8_t *vc = vl + (size_t)G.nxf * (G.nunion > 0 ? G.nunion : 1);
+   if (palette)
+      for (int e = tid; e < 1024; e += nt) {
+         double v = pd[e];
+         dmap[e] = (R)v;
+         dmapu[e] = (uint32_t)v;
+         dmapd[e] = v;
+      }
    __syncthreads();
    for (int x = tid; x < G.nxf; x += nt) {
       int k = 0;

@@ -183,6 +183,28 @@ extern "C" int fl3h_profile_get(fl3h_stage_times *out) {
    return 0;
 }
 
+extern "C" int fl3h_copy_probe(long long bytes) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   if (ctx_init()) return 1;
+   if (bytes <= 0 || bytes % 16) {
+      set_err("fl3h_copy_probe: bytes must be a positive multiple of 16, not %s%ld", "", (long)bytes);
+      return 1;
+   }
+   void *src = nullptr, *dst = nullptr;
+   HIPCHK(hipMalloc(&src, (size_t)bytes));
+   if (hipMalloc(&dst, (size_t)bytes) != hipSuccess) {
+      (void)hipFree(src);
+      set_err("fl3h_copy_probe: out of device memory");
+      return 1;
+   }
+   int rc = fl3k_launch_copy_probe(src, dst, bytes / 16, g_ctx.stream);
+   if (hipStreamSynchronize(g_ctx.stream) != hipSuccess) rc = 1;
+   (void)hipFree(src);
+   (void)hipFree(dst);
+   if (rc) set_err("fl3h_copy_probe: launch failed");
+   return rc;
+}
+
 static size_t bucket_bytes(int bits) { return bits == 64 ? 40 : 20; }
 static size_t accum_bytes(int bits) { return bits == 64 ? 32 : 16; }
 
@@ -279,6 +301,7 @@ struct JobSet {
    const long long *dev_off = nullptr;     // n > 1: byte offsets
    const unsigned long long *dev_keys = nullptr;
    size_t lds = 0;                         // largest genome LDS image
+   size_t lds_rec = 0;                     // the same without the palette (record modes)
    int tier = 0, nxf_max = 0;
    bool same_tables = true;                // one palette and visibility set for all jobs
    bool linear = false;
@@ -382,7 +405,7 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    a.blk_hist = (unsigned *)g_ctx.blkhist.p;
    a.bin_shift = shift;
    a.nbins = nbins;
-   a.hist_off = (unsigned)((js.lds + 15) & ~(size_t)15);
+   a.hist_off = (unsigned)((js.lds_rec + 15) & ~(size_t)15);
    const size_t lds_rec = a.hist_off + sizeof(unsigned) * (size_t)nbins;
    for (long long done = 0; done < p->nsamples;) {
       const long long c = p->nsamples - done < Wj * per ? p->nsamples - done : Wj * per;
@@ -425,8 +448,9 @@ extern "C" int fl3d_iterate_batch(const void *const *jobs, const size_t *job_byt
    js.n = njobs;
    for (int j = 0; j < njobs; j++) {
       hs[j] = (const fl3_job_hdr *)jobs[j];
-      const size_t l = lds_bytes_for<float>(hs[j]);
+      const size_t l = lds_bytes_for<float>(hs[j]), lr = lds_bytes_for<float>(hs[j], false);
       js.lds = l > js.lds ? l : js.lds;
+      js.lds_rec = lr > js.lds_rec ? lr : js.lds_rec;
       const int t = pick_tier(hs[j]);
       js.tier = t > js.tier ? t : js.tier;
       js.nxf_max = hs[j]->nxf > js.nxf_max ? hs[j]->nxf : js.nxf_max;
@@ -481,8 +505,8 @@ extern "C" int fl3d_iterate_batch(const void *const *jobs, const size_t *job_byt
       const bool binnable = g_ctx.bits != 64 && js.nxf_max <= 256 && shift > 0 && !(njobs > 1 && js.linear &&
                                                                                     !js.same_tables);
       if (p->deposit == 2 && !binnable) {
-         set_err("tile-binned deposit needs bits 32/33, at most 256 xforms and at most %s%ld tiles", "",
-                 (long)FL3_MAX_BINS);
+         set_err("tile-binned deposit needs bits 32/33, at most 256 xforms, at most %s%ld tiles and, for "
+                 "temporal samples with linear palettes, one palette for all", "", (long)FL3_MAX_BINS);
          return 1;
       }
       const bool binned =

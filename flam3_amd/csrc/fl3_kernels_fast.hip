@@ -325,3 +325,16 @@ extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_
    }
    return hipGetLastError() != hipSuccess;
 }
+
+// A plain copy of n16 16-byte words, one per thread: the reference kernel of
+// known traffic for calibrating hardware byte counters.
+__global__ __launch_bounds__(256) void k_copy_probe(const uint4 *src, uint4 *dst, long long n16) {
+   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n16) dst[i] = src[i];
+}
+
+extern "C" int fl3k_launch_copy_probe(const void *src, void *dst, long long n16, hipStream_t s) {
+   hipLaunchKernelGGL(k_copy_probe, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, s, (const uint4 *)src,
+                      (uint4 *)dst, n16);
+   return hipGetLastError() != hipSuccess;
+}

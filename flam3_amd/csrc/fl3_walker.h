@@ -22,7 +22,8 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
    GenL<float> G;
    const int nj = a.njobs > 1 ? a.njobs : 1;
    const int jb = nj > 1 ? (int)(blockIdx.x % nj) : 0;
-   load_genome<float>(nj > 1 ? a.jobs + a.job_off[jb] : a.job, lds, G);
+   // record modes leave the colours to the tile accumulation: no palette in LDS
+   load_genome<float>(nj > 1 ? a.jobs + a.job_off[jb] : a.job, lds, G, RECORDS == FL3_REC_DIRECT);
    unsigned *bh = (unsigned *)(lds + a.hist_off);
    if (RECORDS)
       for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) bh[b] = 0;

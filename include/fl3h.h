@@ -154,6 +154,12 @@ typedef struct {
 void fl3h_profile_reset(int enable);
 int fl3h_profile_get(fl3h_stage_times *out);
 
+/* Hardware-counter calibration (no counterpart in the reference): copies
+ * `bytes` (a multiple of 16) between two fresh device buffers with one plain
+ * copy kernel, k_copy_probe, so that a profiler's byte counters can be
+ * scaled by a kernel of known traffic (`bytes` read, `bytes` written). */
+int fl3h_copy_probe(long long bytes);
+
 const char *fl3h_last_error(void);
 
 #ifdef __cplusplus

@@ -293,6 +293,54 @@ def test_replay_animation_frame_without_de(prod, ref, t):
     assert (d > 0).mean() < 0.005, ((d > 0).mean(), d.max())
 
 
+def _batched_case(kind):
+    gs = _animated_pair(94, 0.0)
+    for g in gs:
+        g.quality = 40
+    if kind == "job_records":
+        gs[1].xforms[1].opacity = 0.5  # visibility differs between samples too
+        return gs
+    if kind == "same_palette":
+        gs[1].palette = gs[0].palette
+        return gs
+    # one control point: every temporal sample has the same tables
+    if kind == "shared_linear_wide":
+        gs[0].palette_mode = "linear"
+    return gs[:1]
+
+
+@pytest.mark.parametrize("kind", ["job_records", "same_palette", "shared_palette_narrow", "shared_linear_wide"])
+def test_batched_temporal_samples_binned_equals_atomic(prod, kind):
+    """A frame's temporal samples run as one launch.  Each sample's palette
+    (scaled by its temporal weight) and visibilities stay its own: records
+    carry the sample index when the tables differ ("job_records"), else the
+    narrow or wide records use the shared tables.  The tile-binned deposit
+    of the batch equals per-sample atomics into the same histogram."""
+    gs = _batched_case(kind)
+    a, sa = prod.render(gs, seed="batch", time=0.45, nwalkers=2048, deposit=1)
+    b, sb = prod.render(gs, seed="batch", time=0.45, nwalkers=2048, deposit=2, chunk_samples=2048 * 7 + 3)
+    assert sa["num_iters"] == sb["num_iters"]
+    d = np.abs(a.astype(np.int64) - b.astype(np.int64))
+    if kind == "shared_linear_wide":
+        assert d.max() <= 2 and (d > 0).mean() < 0.03
+    else:
+        assert d.max() == 0, d.max()
+    assert a.mean() > 1.0
+
+
+def test_batched_linear_palettes_that_differ_are_not_binned(prod):
+    """Linear palettes that change between temporal samples would need the
+    sample index and the blend fraction in one record: the batch deposits
+    with atomics, and a forced binned deposit is an error."""
+    gs = _batched_case("job_records")
+    for g in gs:
+        g.palette_mode = "linear"
+    a, _ = prod.render(gs, seed="batch", time=0.45, nwalkers=2048)
+    assert a.mean() > 1.0
+    with pytest.raises(RuntimeError):
+        prod.render(gs, seed="batch", time=0.45, nwalkers=2048, deposit=2)
+
+
 def test_production_animation_frame_within_noise_floor(prod, ref):
     gs = _animated_pair(92, 9.0)
     for g in gs:
