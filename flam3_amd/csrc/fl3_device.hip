@@ -81,8 +81,14 @@ struct Ctx {
    int ncu = 256;
    size_t lds_max = 65536;   // LDS bytes one workgroup may allocate
    hipStream_t stream = nullptr;
+   // The tile-binned deposit sorts and accumulates chunk k on bin_stream
+   // while the walkers produce chunk k + 1 on stream: records and tile
+   // counts are double-buffered (rec/blkhist for even chunks, rec2/blkhist2
+   // for odd ones).
+   hipStream_t bin_stream = nullptr;
+   hipEvent_t ev_walk[2] = {nullptr, nullptr}, ev_bin[2] = {nullptr, nullptr}, ev_join = nullptr;
    DevBuf buckets, accum, job, table, image, isaac, points, misc, findex;
-   DevBuf rec, sorted, wstate, bins, blkhist, jobtab;   // tile-binned deposit
+   DevBuf rec, rec2, sorted, wstate, bins, blkhist, blkhist2, jobtab;   // tile-binned deposit
    DevBuf srcv, srcl;                  // density estimation per-source values
    unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
    void *staging = nullptr;                   // pinned host buffer for the image download
@@ -101,6 +107,12 @@ static int ctx_init() {
    g_ctx.ncu = prop.multiProcessorCount;
    g_ctx.lds_max = prop.sharedMemPerBlock;
    HIPCHK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+   HIPCHK(hipStreamCreateWithFlags(&g_ctx.bin_stream, hipStreamNonBlocking));
+   for (int i = 0; i < 2; i++) {
+      HIPCHK(hipEventCreateWithFlags(&g_ctx.ev_walk[i], hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&g_ctx.ev_bin[i], hipEventDisableTiming));
+   }
+   HIPCHK(hipEventCreateWithFlags(&g_ctx.ev_join, hipEventDisableTiming));
    HIPCHK(hipMalloc((void **)&g_ctx.counters, 64));
    g_ctx.ready = 1;
    return 0;
@@ -135,7 +147,7 @@ static void prof_flush() {
    g_prof.n = 0;
 }
 
-static int prof_begin(int stage) {
+static int prof_begin(int stage, hipStream_t st = nullptr) {
    if (!g_prof.enabled) return -1;
    if (!g_prof.created) {
       for (int i = 0; i < 64; i++) {
@@ -147,12 +159,12 @@ static int prof_begin(int stage) {
    if (g_prof.n == 64) prof_flush();
    const int k = g_prof.n++;
    g_prof.stage[k] = stage;
-   (void)hipEventRecord(g_prof.ev[k][0], g_ctx.stream);
+   (void)hipEventRecord(g_prof.ev[k][0], st ? st : g_ctx.stream);
    return k;
 }
 
-static void prof_end(int k) {
-   if (k >= 0) (void)hipEventRecord(g_prof.ev[k][1], g_ctx.stream);
+static void prof_end(int k, hipStream_t st = nullptr) {
+   if (k >= 0) (void)hipEventRecord(g_prof.ev[k][1], st ? st : g_ctx.stream);
 }
 
 extern "C" void fl3h_profile_reset(int enable) {
@@ -339,9 +351,12 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    }
    const long long nblk = W / 256;
    const size_t rec_bytes = (narrow ? 4 : 8) * (size_t)W * per;
+   const long long nchunks = (p->nsamples + Wj * per - 1) / (Wj * per);
    if (g_ctx.rec.ensure(rec_bytes) || g_ctx.sorted.ensure(rec_bytes)) return 1;
+   if (nchunks > 1 && g_ctx.rec2.ensure(rec_bytes)) return 1;
    if (g_ctx.wstate.ensure(sizeof(uint32_t) * 10 * (size_t)W)) return 1;
    if (g_ctx.blkhist.ensure(sizeof(unsigned) * (size_t)nblk * nbins)) return 1;
+   if (nchunks > 1 && g_ctx.blkhist2.ensure(sizeof(unsigned) * (size_t)nblk * nbins)) return 1;
    const size_t bin_words = 3 * (size_t)nbins + 8;
    if (g_ctx.bins.ensure(sizeof(unsigned) * bin_words)) return 1;
    unsigned *bw = (unsigned *)g_ctx.bins.p;
@@ -400,36 +415,53 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
       b.vis_stride = vs;
    }
    a.rec_mode = narrow ? 2 : (format == 2 ? 3 : 1);
-   a.rec = (unsigned long long *)g_ctx.rec.p;
    a.wstate = (uint32_t *)g_ctx.wstate.p;
-   a.blk_hist = (unsigned *)g_ctx.blkhist.p;
    a.bin_shift = shift;
    a.nbins = nbins;
    a.hist_off = (unsigned)((js.lds_rec + 15) & ~(size_t)15);
    const size_t lds_rec = a.hist_off + sizeof(unsigned) * (size_t)nbins;
-   for (long long done = 0; done < p->nsamples;) {
+   // chunk k: walkers on the render stream, then its sort and accumulation
+   // on the bin stream, which chunk k + 2's walkers wait for before they
+   // overwrite the same record buffer
+   hipStream_t ws = g_ctx.stream, bs = g_ctx.bin_stream;
+   HIPCHK(hipEventRecord(g_ctx.ev_join, ws));   // earlier work on the histogram
+   HIPCHK(hipStreamWaitEvent(bs, g_ctx.ev_join, 0));
+   long long k = 0;
+   for (long long done = 0; done < p->nsamples; k++) {
+      const int buf = (int)(k & 1);
       const long long c = p->nsamples - done < Wj * per ? p->nsamples - done : Wj * per;
       a.nsamples = c;
       a.rows_out = (c + Wj - 1) / Wj;
       a.resume = done > 0;
-      int pk = prof_begin(ST_ITER);
-      int lrc = fl3k_launch_iter_fast(&a, (unsigned)nblk, lds_rec, g_ctx.stream, g_ctx.bits, 1, js.tier);
-      prof_end(pk);
+      a.rec = (unsigned long long *)(buf ? g_ctx.rec2.p : g_ctx.rec.p);
+      a.blk_hist = (unsigned *)(buf ? g_ctx.blkhist2.p : g_ctx.blkhist.p);
+      if (k >= 2) HIPCHK(hipStreamWaitEvent(ws, g_ctx.ev_bin[buf], 0));
+      int pk = prof_begin(ST_ITER, ws);
+      int lrc = fl3k_launch_iter_fast(&a, (unsigned)nblk, lds_rec, ws, g_ctx.bits, 1, js.tier);
+      prof_end(pk, ws);
       if (lrc) {
          set_err("k_iter_fast launch failed");
          return 1;
       }
-      HIPCHK(hipMemsetAsync(b.item_counter, 0, sizeof(unsigned), g_ctx.stream));
+      HIPCHK(hipEventRecord(g_ctx.ev_walk[buf], ws));
+      HIPCHK(hipStreamWaitEvent(bs, g_ctx.ev_walk[buf], 0));
+      HIPCHK(hipMemsetAsync(b.item_counter, 0, sizeof(unsigned), bs));
+      b.rec = a.rec;
+      b.blk_hist = a.blk_hist;
       b.rows = a.rows_out;
-      pk = prof_begin(ST_BIN);
-      lrc = fl3k_launch_bin(&b, g_ctx.ncu, format, g_ctx.stream);
-      prof_end(pk);
+      pk = prof_begin(ST_BIN, bs);
+      lrc = fl3k_launch_bin(&b, g_ctx.ncu, format, bs);
+      prof_end(pk, bs);
       if (lrc) {
          set_err("binned deposit launch failed");
          return 1;
       }
+      HIPCHK(hipEventRecord(g_ctx.ev_bin[buf], bs));
       done += c;
    }
+   // the render stream continues once every chunk is accumulated
+   HIPCHK(hipEventRecord(g_ctx.ev_join, bs));
+   HIPCHK(hipStreamWaitEvent(ws, g_ctx.ev_join, 0));
    return 0;
 }
 
