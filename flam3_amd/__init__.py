@@ -7,7 +7,7 @@ ctypes and adds genome construction and a render helper for Python callers.
 from . import abi
 from ._lib import LIB_PATH, lib, last_error
 from .genome import Genome, Xform, flame_document
-from .render import make_frame, render, render_frame
+from .render import make_frame, render, render_frame, render_many
 
 __all__ = ["abi", "LIB_PATH", "lib", "last_error", "Genome", "Xform", "flame_document", "make_frame",
            "render", "render_frame"]

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libflam3_amd.so")
 _lib = None
 
 EXPORTS = [
-    "flam3_render", "flam3_render_memory_required", "fl3h_render", "fl3h_iterate",
+    "flam3_render", "flam3_render_memory_required", "fl3h_render", "fl3h_render_many", "fl3h_iterate",
     "fl3h_xform_preview", "fl3h_deposit_points", "fl3h_density_estimate", "fl3h_tonemap",
     "fl3h_init_frame", "fl3h_genome_new", "fl3h_genome_free", "fl3h_last_error",
 ]
@@ -24,6 +24,12 @@ EXPORTS = [
 class fl3h_render_opts(ctypes.Structure):
     _fields_ = [("replay", ctypes.c_int), ("nwalkers", ctypes.c_int), ("deposit", ctypes.c_int),
                 ("chunk_samples", ctypes.c_longlong)]
+
+
+class fl3h_request(ctypes.Structure):
+    _fields_ = [("frame", ctypes.c_void_p), ("out", ctypes.c_void_p), ("field", ctypes.c_int),
+                ("nchan", ctypes.c_int), ("transp", ctypes.c_int), ("stats", ctypes.c_void_p),
+                ("rc", ctypes.c_int)]
 
 
 class fl3h_camera(ctypes.Structure):
@@ -60,6 +66,8 @@ def lib():
     L.flam3_render_memory_required.restype = cd
     L.fl3h_render.argtypes = [P(abi.flam3_frame), vp, ci, ci, ci, P(abi.stat_struct), P(fl3h_render_opts)]
     L.fl3h_render.restype = ci
+    L.fl3h_render_many.argtypes = [P(fl3h_request), ci, P(fl3h_render_opts)]
+    L.fl3h_render_many.restype = ci
     L.fl3h_iterate.argtypes = [P(abi.flam3_genome), ci, ci, vp, vp, P(abi.randctx), ci, vp]
     L.fl3h_iterate.restype = ci
     L.fl3h_xform_preview.argtypes = [P(abi.flam3_genome), ci, cd, ci, ci, vp, P(abi.randctx)]

@@ -21,6 +21,12 @@ const char *fl3d_error(void);
 int fl3d_begin(int width, int height, int bits);
 /* Zero the histogram (start of each batch, rect.c:721). */
 int fl3d_clear_buckets(void);
+/* Histograms of n renders in one allocation (cleared); fl3d_select(r) makes
+ * render r the one that fl3d_clear_buckets, fl3d_iterate*, fl3d_density,
+ * fl3d_logscale and fl3d_tonemap* work on.  fl3d_begin(w, h, bits) is the
+ * one-render case. */
+int fl3d_begin_many(const int *width, const int *height, int n, int bits);
+int fl3d_select(int r);
 
 typedef struct {
    int fuse;            /* 15, or 100 with earlyclip (rect.c:36-37, 266) */
@@ -37,6 +43,22 @@ typedef struct {
  * fast mode (the caller then repeats the batch with exact = 1). */
 int fl3d_iterate(const void *job, size_t job_bytes, const fl3d_iter_params *p,
                  double *badvals, int *saturated);
+/* Jobs of several begun renders in one launch (per-sample atomic deposits
+ * into each job's render); walkers: a multiple of 256, e.g. from
+ * fl3d_job_walkers(nsamples, n, 0) for a run of n jobs of nsamples each,
+ * which reproduces fl3d_iterate_batch's walkers.  badvals / saturated: one
+ * entry per render. */
+typedef struct {
+   const void *job;
+   size_t bytes;
+   int64_t nsamples;
+   uint64_t key[2];
+   int render;
+   int walkers;
+} fl3d_multi_job;
+long long fl3d_job_walkers(int64_t nsamples, int njobs, int requested);
+int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fuse, int exact, int nrenders,
+                       double *badvals, int *saturated);
 /* The same for njobs jobs in one launch (the temporal samples of a batch):
  * p->nsamples samples each, job j's walkers keyed by keys[2j], keys[2j+1]. */
 int fl3d_iterate_batch(const void *const *jobs, const size_t *job_bytes, int njobs, const fl3d_iter_params *p,
@@ -64,6 +86,18 @@ typedef struct {
 /* Spatial filter + tone map + quantise (rect.c:1067-1247), written to a
  * host buffer with a row stride of out_stride bytes. */
 int fl3d_tonemap(const fl3d_tm_params *p, const double *filter, void *out_host, size_t out_stride);
+/* The same in three steps for several renders: device buffers for all
+ * images and filters, one tone map per selected render at byte offsets,
+ * then one download into the callers' rows. */
+typedef struct {
+   size_t img_off;
+   void *out;
+   size_t out_stride, row_bytes;
+   int rows;
+} fl3d_image_dst;
+int fl3d_images_begin(size_t image_bytes, size_t filter_bytes);
+int fl3d_tonemap_async(const fl3d_tm_params *p, const double *filter, size_t img_off, size_t filt_off);
+int fl3d_images_fetch(const fl3d_image_dst *d, int n);
 
 /* Replay iteration: nwalkers FP64 walkers, walker w being reference thread w
  * with ISAAC state isaac[36w..36w+35] (count, 16 results, 16 memory words,

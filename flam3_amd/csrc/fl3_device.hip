@@ -94,6 +94,13 @@ struct Ctx {
    void *staging = nullptr;                   // pinned host buffer for the image download
    size_t staging_cap = 0;
    int width = 0, height = 0, bits = 33;
+   // Several renders may share the histogram buffers (fl3d_begin_many):
+   // render r's cells start at rcell0[r]; width/height/cell0 are those of
+   // the selected render, which the per-render calls work on.
+   std::vector<long long> rcell0;
+   std::vector<int> rwidth, rheight;
+   long long cell0 = 0;
+   DevBuf rcount, descs;   // per-render counters, job descriptors + workgroup table
 };
 
 static Ctx g_ctx;
@@ -218,25 +225,55 @@ extern "C" int fl3h_copy_probe(long long bytes) {
 }
 
 static size_t bucket_bytes(int bits) { return bits == 64 ? 40 : 20; }
+static size_t accum_bytes(int bits);
+// the selected render's histogram and accumulator
+static void *cur_buckets() { return (unsigned char *)g_ctx.buckets.p + g_ctx.cell0 * bucket_bytes(g_ctx.bits); }
+static void *cur_accum() { return (unsigned char *)g_ctx.accum.p + g_ctx.cell0 * accum_bytes(g_ctx.bits); }
 static size_t accum_bytes(int bits) { return bits == 64 ? 32 : 16; }
 
 extern "C" int fl3d_begin(int width, int height, int bits) {
+   return fl3d_begin_many(&width, &height, 1, bits);
+}
+
+extern "C" int fl3d_begin_many(const int *width, const int *height, int n, int bits) {
    std::lock_guard<std::recursive_mutex> lk(g_mu);
    if (ctx_init()) return 1;
-   size_t n = (size_t)width * height;
-   if (g_ctx.buckets.ensure(n * bucket_bytes(bits))) return 1;
-   if (g_ctx.accum.ensure(n * accum_bytes(bits))) return 1;
-   g_ctx.width = width;
-   g_ctx.height = height;
+   if (n < 1) {
+      set_err("fl3d_begin_many: no renders");
+      return 1;
+   }
+   g_ctx.rcell0.resize(n);
+   g_ctx.rwidth.assign(width, width + n);
+   g_ctx.rheight.assign(height, height + n);
+   size_t cells = 0;
+   for (int r = 0; r < n; r++) {
+      g_ctx.rcell0[r] = (long long)cells;
+      cells += (size_t)width[r] * height[r];
+   }
+   if (g_ctx.buckets.ensure(cells * bucket_bytes(bits) + 16)) return 1;
+   if (g_ctx.accum.ensure(cells * accum_bytes(bits) + 16)) return 1;
+   if (g_ctx.rcount.ensure(16 * (size_t)n)) return 1;
    g_ctx.bits = bits;
-   HIPCHK(hipMemsetAsync(g_ctx.accum.p, 0, n * accum_bytes(bits), g_ctx.stream));
-   HIPCHK(hipMemsetAsync(g_ctx.buckets.p, 0, n * bucket_bytes(bits), g_ctx.stream));
+   HIPCHK(hipMemsetAsync(g_ctx.accum.p, 0, cells * accum_bytes(bits), g_ctx.stream));
+   HIPCHK(hipMemsetAsync(g_ctx.buckets.p, 0, cells * bucket_bytes(bits), g_ctx.stream));
+   return fl3d_select(0);
+}
+
+extern "C" int fl3d_select(int r) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   if (r < 0 || r >= (int)g_ctx.rcell0.size()) {
+      set_err("fl3d_select: no render %s%ld", "", (long)r);
+      return 1;
+   }
+   g_ctx.cell0 = g_ctx.rcell0[r];
+   g_ctx.width = g_ctx.rwidth[r];
+   g_ctx.height = g_ctx.rheight[r];
    return 0;
 }
 
 extern "C" int fl3d_clear_buckets(void) {
    size_t n = (size_t)g_ctx.width * g_ctx.height;
-   HIPCHK(hipMemsetAsync(g_ctx.buckets.p, 0, n * bucket_bytes(g_ctx.bits), g_ctx.stream));
+   HIPCHK(hipMemsetAsync(cur_buckets(), 0, n * bucket_bytes(g_ctx.bits), g_ctx.stream));
    return 0;
 }
 
@@ -265,6 +302,15 @@ static long long choose_walkers(long long nsamples, int requested) {
    if (w < 256) w = 256;
    return (w + 255) / 256 * 256;
 }
+
+// Walkers of one job among njobs that share nsamples, as fl3d_iterate_batch
+// splits them: a multiple of 256.
+static long long job_walkers(long long nsamples, int njobs, int requested) {
+   long long W = choose_walkers(nsamples * njobs, requested);
+   long long Wj = (W / njobs) / 256 * 256;
+   return Wj < 256 ? 256 : Wj;
+}
+
 
 // Smallest walker-kernel tier holding every variation the genome uses.
 static int pick_tier(const fl3_job_hdr *h) {
@@ -521,7 +567,7 @@ extern "C" int fl3d_iterate_batch(const void *const *jobs, const size_t *job_byt
       a.job_off = js.dev_off;
       a.keys = js.dev_keys;
       a.njobs = njobs;
-      a.buckets = g_ctx.buckets.p;
+      a.buckets = cur_buckets();
       a.nsamples = p->nsamples;
       a.fuse = p->fuse;
       a.key0 = keys[0];
@@ -529,10 +575,8 @@ extern "C" int fl3d_iterate_batch(const void *const *jobs, const size_t *job_byt
       a.badvals = g_ctx.counters;
       a.sat = (uint32_t *)(g_ctx.counters + 1);
       // walkers: a multiple of 256 per job
-      long long W = choose_walkers(p->nsamples * njobs, p->nwalkers);
-      long long Wj = (W / njobs) / 256 * 256;
-      if (Wj < 256) Wj = 256;
-      W = Wj * njobs;
+      const long long Wj = job_walkers(p->nsamples, njobs, p->nwalkers);
+      const long long W = Wj * njobs;
       const int shift = bin_shift();
       const bool binnable = g_ctx.bits != 64 && js.nxf_max <= 256 && shift > 0 && !(njobs > 1 && js.linear &&
                                                                                     !js.same_tables);
@@ -560,6 +604,110 @@ extern "C" int fl3d_iterate_batch(const void *const *jobs, const size_t *job_byt
    return read_counters(badvals, saturated);
 }
 
+extern "C" long long fl3d_job_walkers(int64_t nsamples, int njobs, int requested) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   if (ctx_init()) return -1;
+   return job_walkers(nsamples, njobs > 0 ? njobs : 1, requested);
+}
+
+// Jobs of several renders (begun with fl3d_begin_many) in one launch with
+// per-sample atomic deposits into each job's render.  A run of consecutive
+// jobs of one render with equal sample counts is what fl3d_iterate_batch
+// would launch for that render alone: it gets that launch's kernel tier, so
+// its arithmetic is the same; jobs of different tiers go to one launch per
+// tier.
+extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fuse, int exact, int nrenders,
+                                  double *badvals, int *saturated) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   if (njobs < 1 || nrenders < 1 || nrenders > (int)g_ctx.rcell0.size()) {
+      set_err("fl3d_iterate_multi: %s%ld jobs for the begun renders", "", (long)njobs);
+      return 1;
+   }
+   size_t total = 0, lds = 0;
+   long long samples = 0;
+   std::vector<int> tier((size_t)njobs);
+   for (int j = 0; j < njobs;) {
+      int e = j, t = 0;
+      while (e < njobs && e - j < 4096 && jobs[e].render == jobs[j].render && jobs[e].nsamples == jobs[j].nsamples) {
+         const int te = pick_tier((const fl3_job_hdr *)jobs[e].job);
+         t = te > t ? te : t;
+         e++;
+      }
+      for (int k = j; k < e; k++) tier[k] = t;
+      j = e;
+   }
+   std::vector<JobDesc> d((size_t)njobs);
+   long long gblocks[FL3_TIER_FULL + 1] = {0};
+   for (int j = 0; j < njobs; j++) {
+      const fl3_job_hdr *h = (const fl3_job_hdr *)jobs[j].job;
+      const size_t l = lds_bytes_for<float>(h);
+      lds = l > lds ? l : lds;
+      if (jobs[j].walkers < 256 || jobs[j].walkers % 256 || jobs[j].render < 0 || jobs[j].render >= nrenders) {
+         set_err("fl3d_iterate_multi: job %s%ld needs a multiple of 256 walkers and a begun render", "", (long)j);
+         return 1;
+      }
+      memset(&d[j], 0, sizeof(JobDesc));
+      d[j].off = (long long)total;
+      d[j].nsamples = jobs[j].nsamples;
+      d[j].key0 = jobs[j].key[0];
+      d[j].key1 = jobs[j].key[1];
+      d[j].cell0 = g_ctx.rcell0[jobs[j].render];
+      d[j].block0 = (int)gblocks[tier[j]];   // within its tier's launch
+      d[j].nblocks = jobs[j].walkers / 256;
+      d[j].render = jobs[j].render;
+      gblocks[tier[j]] += d[j].nblocks;
+      samples += jobs[j].nsamples;
+      total += (jobs[j].bytes + 255) & ~(size_t)255;
+   }
+   long long gstart[FL3_TIER_FULL + 2] = {0};
+   for (int t = 0; t <= FL3_TIER_FULL; t++) {
+      if (gblocks[t] > 0x7fffffffLL) {
+         set_err("fl3d_iterate_multi: too many workgroups");
+         return 1;
+      }
+      gstart[t + 1] = gstart[t] + gblocks[t];
+   }
+   const long long blocks = gstart[FL3_TIER_FULL + 1];
+   std::vector<int> blk((size_t)blocks);
+   for (int j = 0; j < njobs; j++)
+      for (int b = 0; b < d[j].nblocks; b++) blk[(size_t)(gstart[tier[j]] + d[j].block0 + b)] = j;
+   const size_t dbytes = sizeof(JobDesc) * njobs, bbytes = sizeof(int) * (size_t)blocks;
+   std::vector<unsigned char> host(total + dbytes + bbytes);
+   for (int j = 0; j < njobs; j++) memcpy(host.data() + d[j].off, jobs[j].job, jobs[j].bytes);
+   memcpy(host.data() + total, d.data(), dbytes);
+   memcpy(host.data() + total + dbytes, blk.data(), bbytes);
+   if (g_ctx.job.ensure(host.size())) return 1;
+   HIPCHK(hipMemcpyAsync(g_ctx.job.p, host.data(), host.size(), hipMemcpyHostToDevice, g_ctx.stream));
+   HIPCHK(hipMemsetAsync(g_ctx.rcount.p, 0, 16 * (size_t)nrenders, g_ctx.stream));
+   IterArgs a;
+   memset(&a, 0, sizeof(a));
+   a.jobs = (const unsigned char *)g_ctx.job.p;
+   a.descs = (const JobDesc *)(a.jobs + total);
+   a.buckets = g_ctx.buckets.p;
+   a.badvals = (unsigned long long *)g_ctx.rcount.p;
+   a.fuse = fuse;
+   g_prof.samples += samples;
+   for (int t = 0; t <= FL3_TIER_FULL; t++) {
+      if (!gblocks[t]) continue;
+      a.blk_job = (const int *)(a.jobs + total + dbytes) + gstart[t];
+      const int pk = prof_begin(ST_ITER);
+      const int lrc = fl3k_launch_iter_fast(&a, (unsigned)gblocks[t], lds, g_ctx.stream, g_ctx.bits, exact, t);
+      prof_end(pk);
+      if (lrc) {
+         set_err("k_iter_fast launch failed");
+         return 1;
+      }
+   }
+   std::vector<unsigned long long> c(2 * (size_t)nrenders);
+   HIPCHK(hipMemcpyAsync(c.data(), g_ctx.rcount.p, 16 * (size_t)nrenders, hipMemcpyDeviceToHost, g_ctx.stream));
+   HIPCHK(hipStreamSynchronize(g_ctx.stream));   // also keeps host alive for the upload
+   for (int r = 0; r < nrenders; r++) {
+      if (badvals) badvals[r] = (double)c[2 * r];
+      if (saturated) saturated[r] = (c[2 * r + 1] & 0xffffffffull) != 0;
+   }
+   return 0;
+}
+
 extern "C" int fl3d_iterate(const void *job, size_t job_bytes, const fl3d_iter_params *p, double *badvals,
                             int *saturated) {
    const void *jobs[1] = {job};
@@ -580,7 +728,7 @@ static int iterate_replay(const void *job, size_t job_bytes, uint32_t *isaac_hos
    IterArgs a;
    memset(&a, 0, sizeof(a));
    a.job = (const unsigned char *)g_ctx.job.p;
-   a.buckets = g_ctx.buckets.p;
+   a.buckets = cur_buckets();
    a.fuse = fuse;
    a.badvals = g_ctx.counters;
    a.sat = (uint32_t *)(g_ctx.counters + 1);
@@ -623,8 +771,8 @@ extern "C" int fl3d_density(const fl3d_de_params *p, const fl3_de_filters *de) {
    double *coefs, *widths;
    float *coefs_f;
    if (upload_de(de, &coefs, &widths, &coefs_f)) return 1;
-   a.buckets = g_ctx.buckets.p;
-   a.accum = g_ctx.accum.p;
+   a.buckets = cur_buckets();
+   a.accum = cur_accum();
    a.coefs = coefs;
    a.widths = widths;
    a.coefs_t = g_ctx.bits == 64 ? (const void *)coefs : (const void *)coefs_f;
@@ -670,7 +818,7 @@ extern "C" int fl3d_logscale(double k1, double k2) {
    std::lock_guard<std::recursive_mutex> lk(g_mu);
    const long n = (long)g_ctx.width * g_ctx.height;
    const int pk = prof_begin(ST_DE);
-   const int lrc = fl3k_launch_logscale(g_ctx.buckets.p, g_ctx.accum.p, n, k1, k2, g_ctx.stream, g_ctx.bits);
+   const int lrc = fl3k_launch_logscale(cur_buckets(), cur_accum(), n, k1, k2, g_ctx.stream, g_ctx.bits);
    prof_end(pk);
    if (lrc) {
       set_err("k_logscale launch failed");
@@ -679,19 +827,31 @@ extern "C" int fl3d_logscale(double k1, double k2) {
    return 0;
 }
 
-extern "C" int fl3d_tonemap(const fl3d_tm_params *p, const double *filter, void *out_host, size_t out_stride) {
+// Images of one or more renders: each tone map writes its image at an offset
+// of the device image buffer; one copy brings them all back.
+extern "C" int fl3d_images_begin(size_t image_bytes, size_t filter_bytes) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   if (g_ctx.image.ensure(image_bytes > 0 ? image_bytes : 1)) return 1;
+   if (g_ctx.misc.ensure(filter_bytes > 0 ? filter_bytes : 1)) return 1;
+   return 0;
+}
+
+extern "C" int fl3d_tonemap_async(const fl3d_tm_params *p, const double *filter, size_t img_off, size_t filt_off) {
    std::lock_guard<std::recursive_mutex> lk(g_mu);
    const size_t row = (size_t)p->image_width * p->nchan * p->bytes_per_channel;
    const size_t img = row * p->image_height;
    const size_t fb = sizeof(double) * p->filter_width * p->filter_width;
-   if (g_ctx.image.ensure(img > 0 ? img : 1)) return 1;
-   if (g_ctx.misc.ensure(fb)) return 1;
-   HIPCHK(hipMemcpyAsync(g_ctx.misc.p, filter, fb, hipMemcpyHostToDevice, g_ctx.stream));
+   if (img_off + img > g_ctx.image.cap || filt_off + fb > g_ctx.misc.cap) {
+      set_err("fl3d_tonemap_async: image or filter outside the buffers of fl3d_images_begin");
+      return 1;
+   }
+   double *dfilt = (double *)((unsigned char *)g_ctx.misc.p + filt_off);
+   HIPCHK(hipMemcpyAsync(dfilt, filter, fb, hipMemcpyHostToDevice, g_ctx.stream));
    TMArgs a;
    memset(&a, 0, sizeof(a));
-   a.accum = g_ctx.accum.p;
-   a.filter = (const double *)g_ctx.misc.p;
-   a.out = (unsigned char *)g_ctx.image.p;
+   a.accum = cur_accum();
+   a.filter = dfilt;
+   a.out = (unsigned char *)g_ctx.image.p + img_off;
    a.W = g_ctx.width;
    a.iw = p->image_width;
    a.ih = p->image_height;
@@ -709,7 +869,7 @@ extern "C" int fl3d_tonemap(const fl3d_tm_params *p, const double *filter, void 
    for (int k = 0; k < 3; k++) a.bg[k] = p->background[k];
    const long n = (long)g_ctx.width * g_ctx.height;
    if (p->earlyclip) {
-      if (fl3k_launch_earlyclip(g_ctx.accum.p, n, &a, g_ctx.stream, g_ctx.bits)) {
+      if (fl3k_launch_earlyclip(cur_accum(), n, &a, g_ctx.stream, g_ctx.bits)) {
          set_err("k_earlyclip launch failed");
          return 1;
       }
@@ -722,37 +882,67 @@ extern "C" int fl3d_tonemap(const fl3d_tm_params *p, const double *filter, void 
          set_err("k_tonemap launch failed");
          return 1;
       }
-      // one copy into pinned staging, then rows into the caller's buffer
-      // (a strided copy straight to pageable memory goes row by row)
-      if (g_ctx.staging_cap < img) {
-         if (g_ctx.staging) (void)hipHostFree(g_ctx.staging);
-         g_ctx.staging = nullptr;
-         g_ctx.staging_cap = 0;
-         HIPCHK(hipHostMalloc(&g_ctx.staging, img, hipHostMallocDefault));
-         g_ctx.staging_cap = img;
-      }
-      HIPCHK(hipMemcpyAsync(g_ctx.staging, g_ctx.image.p, img, hipMemcpyDeviceToHost, g_ctx.stream));
-      HIPCHK(hipStreamSynchronize(g_ctx.stream));
-      const unsigned char *src = (const unsigned char *)g_ctx.staging;
-      unsigned char *dst = (unsigned char *)out_host;
-      if (out_stride == row)
-         memcpy(dst, src, img);
-      else
-         for (int j = 0; j < p->image_height; j++) memcpy(dst + (size_t)j * out_stride, src + (size_t)j * row, row);
-      return 0;
    }
-   HIPCHK(hipStreamSynchronize(g_ctx.stream));
    return 0;
 }
 
+extern "C" int fl3d_images_fetch(const fl3d_image_dst *d, int n) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   size_t end = 0;
+   for (int i = 0; i < n; i++) {
+      const size_t e = d[i].img_off + d[i].row_bytes * (size_t)d[i].rows;
+      end = e > end ? e : end;
+   }
+   if (end == 0) {
+      HIPCHK(hipStreamSynchronize(g_ctx.stream));
+      return 0;
+   }
+   // one copy into pinned staging, then rows into the callers' buffers
+   // (a strided copy straight to pageable memory goes row by row)
+   if (g_ctx.staging_cap < end) {
+      if (g_ctx.staging) (void)hipHostFree(g_ctx.staging);
+      g_ctx.staging = nullptr;
+      g_ctx.staging_cap = 0;
+      HIPCHK(hipHostMalloc(&g_ctx.staging, end, hipHostMallocDefault));
+      g_ctx.staging_cap = end;
+   }
+   HIPCHK(hipMemcpyAsync(g_ctx.staging, g_ctx.image.p, end, hipMemcpyDeviceToHost, g_ctx.stream));
+   HIPCHK(hipStreamSynchronize(g_ctx.stream));
+   for (int i = 0; i < n; i++) {
+      const unsigned char *src = (const unsigned char *)g_ctx.staging + d[i].img_off;
+      unsigned char *dst = (unsigned char *)d[i].out;
+      if (d[i].out_stride == d[i].row_bytes)
+         memcpy(dst, src, d[i].row_bytes * (size_t)d[i].rows);
+      else
+         for (int j = 0; j < d[i].rows; j++)
+            memcpy(dst + (size_t)j * d[i].out_stride, src + (size_t)j * d[i].row_bytes, d[i].row_bytes);
+   }
+   return 0;
+}
+
+extern "C" int fl3d_tonemap(const fl3d_tm_params *p, const double *filter, void *out_host, size_t out_stride) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   fl3d_image_dst d;
+   d.img_off = 0;
+   d.out = out_host;
+   d.out_stride = out_stride;
+   d.row_bytes = (size_t)p->image_width * p->nchan * p->bytes_per_channel;
+   d.rows = p->image_width > 0 ? p->image_height : 0;
+   if (fl3d_images_begin(d.row_bytes * (size_t)p->image_height,
+                         sizeof(double) * p->filter_width * p->filter_width))
+      return 1;
+   if (fl3d_tonemap_async(p, filter, 0, 0)) return 1;
+   return fl3d_images_fetch(&d, 1);
+}
+
 extern "C" int fl3d_download_buckets(void *host, size_t bytes) {
-   HIPCHK(hipMemcpyAsync(host, g_ctx.buckets.p, bytes, hipMemcpyDeviceToHost, g_ctx.stream));
+   HIPCHK(hipMemcpyAsync(host, cur_buckets(), bytes, hipMemcpyDeviceToHost, g_ctx.stream));
    HIPCHK(hipStreamSynchronize(g_ctx.stream));
    return 0;
 }
 
 extern "C" int fl3d_download_accum(void *host, size_t bytes) {
-   HIPCHK(hipMemcpyAsync(host, g_ctx.accum.p, bytes, hipMemcpyDeviceToHost, g_ctx.stream));
+   HIPCHK(hipMemcpyAsync(host, cur_accum(), bytes, hipMemcpyDeviceToHost, g_ctx.stream));
    HIPCHK(hipStreamSynchronize(g_ctx.stream));
    return 0;
 }
@@ -984,6 +1174,7 @@ extern "C" int fl3h_density_estimate(const void *buckets, int width, int height,
    g_ctx.width = width;
    g_ctx.height = height;
    g_ctx.bits = bits;
+   g_ctx.cell0 = 0;
    HIPCHK(hipMemcpyAsync(g_ctx.buckets.p, buckets, cells * bucket_bytes(bits), hipMemcpyHostToDevice, g_ctx.stream));
    HIPCHK(hipMemcpyAsync(g_ctx.accum.p, accum, cells * accum_bytes(bits), hipMemcpyHostToDevice, g_ctx.stream));
    fl3_de_filters de;
@@ -1018,6 +1209,7 @@ extern "C" int fl3h_tonemap(const void *accum, int width, int height, int bits, 
    g_ctx.width = width;
    g_ctx.height = height;
    g_ctx.bits = bits;
+   g_ctx.cell0 = 0;
    HIPCHK(hipMemcpyAsync(g_ctx.accum.p, accum, cells * accum_bytes(bits), hipMemcpyHostToDevice, g_ctx.stream));
    fl3d_tm_params p;
    p.image_width = tp->image_width;

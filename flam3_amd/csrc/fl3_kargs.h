@@ -4,8 +4,22 @@
 #define FL3_KARGS_H
 #include <stdint.h>
 
+// Independent renders in one launch (per-sample atomic deposits): the
+// workgroups of a job are contiguous, workgroup b runs descs[blk_job[b]].
+struct JobDesc {
+   long long off;              // byte offset of the job in IterArgs::jobs
+   long long nsamples;         // plotted samples of the job
+   unsigned long long key0, key1;
+   long long cell0;            // first histogram cell of the job's render
+   int block0, nblocks;        // the job's workgroups
+   int render;                 // counters of the render: badvals[2 render], saturation [2 render + 1]
+   int pad;
+};
+
 struct IterArgs {
    const unsigned char *job;
+   const JobDesc *descs;       // non-null: independent renders (direct deposit only)
+   const int *blk_job;
    // several jobs (the temporal samples of a batch) in one launch: workgroup
    // b iterates job b % njobs with walker key keys[2 * job .. +1]; nsamples
    // is then per job

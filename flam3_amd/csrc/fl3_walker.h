@@ -20,10 +20,12 @@ template <int BITS, bool EXACT, int RECORDS, int TIER>
 __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
    unsigned char *lds = fl3_lds;
    GenL<float> G;
-   const int nj = a.njobs > 1 ? a.njobs : 1;
+   const JobDesc *dsc = RECORDS == FL3_REC_DIRECT && a.descs ? a.descs + a.blk_job[blockIdx.x] : nullptr;
+   const int nj = !dsc && a.njobs > 1 ? a.njobs : 1;
    const int jb = nj > 1 ? (int)(blockIdx.x % nj) : 0;
    // record modes leave the colours to the tile accumulation: no palette in LDS
-   load_genome<float>(nj > 1 ? a.jobs + a.job_off[jb] : a.job, lds, G, RECORDS == FL3_REC_DIRECT);
+   load_genome<float>(dsc ? a.jobs + dsc->off : (nj > 1 ? a.jobs + a.job_off[jb] : a.job), lds, G,
+                      RECORDS == FL3_REC_DIRECT);
    unsigned *bh = (unsigned *)(lds + a.hist_off);
    if (RECORDS)
       for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) bh[b] = 0;
@@ -32,9 +34,17 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
    // Wj, wj: this walker's job-local count and index
    const long long W = (long long)gridDim.x * blockDim.x;
    const long long wid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-   const long long Wj = (long long)(gridDim.x / nj) * blockDim.x;
-   const long long wj = (long long)(blockIdx.x / nj) * blockDim.x + threadIdx.x;
-   const long long n = a.nsamples / Wj + (wj < a.nsamples % Wj ? 1 : 0);
+   const long long Wj = dsc ? (long long)dsc->nblocks * blockDim.x : (long long)(gridDim.x / nj) * blockDim.x;
+   const long long wj = dsc ? (long long)(blockIdx.x - dsc->block0) * blockDim.x + threadIdx.x
+                            : (long long)(blockIdx.x / nj) * blockDim.x + threadIdx.x;
+   const long long nsj = dsc ? dsc->nsamples : a.nsamples;
+   const long long n = nsj / Wj + (wj < nsj % Wj ? 1 : 0);
+   // deposit target and counters: the job's render
+   void *const buckets = !dsc ? a.buckets
+                         : BITS == 64 ? (void *)((double *)a.buckets + dsc->cell0 * 5)
+                                      : (void *)((uint32_t *)a.buckets + dsc->cell0 * 5);
+   unsigned long long *const badp = dsc ? a.badvals + 2 * dsc->render : a.badvals;
+   uint32_t *const satp = dsc ? (uint32_t *)(a.badvals + 2 * dsc->render + 1) : a.sat;
    WalkerRng rng;
    float p[4], q[4];
    int lastxf = 0, consec = 0;
@@ -48,7 +58,9 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
       lastxf = (int)ws[8 * W + wid];
       consec = (int)ws[9 * W + wid];
    } else {
-      if (nj > 1)
+      if (dsc)
+         rng.seed(dsc->key0, dsc->key1, (uint64_t)wj);
+      else if (nj > 1)
          rng.seed(a.keys[2 * jb], a.keys[2 * jb + 1], (uint64_t)wj);
       else
          rng.seed(a.key0, a.key1, (uint64_t)wid);
@@ -98,7 +110,7 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
                                                                      : (cell | ((unsigned)(v >> 32) & 0xffu) << 24);
             if (cell != FL3_REC_NONE) atomicAdd(&bh[cell >> a.bin_shift], 1u);
          } else
-            deposit<float, BITS, EXACT>(G, q, a.buckets, a.sat);
+            deposit<float, BITS, EXACT>(G, q, buckets, satp);
       }
       i++;
    }
@@ -120,7 +132,7 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
       ws[8 * W + wid] = (uint32_t)lastxf;
       ws[9 * W + wid] = (uint32_t)consec;
    }
-   add_bad(a.badvals, bad);
+   add_bad(badp, bad);
 }
 
 // One FP32 application of xform xi to each of n points (x, y, colour) in the

@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 
 from . import abi
-from ._lib import check, fl3h_render_opts, lib
+from ._lib import check, fl3h_render_opts, fl3h_request, last_error, lib
 from .genome import Genome, genome_array
 
 
@@ -62,6 +62,48 @@ def render(genomes, *, nchan=3, transp=0, bits=33, bpc=1, earlyclip=0, time=0.0,
         return render_frame(f, genomes[0].width, genomes[0].height, nchan=nchan, transp=transp,
                             field=field, replay=replay, nwalkers=nwalkers, deposit=deposit,
                             chunk_samples=chunk_samples)
+    finally:
+        for gp in owned:
+            lib().fl3h_genome_free(gp)
+
+
+def render_many(items, *, nwalkers=0, deposit=0):
+    """Render several independent images with fl3h_render_many (small ones
+    share the GPU).  items: dicts with the keys of ``render`` (``genomes``
+    required).  Returns [(image, stats)] in order; raises if any failed."""
+    frames, imgs, stats, owned, arrs = [], [], [], [], []
+    try:
+        reqs = (fl3h_request * len(items))()
+        for i, it in enumerate(items):
+            gs = it["genomes"]
+            if isinstance(gs, Genome):
+                gs = [gs]
+            arr, own = genome_array(gs)
+            arrs.append(arr)
+            owned.extend(own)
+            f = make_frame(ctypes.addressof(arr), len(gs), bits=it.get("bits", 33), bpc=it.get("bpc", 1),
+                           earlyclip=it.get("earlyclip", 0), time=it.get("time", 0.0),
+                           nthreads=it.get("nthreads", 1), sub_batch_size=it.get("sub_batch_size", 10000),
+                           seed=it.get("seed", "flam3"))
+            frames.append(f)
+            nchan = it.get("nchan", 3)
+            dtype = np.uint16 if it.get("bpc", 1) == 2 else np.uint8
+            img = np.zeros((gs[0].height, gs[0].width, nchan), dtype=dtype)
+            imgs.append(img)
+            st = abi.stat_struct()
+            stats.append(st)
+            reqs[i].frame = ctypes.cast(ctypes.byref(f), ctypes.c_void_p)
+            reqs[i].out = img.ctypes.data_as(ctypes.c_void_p)
+            reqs[i].field = it.get("field", 0)
+            reqs[i].nchan = nchan
+            reqs[i].transp = it.get("transp", 0)
+            reqs[i].stats = ctypes.cast(ctypes.byref(st), ctypes.c_void_p)
+        opts = fl3h_render_opts(0, int(nwalkers), int(deposit), 0)
+        nfail = lib().fl3h_render_many(reqs, len(items), ctypes.byref(opts))
+        if nfail:
+            raise RuntimeError("fl3h_render_many: %d of %d renders failed: %s" % (nfail, len(items), last_error()))
+        return [(img, {"badvals": st.badvals, "num_iters": st.num_iters, "render_seconds": st.render_seconds})
+                for img, st in zip(imgs, stats)]
     finally:
         for gp in owned:
             lib().fl3h_genome_free(gp)

@@ -42,6 +42,29 @@ typedef struct {
                       a multiple of the walker count), 0 = library choice */
 } fl3h_render_opts;
 
+/* One render of fl3h_render_many: flam3_render's arguments and its return
+ * value (rc). */
+typedef struct {
+   flam3_frame *frame;
+   void *out;
+   int field, nchan, transp;
+   stat_struct *stats;
+   int rc;
+} fl3h_request;
+
+/* Renders n frames, each exactly as fl3h_render(frame, out, field, nchan,
+ * transp, stats, opts) would (same image, stats and f->rc), with small
+ * renders sharing the GPU: the walkers of all temporal samples of all
+ * renders below 2^24 plotted samples (one batch, production mode, the bits
+ * of the first such render, no progress callback) run in one launch, then
+ * their density estimation and tone maps, and their images come back in one
+ * copy.  Other renders run one after another.  Returns the number of
+ * requests whose rc is not 0.  No counterpart in the reference: the
+ * preview loop of flam3-genome (flam3-genome.c:1020-1116) and
+ * flam3-animate's frame loop (flam3-animate.c:225-327) call flam3_render
+ * once per image. */
+int fl3h_render_many(fl3h_request *reqs, int n, const fl3h_render_opts *opts);
+
 /* flam3_render with options; flam3_render(f, ...) == fl3h_render(f, ..., NULL). */
 int fl3h_render(flam3_frame *f, void *out, int field, int nchan, int transp, stat_struct *stats,
                 const fl3h_render_opts *opts);
