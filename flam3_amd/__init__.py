@@ -8,6 +8,7 @@ from . import abi
 from ._lib import LIB_PATH, lib, last_error
 from .genome import Genome, Xform, flame_document
 from .render import make_frame, render, render_frame, render_many
+from .serve import Server
 
 __all__ = ["abi", "LIB_PATH", "lib", "last_error", "Genome", "Xform", "flame_document", "make_frame",
            "render", "render_frame"]

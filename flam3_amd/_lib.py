@@ -15,7 +15,9 @@ LIB_PATH = os.path.join(_HERE, "libflam3_amd.so")
 _lib = None
 
 EXPORTS = [
-    "flam3_render", "flam3_render_memory_required", "fl3h_render", "fl3h_render_many", "fl3h_iterate",
+    "flam3_render", "flam3_render_memory_required", "fl3h_render", "fl3h_render_many", "fl3h_server_create",
+    "fl3h_server_submit", "fl3h_server_wait", "fl3h_server_done", "fl3h_server_counts", "fl3h_server_destroy",
+    "fl3h_iterate",
     "fl3h_xform_preview", "fl3h_deposit_points", "fl3h_density_estimate", "fl3h_tonemap",
     "fl3h_init_frame", "fl3h_genome_new", "fl3h_genome_free", "fl3h_last_error",
 ]
@@ -30,6 +32,10 @@ class fl3h_request(ctypes.Structure):
     _fields_ = [("frame", ctypes.c_void_p), ("out", ctypes.c_void_p), ("field", ctypes.c_int),
                 ("nchan", ctypes.c_int), ("transp", ctypes.c_int), ("stats", ctypes.c_void_p),
                 ("rc", ctypes.c_int)]
+
+
+class fl3h_server_opts(ctypes.Structure):
+    _fields_ = [("hist_budget_bytes", ctypes.c_double), ("max_batch", ctypes.c_int), ("opts", fl3h_render_opts)]
 
 
 class fl3h_camera(ctypes.Structure):
@@ -68,6 +74,18 @@ def lib():
     L.fl3h_render.restype = ci
     L.fl3h_render_many.argtypes = [P(fl3h_request), ci, P(fl3h_render_opts)]
     L.fl3h_render_many.restype = ci
+    L.fl3h_server_create.argtypes = [P(fl3h_server_opts)]
+    L.fl3h_server_create.restype = vp
+    L.fl3h_server_submit.argtypes = [vp, P(abi.flam3_frame), vp, ci, ci, ci, P(abi.stat_struct)]
+    L.fl3h_server_submit.restype = ctypes.c_longlong
+    L.fl3h_server_wait.argtypes = [vp, ctypes.c_longlong, P(cd)]
+    L.fl3h_server_wait.restype = ci
+    L.fl3h_server_done.argtypes = [vp, ctypes.c_longlong]
+    L.fl3h_server_done.restype = ci
+    L.fl3h_server_counts.argtypes = [vp, P(ctypes.c_longlong), P(ctypes.c_longlong), P(cd), P(cd)]
+    L.fl3h_server_counts.restype = None
+    L.fl3h_server_destroy.argtypes = [vp]
+    L.fl3h_server_destroy.restype = None
     L.fl3h_iterate.argtypes = [P(abi.flam3_genome), ci, ci, vp, vp, P(abi.randctx), ci, vp]
     L.fl3h_iterate.restype = ci
     L.fl3h_xform_preview.argtypes = [P(abi.flam3_genome), ci, cd, ci, ci, vp, P(abi.randctx)]

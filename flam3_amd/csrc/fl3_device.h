@@ -47,7 +47,7 @@ int fl3d_iterate(const void *job, size_t job_bytes, const fl3d_iter_params *p,
  * into each job's render); walkers: a multiple of 256, e.g. from
  * fl3d_job_walkers(nsamples, n, 0) for a run of n jobs of nsamples each,
  * which reproduces fl3d_iterate_batch's walkers.  badvals / saturated: one
- * entry per render. */
+ * entry per render.  exact: every job deposits with saturating adds. */
 typedef struct {
    const void *job;
    size_t bytes;
@@ -55,6 +55,7 @@ typedef struct {
    uint64_t key[2];
    int render;
    int walkers;
+   int exact;       /* saturating deposits (a render that could reach 2^32 in a cell) */
 } fl3d_multi_job;
 long long fl3d_job_walkers(int64_t nsamples, int njobs, int requested);
 int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fuse, int exact, int nrenders,

@@ -22,6 +22,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <atomic>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -105,6 +106,7 @@ struct Ctx {
 
 static Ctx g_ctx;
 static std::recursive_mutex g_mu;
+static std::atomic<int> g_ready{0};   // g_ctx.ready, for readers without g_mu
 
 static int ctx_init() {
    if (g_ctx.ready) return 0;
@@ -122,6 +124,7 @@ static int ctx_init() {
    HIPCHK(hipEventCreateWithFlags(&g_ctx.ev_join, hipEventDisableTiming));
    HIPCHK(hipMalloc((void **)&g_ctx.counters, 64));
    g_ctx.ready = 1;
+   g_ready.store(1, std::memory_order_release);
    return 0;
 }
 
@@ -330,8 +333,9 @@ static int pick_tier(const fl3_job_hdr *h) {
 
 // Tile shift of the binned deposit for the current histogram (cells per
 // tile = 1 << shift), or 0 when the histogram is too large for it.
-static int bin_shift() {
-   const long long ncells = (long long)g_ctx.width * g_ctx.height;
+static int bin_shift_cells(long long ncells);
+static int bin_shift() { return bin_shift_cells((long long)g_ctx.width * g_ctx.height); }
+static int bin_shift_cells(long long ncells) {
    // 4096-cell tiles when LDS allows (fewer, larger scatter runs), else 2048
    for (int sh = 12; sh >= 11; sh--)
       if (((ncells + (1LL << sh) - 1) >> sh) <= FL3_MAX_BINS && fl3k_bin_lds_bytes(sh) <= g_ctx.lds_max) return sh;
@@ -374,6 +378,50 @@ static bool same_tables(const fl3_job_hdr *a, const fl3_job_hdr *b) {
    for (int e = 0; e < a->nxf; e++)
       if (pa[e].vis != pb[e].vis) return false;
    return true;
+}
+
+// Palette (as integers and floats) and visibilities of every job of a
+// launch, for the accumulation of job records.
+static int upload_job_tables(const fl3_job_hdr *const *h, int n, int nxf_max, BinArgs *b) {
+   const int vs = nxf_max > 0 ? nxf_max : 1;
+   std::vector<uint32_t> du((size_t)n * 1024);
+   std::vector<float> df((size_t)n * 1024), vis((size_t)n * vs, 1.0f);
+   for (int j = 0; j < n; j++) {
+      const unsigned char *hb = (const unsigned char *)h[j];
+      const double *dm = (const double *)(hb + h[j]->off_dmap);
+      const fl3_xform_pack *px = (const fl3_xform_pack *)(hb + h[j]->off_xf);
+      for (int e = 0; e < 1024; e++) {
+         du[(size_t)j * 1024 + e] = (uint32_t)dm[e];
+         df[(size_t)j * 1024 + e] = (float)dm[e];
+      }
+      for (int e = 0; e < h[j]->nxf; e++) vis[(size_t)j * vs + e] = (float)px[e].vis;
+   }
+   const size_t nb = du.size() * 4, fb = df.size() * 4, vb = vis.size() * 4;
+   if (g_ctx.jobtab.ensure(nb + fb + vb)) return 1;
+   unsigned char *t = (unsigned char *)g_ctx.jobtab.p;
+   HIPCHK(hipMemcpyAsync(t, du.data(), nb, hipMemcpyHostToDevice, g_ctx.stream));
+   HIPCHK(hipMemcpyAsync(t + nb, df.data(), fb, hipMemcpyHostToDevice, g_ctx.stream));
+   HIPCHK(hipMemcpyAsync(t + nb + fb, vis.data(), vb, hipMemcpyHostToDevice, g_ctx.stream));
+   HIPCHK(hipStreamSynchronize(g_ctx.stream));  // the host vectors go out of scope
+   b->dmapu_all = (const uint32_t *)t;
+   b->dmapf_all = (const float *)(t + nb);
+   b->vis_all = (const float *)(t + nb + fb);
+   b->vis_stride = vs;
+   return 0;
+}
+
+// Largest records per accumulation work item: a cell's sums must fit 32
+// bits (and the packed pairs must not carry): at most 2^20 records, fewer
+// if a palette entry exceeds 255.
+static int item_records(const fl3_job_hdr *const *h, int n) {
+   double maxd = 255.0;
+   for (int j = 0; j < n; j++) {
+      const double *dm = (const double *)((const unsigned char *)h[j] + h[j]->off_dmap);
+      for (int e = 0; e < 1024; e++) maxd = dm[e] > maxd ? dm[e] : maxd;
+   }
+   const double lim = 4294967295.0 / (maxd + 1.0);
+   const int c = lim < (double)FL3_BIN_ITEM_RECORDS ? (int)lim : FL3_BIN_ITEM_RECORDS;
+   return c < 1 ? 1 : c;
 }
 
 // Binned deposit: the walkers write records in chunks of `per` rows (the
@@ -423,43 +471,8 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    b.nbins = nbins;
    b.width = g_ctx.width;
    b.height = g_ctx.height;
-   // a work item's per-cell sums must fit 32 bits (and the packed pairs must
-   // not carry): at most 2^20 records, fewer if a palette entry exceeds 255
-   double maxd = 255.0;
-   for (int j = 0; j < js.n; j++) {
-      const double *dm = (const double *)((const unsigned char *)js.h[j] + js.h[j]->off_dmap);
-      for (int e = 0; e < 1024; e++) maxd = dm[e] > maxd ? dm[e] : maxd;
-   }
-   const double lim = 4294967295.0 / (maxd + 1.0);
-   b.chunk = lim < (double)FL3_BIN_ITEM_RECORDS ? (int)lim : FL3_BIN_ITEM_RECORDS;
-   if (b.chunk < 1) b.chunk = 1;
-   if (format == 2) {
-      // palette and visibility of every job for the accumulation
-      const int vs = js.nxf_max > 0 ? js.nxf_max : 1;
-      std::vector<uint32_t> du((size_t)js.n * 1024);
-      std::vector<float> df((size_t)js.n * 1024), vis((size_t)js.n * vs, 1.0f);
-      for (int j = 0; j < js.n; j++) {
-         const unsigned char *hb = (const unsigned char *)js.h[j];
-         const double *dm = (const double *)(hb + js.h[j]->off_dmap);
-         const fl3_xform_pack *px = (const fl3_xform_pack *)(hb + js.h[j]->off_xf);
-         for (int e = 0; e < 1024; e++) {
-            du[(size_t)j * 1024 + e] = (uint32_t)dm[e];
-            df[(size_t)j * 1024 + e] = (float)dm[e];
-         }
-         for (int e = 0; e < js.h[j]->nxf; e++) vis[(size_t)j * vs + e] = (float)px[e].vis;
-      }
-      const size_t nb = du.size() * 4, fb = df.size() * 4, vb = vis.size() * 4;
-      if (g_ctx.jobtab.ensure(nb + fb + vb)) return 1;
-      unsigned char *t = (unsigned char *)g_ctx.jobtab.p;
-      HIPCHK(hipMemcpyAsync(t, du.data(), nb, hipMemcpyHostToDevice, g_ctx.stream));
-      HIPCHK(hipMemcpyAsync(t + nb, df.data(), fb, hipMemcpyHostToDevice, g_ctx.stream));
-      HIPCHK(hipMemcpyAsync(t + nb + fb, vis.data(), vb, hipMemcpyHostToDevice, g_ctx.stream));
-      HIPCHK(hipStreamSynchronize(g_ctx.stream));  // the host vectors go out of scope
-      b.dmapu_all = (const uint32_t *)t;
-      b.dmapf_all = (const float *)(t + nb);
-      b.vis_all = (const float *)(t + nb + fb);
-      b.vis_stride = vs;
-   }
+   b.chunk = item_records(js.h, js.n);
+   if (format == 2 && upload_job_tables(js.h, js.n, js.nxf_max, &b)) return 1;
    a.rec_mode = narrow ? 2 : (format == 2 ? 3 : 1);
    a.wstate = (uint32_t *)g_ctx.wstate.p;
    a.bin_shift = shift;
@@ -604,18 +617,27 @@ extern "C" int fl3d_iterate_batch(const void *const *jobs, const size_t *job_byt
    return read_counters(badvals, saturated);
 }
 
+// Callable while another thread runs device work: after the first call it
+// only reads the CU count.
 extern "C" long long fl3d_job_walkers(int64_t nsamples, int njobs, int requested) {
-   std::lock_guard<std::recursive_mutex> lk(g_mu);
-   if (ctx_init()) return -1;
+   if (!g_ready.load(std::memory_order_acquire)) {
+      std::lock_guard<std::recursive_mutex> lk(g_mu);
+      if (ctx_init()) return -1;
+   }
    return job_walkers(nsamples, njobs > 0 ? njobs : 1, requested);
 }
 
-// Jobs of several renders (begun with fl3d_begin_many) in one launch with
-// per-sample atomic deposits into each job's render.  A run of consecutive
-// jobs of one render with equal sample counts is what fl3d_iterate_batch
-// would launch for that render alone: it gets that launch's kernel tier, so
-// its arithmetic is the same; jobs of different tiers go to one launch per
-// tier.
+// Jobs of several renders (begun with fl3d_begin_many) in one launch.  A
+// run of consecutive jobs of one render with equal sample counts is what
+// fl3d_iterate_batch would launch for that render alone: it gets that
+// launch's kernel tier, so its arithmetic is the same.  Jobs with a stepped
+// palette write job records into the renders' joint cell space, which the
+// tile-binned deposit sums (the same integers per-sample atomics would add);
+// jobs with a linear palette, or every job when the renders' cells exceed
+// the tile limits, deposit with per-sample atomics (saturating for
+// jobs[j].exact).  Each tier, and each deposit kind, is a launch of its own.
+enum { MG_DIRECT = 0, MG_EXACT = FL3_TIER_FULL + 1, MG_REC = 2 * (FL3_TIER_FULL + 1), MG_N = 3 * (FL3_TIER_FULL + 1) };
+
 extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fuse, int exact, int nrenders,
                                   double *badvals, int *saturated) {
    std::lock_guard<std::recursive_mutex> lk(g_mu);
@@ -623,54 +645,82 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
       set_err("fl3d_iterate_multi: %s%ld jobs for the begun renders", "", (long)njobs);
       return 1;
    }
-   size_t total = 0, lds = 0;
+   size_t total = 0, lds = 0, lds_rec = 0;
    long long samples = 0;
-   std::vector<int> tier((size_t)njobs);
+   int nxf_max = 0;
+   std::vector<const fl3_job_hdr *> hs((size_t)njobs);
+   std::vector<int> tier((size_t)njobs), grp((size_t)njobs);
+   for (int j = 0; j < njobs; j++) {
+      hs[j] = (const fl3_job_hdr *)jobs[j].job;
+      nxf_max = hs[j]->nxf > nxf_max ? hs[j]->nxf : nxf_max;
+      samples += jobs[j].nsamples;
+   }
    for (int j = 0; j < njobs;) {
       int e = j, t = 0;
       while (e < njobs && e - j < 4096 && jobs[e].render == jobs[j].render && jobs[e].nsamples == jobs[j].nsamples) {
-         const int te = pick_tier((const fl3_job_hdr *)jobs[e].job);
+         const int te = pick_tier(hs[e]);
          t = te > t ? te : t;
          e++;
       }
       for (int k = j; k < e; k++) tier[k] = t;
       j = e;
    }
+   // the joint cell space of the renders, in tiles
+   long long tcells = 0;
+   for (size_t r = 0; r < g_ctx.rcell0.size(); r++) tcells += (long long)g_ctx.rwidth[r] * g_ctx.rheight[r];
+   const int shift = g_ctx.bits != 64 && njobs < 65536 && nxf_max <= 256 ? bin_shift_cells(tcells) : 0;
+   const int nbins = shift > 0 ? (int)((tcells + (1LL << shift) - 1) >> shift) : 0;
+   bool use_rec = shift > 0 && tcells < 0xffffffffLL;
    std::vector<JobDesc> d((size_t)njobs);
-   long long gblocks[FL3_TIER_FULL + 1] = {0};
+   long long gblocks[MG_N] = {0};
    for (int j = 0; j < njobs; j++) {
-      const fl3_job_hdr *h = (const fl3_job_hdr *)jobs[j].job;
-      const size_t l = lds_bytes_for<float>(h);
-      lds = l > lds ? l : lds;
       if (jobs[j].walkers < 256 || jobs[j].walkers % 256 || jobs[j].render < 0 || jobs[j].render >= nrenders) {
          set_err("fl3d_iterate_multi: job %s%ld needs a multiple of 256 walkers and a begun render", "", (long)j);
          return 1;
       }
+      const bool rec = use_rec && hs[j]->palette_mode == 0;
+      grp[j] = rec ? MG_REC + tier[j] : (exact || jobs[j].exact ? MG_EXACT : MG_DIRECT) + tier[j];
+      const size_t l = lds_bytes_for<float>(hs[j], !rec);
+      if (rec)
+         lds_rec = l > lds_rec ? l : lds_rec;
+      else
+         lds = l > lds ? l : lds;
       memset(&d[j], 0, sizeof(JobDesc));
       d[j].off = (long long)total;
       d[j].nsamples = jobs[j].nsamples;
       d[j].key0 = jobs[j].key[0];
       d[j].key1 = jobs[j].key[1];
       d[j].cell0 = g_ctx.rcell0[jobs[j].render];
-      d[j].block0 = (int)gblocks[tier[j]];   // within its tier's launch
+      d[j].block0 = (int)gblocks[grp[j]];   // within its group's launch
       d[j].nblocks = jobs[j].walkers / 256;
       d[j].render = jobs[j].render;
-      gblocks[tier[j]] += d[j].nblocks;
-      samples += jobs[j].nsamples;
+      gblocks[grp[j]] += d[j].nblocks;
       total += (jobs[j].bytes + 255) & ~(size_t)255;
    }
-   long long gstart[FL3_TIER_FULL + 2] = {0};
-   for (int t = 0; t <= FL3_TIER_FULL; t++) {
-      if (gblocks[t] > 0x7fffffffLL) {
+   const unsigned hist_off = (unsigned)((lds_rec + 15) & ~(size_t)15);
+   if (hist_off + sizeof(unsigned) * (size_t)nbins > g_ctx.lds_max) use_rec = false;
+   if (!use_rec)   // every job deposits with atomics
+      for (int j = 0; j < njobs; j++)
+         if (grp[j] >= MG_REC) {
+            gblocks[grp[j]] -= d[j].nblocks;
+            grp[j] = (exact || jobs[j].exact ? MG_EXACT : MG_DIRECT) + tier[j];
+            d[j].block0 = (int)gblocks[grp[j]];
+            gblocks[grp[j]] += d[j].nblocks;
+            const size_t l = lds_bytes_for<float>(hs[j]);
+            lds = l > lds ? l : lds;
+         }
+   long long gstart[MG_N + 1] = {0};
+   for (int g = 0; g < MG_N; g++) {
+      if (gblocks[g] > 0x7fffffffLL) {
          set_err("fl3d_iterate_multi: too many workgroups");
          return 1;
       }
-      gstart[t + 1] = gstart[t] + gblocks[t];
+      gstart[g + 1] = gstart[g] + gblocks[g];
    }
-   const long long blocks = gstart[FL3_TIER_FULL + 1];
+   const long long blocks = gstart[MG_N];
    std::vector<int> blk((size_t)blocks);
    for (int j = 0; j < njobs; j++)
-      for (int b = 0; b < d[j].nblocks; b++) blk[(size_t)(gstart[tier[j]] + d[j].block0 + b)] = j;
+      for (int b = 0; b < d[j].nblocks; b++) blk[(size_t)(gstart[grp[j]] + d[j].block0 + b)] = j;
    const size_t dbytes = sizeof(JobDesc) * njobs, bbytes = sizeof(int) * (size_t)blocks;
    std::vector<unsigned char> host(total + dbytes + bbytes);
    for (int j = 0; j < njobs; j++) memcpy(host.data() + d[j].off, jobs[j].job, jobs[j].bytes);
@@ -686,17 +736,112 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
    a.buckets = g_ctx.buckets.p;
    a.badvals = (unsigned long long *)g_ctx.rcount.p;
    a.fuse = fuse;
+   const int *blk_dev = (const int *)(a.jobs + total + dbytes);
    g_prof.samples += samples;
-   for (int t = 0; t <= FL3_TIER_FULL; t++) {
-      if (!gblocks[t]) continue;
-      a.blk_job = (const int *)(a.jobs + total + dbytes) + gstart[t];
+   for (int g = 0; g < MG_REC; g++) {
+      if (!gblocks[g]) continue;
+      a.blk_job = blk_dev + gstart[g];
       const int pk = prof_begin(ST_ITER);
-      const int lrc = fl3k_launch_iter_fast(&a, (unsigned)gblocks[t], lds, g_ctx.stream, g_ctx.bits, exact, t);
+      const int lrc = fl3k_launch_iter_fast(&a, (unsigned)gblocks[g], lds, g_ctx.stream, g_ctx.bits, g >= MG_EXACT,
+                                            g % (FL3_TIER_FULL + 1));
       prof_end(pk);
       if (lrc) {
          set_err("k_iter_fast launch failed");
          return 1;
       }
+   }
+   const long long Wrec = 256 * (gstart[MG_N] - gstart[MG_REC]);
+   if (Wrec > 0) {
+      // job records in chunks of `per` rows, sorted and accumulated on the
+      // bin stream while the walkers produce the next chunk
+      long long rows_total = 0;
+      for (int j = 0; j < njobs; j++)
+         if (grp[j] >= MG_REC) {
+            const long long wj = 256LL * d[j].nblocks, r = (d[j].nsamples + wj - 1) / wj;
+            rows_total = r > rows_total ? r : rows_total;
+         }
+      long long per = FL3_BIN_CHUNK_SAMPLES / Wrec;
+      if (per < 1) per = 1;
+      if (per > rows_total) per = rows_total;
+      if ((double)Wrec * per >= 4.0e9) {
+         set_err("binned deposit chunk exceeds 2^32 records");
+         return 1;
+      }
+      const long long nblk = Wrec / 256;
+      const long long nchunks = (rows_total + per - 1) / per;
+      const size_t rec_bytes = 8 * (size_t)Wrec * per;
+      if (g_ctx.rec.ensure(rec_bytes) || g_ctx.sorted.ensure(rec_bytes)) return 1;
+      if (nchunks > 1 && g_ctx.rec2.ensure(rec_bytes)) return 1;
+      if (g_ctx.wstate.ensure(sizeof(uint32_t) * 10 * (size_t)Wrec)) return 1;
+      if (g_ctx.blkhist.ensure(sizeof(unsigned) * (size_t)nblk * nbins)) return 1;
+      if (nchunks > 1 && g_ctx.blkhist2.ensure(sizeof(unsigned) * (size_t)nblk * nbins)) return 1;
+      if (g_ctx.bins.ensure(sizeof(unsigned) * (3 * (size_t)nbins + 8))) return 1;
+      unsigned *bw = (unsigned *)g_ctx.bins.p;
+      BinArgs b;
+      memset(&b, 0, sizeof(b));
+      b.job = a.jobs;
+      b.buckets = g_ctx.buckets.p;
+      b.sorted = g_ctx.sorted.p;
+      b.nblk = (int)nblk;
+      b.walkers = Wrec;
+      b.bin_total = bw;
+      b.bin_start = bw + nbins;
+      b.item_start = bw + 2 * nbins + 1;
+      b.item_counter = bw + 3 * nbins + 2;
+      b.shift = shift;
+      b.nbins = nbins;
+      b.width = (int)(tcells < 0x7fffffffLL ? tcells : 0x7fffffffLL);
+      b.height = 1;
+      b.chunk = item_records(hs.data(), njobs);
+      if (upload_job_tables(hs.data(), njobs, nxf_max, &b)) return 1;
+      IterArgs ar = a;
+      ar.rec_mode = 3;   // job records
+      ar.wstate = (uint32_t *)g_ctx.wstate.p;
+      ar.bin_shift = shift;
+      ar.nbins = nbins;
+      ar.hist_off = hist_off;
+      ar.wtot = Wrec;
+      const size_t lds_r = hist_off + sizeof(unsigned) * (size_t)nbins;
+      hipStream_t ws = g_ctx.stream, bs = g_ctx.bin_stream;
+      HIPCHK(hipEventRecord(g_ctx.ev_join, ws));
+      HIPCHK(hipStreamWaitEvent(bs, g_ctx.ev_join, 0));
+      for (long long k = 0; k < nchunks; k++) {
+         const int buf = (int)(k & 1);
+         ar.row0 = k * per;
+         ar.rows_out = rows_total - ar.row0 < per ? rows_total - ar.row0 : per;
+         ar.resume = k > 0;
+         ar.rec = (unsigned long long *)(buf ? g_ctx.rec2.p : g_ctx.rec.p);
+         ar.blk_hist = (unsigned *)(buf ? g_ctx.blkhist2.p : g_ctx.blkhist.p);
+         if (k >= 2) HIPCHK(hipStreamWaitEvent(ws, g_ctx.ev_bin[buf], 0));
+         for (int g = MG_REC; g < MG_N; g++) {
+            if (!gblocks[g]) continue;
+            ar.blk_job = blk_dev + gstart[g];
+            ar.blk0 = (int)(gstart[g] - gstart[MG_REC]);
+            const int pk = prof_begin(ST_ITER, ws);
+            const int lrc = fl3k_launch_iter_fast(&ar, (unsigned)gblocks[g], lds_r, ws, g_ctx.bits, 1, g - MG_REC);
+            prof_end(pk, ws);
+            if (lrc) {
+               set_err("k_iter_fast launch failed");
+               return 1;
+            }
+         }
+         HIPCHK(hipEventRecord(g_ctx.ev_walk[buf], ws));
+         HIPCHK(hipStreamWaitEvent(bs, g_ctx.ev_walk[buf], 0));
+         HIPCHK(hipMemsetAsync(b.item_counter, 0, sizeof(unsigned), bs));
+         b.rec = ar.rec;
+         b.blk_hist = ar.blk_hist;
+         b.rows = ar.rows_out;
+         const int pk = prof_begin(ST_BIN, bs);
+         const int lrc = fl3k_launch_bin(&b, g_ctx.ncu, 2, bs);
+         prof_end(pk, bs);
+         if (lrc) {
+            set_err("binned deposit launch failed");
+            return 1;
+         }
+         HIPCHK(hipEventRecord(g_ctx.ev_bin[buf], bs));
+      }
+      HIPCHK(hipEventRecord(g_ctx.ev_join, bs));
+      HIPCHK(hipStreamWaitEvent(ws, g_ctx.ev_join, 0));
    }
    std::vector<unsigned long long> c(2 * (size_t)nrenders);
    HIPCHK(hipMemcpyAsync(c.data(), g_ctx.rcount.p, 16 * (size_t)nrenders, hipMemcpyDeviceToHost, g_ctx.stream));
@@ -754,12 +899,13 @@ static int upload_de(const fl3_de_filters *de, double **coefs, double **widths, 
    *coefs = (double *)g_ctx.table.p;
    *widths = *coefs + nc;
    *coefs_f = (float *)(*widths + nw);
-   std::vector<float> cf(nc);
-   for (size_t i = 0; i < nc; i++) cf[i] = (float)de->filter_coefs[i];
    HIPCHK(hipMemcpyAsync(*coefs, de->filter_coefs, sizeof(double) * nc, hipMemcpyHostToDevice, g_ctx.stream));
    HIPCHK(hipMemcpyAsync(*widths, de->filter_widths, sizeof(double) * nw, hipMemcpyHostToDevice, g_ctx.stream));
-   HIPCHK(hipMemcpyAsync(*coefs_f, cf.data(), sizeof(float) * nc, hipMemcpyHostToDevice, g_ctx.stream));
-   // cf is a temporary: the copies finish before it goes
+   if (fl3k_launch_f64_to_f32(*coefs, *coefs_f, (long)nc, g_ctx.stream)) {
+      set_err("k_f64_to_f32 launch failed");
+      return 1;
+   }
+   // the caller may free its host tables once this returns
    HIPCHK(hipStreamSynchronize(g_ctx.stream));
    return 0;
 }

@@ -88,6 +88,14 @@ void *fl3_pack_job(const flam3_genome *cp, const unsigned short *distrib, const 
                    size_t *bytes);
 void *fl3_pack_camera_job(const fl3h_camera *cam, size_t *bytes);
 
+/* Several renders in two phases (fl3h_render_many = prepare + execute):
+ * host work of every render, then the device work.  The requests must stay
+ * valid until fl3_many_execute returns. */
+typedef struct fl3_many fl3_many;
+fl3_many *fl3_many_prepare(fl3h_request *reqs, int n, const fl3h_render_opts *opts);
+int fl3_many_execute(fl3_many *B);
+void fl3_many_free(fl3_many *B);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,8 +18,11 @@ struct JobDesc {
 
 struct IterArgs {
    const unsigned char *job;
-   const JobDesc *descs;       // non-null: independent renders (direct deposit only)
-   const int *blk_job;
+   const JobDesc *descs;       // non-null: independent renders (direct deposits or job records)
+   const int *blk_job;         // job of each workgroup of the launch
+   int blk0;                   // the launch's first workgroup in the record / walker-state columns
+   long long wtot;             // walkers of all launches sharing the records (0: this launch's)
+   long long row0;             // independent renders with records: first row of this chunk
    // several jobs (the temporal samples of a batch) in one launch: workgroup
    // b iterates job b % njobs with walker key keys[2 * job .. +1]; nsamples
    // is then per job
@@ -61,7 +64,7 @@ struct IterArgs {
 
 #define FL3_BIN_CHUNK_SAMPLES (1LL << 28)
 #define FL3_BIN_ITEM_RECORDS (1 << 20)
-#define FL3_BIN_MIN_SAMPLES (1LL << 24)
+#define FL3_BIN_MIN_SAMPLES (1LL << 20)
 
 // Binned deposit (walker records -> tile-sorted records -> LDS tiles).
 struct BinArgs {

@@ -472,6 +472,19 @@ extern "C" int fl3k_launch_de_tiled(const DEArgs *a, hipStream_t s, int bits) {
    return hipGetLastError() != hipSuccess;
 }
 
+// Double to float copy of a table on the device (the density-estimation
+// coefficients of the FP32 accumulator path).
+__global__ __launch_bounds__(256) void k_f64_to_f32(const double *src, float *dst, long n) {
+   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) dst[i] = (float)src[i];
+}
+
+extern "C" int fl3k_launch_f64_to_f32(const double *src, float *dst, long n, hipStream_t s) {
+   if (n <= 0) return 0;
+   hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+   return hipGetLastError() != hipSuccess;
+}
+
 extern "C" int fl3k_launch_logscale(const void *b, void *acc, long n, double k1, double k2, hipStream_t s,
                                     int bits) {
    const dim3 grid((unsigned)((n + 255) / 256)), block(256);

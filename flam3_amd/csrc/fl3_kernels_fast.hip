@@ -6,8 +6,8 @@
 
 extern "C" int fl3k_launch_iter_fast(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int bits,
                                      int exact, int tier) {
-   if (tier >= 0 && tier < FL3_TIER_FULL && (a->rec_mode || (bits != 64 && !exact)))
-      return fl3k_launch_iter_tier(a, grid, lds, s, tier);
+   if (tier >= 0 && tier < FL3_TIER_FULL && (a->rec_mode || bits != 64))
+      return fl3k_launch_iter_tier(a, grid, lds, s, tier, exact);
    if (a->rec_mode == FL3_REC_JOB)
       hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_JOB, FL3_TIER_FULL>), dim3(grid), dim3(256), lds, s, *a);
    else if (a->rec_mode == FL3_REC_NARROW)

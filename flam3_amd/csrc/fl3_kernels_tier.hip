@@ -6,26 +6,30 @@
 #include "fl3_launch.h"
 
 template <int TIER>
-static void launch_tier(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s) {
+static void launch_tier(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int exact) {
    if (a->rec_mode == FL3_REC_JOB)
       hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_JOB, TIER>), dim3(grid), dim3(256), lds, s, *a);
    else if (a->rec_mode == FL3_REC_NARROW)
       hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_NARROW, TIER>), dim3(grid), dim3(256), lds, s, *a);
    else if (a->rec_mode == FL3_REC_WIDE)
       hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_WIDE, TIER>), dim3(grid), dim3(256), lds, s, *a);
+   else if (exact)
+      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_DIRECT, TIER>), dim3(grid), dim3(256), lds, s, *a);
    else
       hipLaunchKernelGGL((k_iter_fast<33, false, FL3_REC_DIRECT, TIER>), dim3(grid), dim3(256), lds, s, *a);
 }
 
-// Records (binned deposit, either format) or direct fast bits-33 deposits.
-extern "C" int fl3k_launch_iter_tier(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int tier) {
+// Records (binned deposit, either format) or direct bits-33 deposits (fast,
+// or saturating with exact).
+extern "C" int fl3k_launch_iter_tier(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int tier,
+                                     int exact) {
    switch (tier) {
-   case 0: launch_tier<0>(a, grid, lds, s); break;
-   case 1: launch_tier<1>(a, grid, lds, s); break;
-   case 2: launch_tier<2>(a, grid, lds, s); break;
-   case 3: launch_tier<3>(a, grid, lds, s); break;
-   case 4: launch_tier<4>(a, grid, lds, s); break;
-   case 5: launch_tier<5>(a, grid, lds, s); break;
+   case 0: launch_tier<0>(a, grid, lds, s, exact); break;
+   case 1: launch_tier<1>(a, grid, lds, s, exact); break;
+   case 2: launch_tier<2>(a, grid, lds, s, exact); break;
+   case 3: launch_tier<3>(a, grid, lds, s, exact); break;
+   case 4: launch_tier<4>(a, grid, lds, s, exact); break;
+   case 5: launch_tier<5>(a, grid, lds, s, exact); break;
    default: return 1;
    }
    return hipGetLastError() != hipSuccess;

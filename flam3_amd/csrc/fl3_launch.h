@@ -12,7 +12,7 @@ extern "C" {
 int fl3k_launch_iter_fast(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int bits, int exact,
                           int tier);
 /* tier in [0, FL3_TIER_FULL): kernels restricted to that variation tier */
-int fl3k_launch_iter_tier(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int tier);
+int fl3k_launch_iter_tier(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int tier, int exact);
 int fl3k_launch_apply_fp32(const unsigned char *job, int xi, const double *pts, long long n, double *out,
                            unsigned long long key, size_t lds, hipStream_t s, int tier);
 int fl3k_launch_apply_fp32_tier(const unsigned char *job, int xi, const double *pts, long long n, double *out,
@@ -28,6 +28,7 @@ int fl3k_launch_preview(const unsigned char *job, int xi, double range, int numv
 int fl3k_launch_deposit(const unsigned char *job, const double *pts, long long n, void *buckets, uint32_t *sat,
                         unsigned grid, size_t lds, hipStream_t s, int bits);
 int fl3k_launch_de_tiled(const DEArgs *a, hipStream_t s, int bits);
+int fl3k_launch_f64_to_f32(const double *src, float *dst, long n, hipStream_t s);
 int fl3k_launch_logscale(const void *b, void *acc, long n, double k1, double k2, hipStream_t s, int bits);
 int fl3k_launch_earlyclip(void *acc, long n, const TMArgs *a, hipStream_t s, int bits);
 int fl3k_launch_tonemap(const TMArgs *a, hipStream_t s, int bits);

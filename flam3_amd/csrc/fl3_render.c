@@ -27,8 +27,13 @@
 
 /* Job records carry the job index in 16 bits. */
 #define MAX_BATCH_JOBS 4096
-/* Renders below this many plotted samples may share walker launches. */
+/* Renders below this many plotted samples may share walker launches, and
+ * with a stepped palette (bits 32/33) up to the second size. */
 #define FL3_SHARED_MAX_SAMPLES (1LL << 24)
+#define FL3_SHARED_MAX_STEP_SAMPLES (1LL << 26)
+/* Histogram cells of one shared launch: its renders' joint cell space in at
+ * most 8192 tiles of 4096 cells. */
+#define FL3_SHARED_MAX_CELLS (32LL * 1000 * 1000)
 
 /* One launch over the *n temporal samples collected so far (nsamples each,
  * keys[2t], keys[2t+1] for the t-th); frees their jobs and empties the set. */
@@ -564,28 +569,41 @@ cleanup:
 
 /* ---------------------------------------------------------------- many renders
  *
- * Small renders share the GPU: the walkers of every temporal sample of every
- * shared render run in one launch, each depositing into its own render's
- * histogram, then each render's density estimation and tone map run and all
- * images come back in one copy.  A shared render draws the walkers, keys and
- * samples fl3h_render would draw for it alone, and its integer deposits do
- * not depend on order, so its image is the one fl3h_render makes. */
+ * Several renders in two phases: fl3_many_prepare does the host work of
+ * every render (plans, temporal-sample jobs, walker keys, DE filters) and
+ * fl3_many_execute the device work, so a caller can prepare the next set
+ * while the GPU runs this one.  Small renders share the GPU: the walkers of
+ * every temporal sample of every small render run in one launch, each
+ * depositing into its own render's histogram, then each render's density
+ * estimation and tone map run and all their images come back in one copy.
+ * Large renders run one after another with fl3h_render's launches.  Every
+ * render draws the walkers, keys and samples fl3h_render would draw for it
+ * alone, and its integer deposits do not depend on order, so its image is
+ * the one fl3h_render makes. */
 
-/* Renders of one batch in production mode whose plotted samples stay below
- * the size from which fl3h_render bins its deposit (per-sample atomics are
- * then what fl3h_render uses too). */
-static int plan_shareable(const render_plan *P, const fl3h_render_opts *opts, int bits) {
+/* Production renders of one batch without a progress callback are
+ * prepared here.  1 = shared (its walkers join the shared launch, with the
+ * bits of the other shared renders): below the size from which fl3h_render
+ * bins its deposit (per-sample atomics are then what fl3h_render uses too),
+ * or, with a stepped palette, below a larger size; 2 = alone, with
+ * fl3h_render's launches.  0: fl3h_render itself. */
+static int plan_kind(const render_plan *P, const fl3h_render_opts *opts, int bits) {
    double est;
    if (P->nbatches != 1 || P->replay || P->spec->progress) return 0;
-   if (opts && opts->deposit == 2) return 0;
-   if (bits && P->spec->bits != bits) return 0;
+   if (opts && opts->deposit == 1) return 0;  /* forced atomics may saturate: fl3h_render's second pass */
+   if (opts && opts->deposit == 2) return 2;
+   if (bits && P->spec->bits != bits) return 2;
    est = P->cp.sample_density * pow(2.0, 2.0 * P->cp.zoom) * P->geo.image_width * P->geo.image_height;
-   return est < (double)FL3_SHARED_MAX_SAMPLES;
+   if (est < (double)FL3_SHARED_MAX_SAMPLES) return 1;
+   /* above it fl3h_render bins the deposit, which sums the same integers as
+    * atomics for a stepped palette (a linear one rounds the blend fraction) */
+   if (P->cp.palette_mode == 0 && P->spec->bits != 64 && est < (double)FL3_SHARED_MAX_STEP_SAMPLES) return 1;
+   return 2;
 }
 
-/* No cell of a shared render can reach 2^32 in any channel, so the fast
- * deposit never saturates (fl3h_render's saturating second pass is never
- * needed for it). */
+/* No cell of the render can reach 2^32 in any channel, so the fast deposit
+ * never saturates (fl3h_render's saturating second pass is never needed);
+ * otherwise its jobs deposit with saturating adds. */
 static int jobs_cannot_saturate(const fl3d_multi_job *m, int n) {
    double total = 0.0, maxd = 255.0;
    int j, e;
@@ -598,165 +616,327 @@ static int jobs_cannot_saturate(const fl3d_multi_job *m, int n) {
    return total * (maxd + 1.0) < 4294967295.0;
 }
 
-int fl3h_render_many(fl3h_request *reqs, int n, const fl3h_render_opts *opts) {
-   render_plan *P = NULL;
-   fl3_de_filters *de = NULL;
-   int *slot = NULL, *sat = NULL, *widths = NULL, *heights = NULL, *live = NULL;
-   double *bad = NULL;
-   fl3d_multi_job *mj = NULL;
-   fl3d_image_dst *dst = NULL;
-   int r, ts, i, nlive = 0, nmj = 0, bits = 0, failures = 0, devok = 0;
-   size_t img_total = 0, filt_total = 0;
-   if (n <= 0) return 0;
-   P = (render_plan *)calloc((size_t)n, sizeof(render_plan));
-   de = (fl3_de_filters *)calloc((size_t)n, sizeof(fl3_de_filters));
-   slot = (int *)malloc(sizeof(int) * (size_t)n);
-   live = (int *)malloc(sizeof(int) * (size_t)n);
-   if (!P || !de || !slot || !live) goto out;
-   for (r = 0; r < n; r++) {
-      reqs[r].rc = 1;
-      slot[r] = -1;
+struct fl3_many {
+   fl3h_request *reqs;
+   int n;
+   fl3h_render_opts opts;
+   int has_opts;
+   render_plan *P;
+   fl3_de_filters *de;
+   int *kind;              /* per request: -1 failed, 0 fl3h_render, 1 small, 2 large */
+   int *job0, *njob;       /* per request: its jobs in mj */
+   int *live;              /* small renders in launch order */
+   int nlive, bits;
+   fl3d_multi_job *mj;
+   int nmj;
+};
+
+void fl3_many_free(fl3_many *B) {
+   int i;
+   if (!B) return;
+   for (i = 0; i < B->nmj; i++) free((void *)B->mj[i].job);
+   if (B->P)
+      for (i = 0; i < B->n; i++) {
+         if (B->de) fl3_free_de_filters(&B->de[i]);
+         plan_close(&B->P[i]);
+      }
+   free(B->mj);
+   free(B->live);
+   free(B->njob);
+   free(B->job0);
+   free(B->kind);
+   free(B->de);
+   free(B->P);
+   free(B);
+}
+
+/* Walkers per run of consecutive temporal samples with equal sample counts,
+ * as fl3h_render launches them. */
+static int assign_walkers(fl3d_multi_job *m, int n, const fl3h_render_opts *opts) {
+   int ts = 0, i;
+   while (ts < n) {
+      int e = ts;
+      long long w;
+      while (e < n && e - ts < MAX_BATCH_JOBS && m[e].nsamples == m[ts].nsamples) e++;
+      w = fl3d_job_walkers(m[ts].nsamples, e - ts, opts ? opts->nwalkers : 0);
+      if (w < 256) return 1;
+      for (i = ts; i < e; i++) m[i].walkers = (int)w;
+      ts = e;
    }
-   /* plans and jobs of the shared renders (their frame streams advance here) */
+   return 0;
+}
+
+fl3_many *fl3_many_prepare(fl3h_request *reqs, int n, const fl3h_render_opts *opts) {
+   fl3_many *B = (fl3_many *)calloc(1, sizeof(fl3_many));
+   int r, ts, i;
+   if (!B) return NULL;
+   B->reqs = reqs;
+   B->n = n;
+   if (opts) {
+      B->opts = *opts;
+      B->has_opts = 1;
+   }
+   opts = B->has_opts ? &B->opts : NULL;
+   B->P = (render_plan *)calloc((size_t)(n > 0 ? n : 1), sizeof(render_plan));
+   B->de = (fl3_de_filters *)calloc((size_t)(n > 0 ? n : 1), sizeof(fl3_de_filters));
+   B->kind = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+   B->job0 = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
+   B->njob = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
+   B->live = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+   if (!B->P || !B->de || !B->kind || !B->job0 || !B->njob || !B->live) {
+      fl3_many_free(B);
+      return NULL;
+   }
    for (r = 0; r < n; r++) {
       fl3h_request *q = &reqs[r];
       randctx rc0;
       fl3d_multi_job *t;
-      if (plan_open(&P[r], q->frame, q->out, q->field, q->nchan, q->transp, q->stats, opts)) continue;
-      if (!plan_shareable(&P[r], opts, bits)) {
-         plan_close(&P[r]);
-         slot[r] = -2; /* rendered alone below */
+      render_plan *Q = &B->P[r];
+      q->rc = 1;
+      B->kind[r] = -1;
+      if (plan_open(Q, q->frame, q->out, q->field, q->nchan, q->transp, q->stats, opts)) continue;
+      B->kind[r] = plan_kind(Q, opts, B->bits);
+      if (B->kind[r] == 0) {
+         plan_close(Q);
          continue;
       }
-      if (plan_batch_de(&P[r], 0, &de[r])) {
-         plan_close(&P[r]);
-         continue;
-      }
-      t = (fl3d_multi_job *)realloc(mj, sizeof(fl3d_multi_job) * (size_t)(nmj + P[r].nts));
-      if (!t) {
-         fl3_free_de_filters(&de[r]);
-         plan_close(&P[r]);
-         continue;
-      }
-      mj = t;
       rc0 = q->frame->rc;
-      memset(mj + nmj, 0, sizeof(fl3d_multi_job) * (size_t)P[r].nts);
-      for (ts = 0; ts < P[r].nts; ts++) {
-         fl3d_multi_job *m = &mj[nmj + ts];
+      if (plan_batch_de(Q, 0, &B->de[r])) {
+         B->kind[r] = -1;
+         plan_close(Q);
+         continue;
+      }
+      t = (fl3d_multi_job *)realloc(B->mj, sizeof(fl3d_multi_job) * (size_t)(B->nmj + Q->nts));
+      if (!t) {
+         B->kind[r] = -1;
+         fl3_free_de_filters(&B->de[r]);
+         plan_close(Q);
+         continue;
+      }
+      B->mj = t;
+      B->job0[r] = B->nmj;
+      memset(B->mj + B->nmj, 0, sizeof(fl3d_multi_job) * (size_t)Q->nts);
+      for (ts = 0; ts < Q->nts; ts++) {
+         fl3d_multi_job *m = &B->mj[B->nmj + ts];
          double batch_size;
          void *job;
-         plan_keys(&P[r], ts, ts);
-         if (plan_ts_job(&P[r], ts, 1, &job, &m->bytes, &batch_size)) break;
+         plan_keys(Q, ts, ts);
+         if (plan_ts_job(Q, ts, 1, &job, &m->bytes, &batch_size)) break;
          m->job = job;
-         m->nsamples = plan_nsamples(&P[r], batch_size);
-         m->key[0] = P[r].keys[2 * ts];
-         m->key[1] = P[r].keys[2 * ts + 1];
-         m->render = nlive;
+         m->nsamples = plan_nsamples(Q, batch_size);
+         m->key[0] = Q->keys[2 * ts];
+         m->key[1] = Q->keys[2 * ts + 1];
       }
-      if (ts == P[r].nts && !jobs_cannot_saturate(mj + nmj, ts)) {
-         /* rendered alone after all, from the frame stream as it was */
-         q->frame->rc = rc0;
-         slot[r] = -2;
+      if (ts == Q->nts && B->kind[r] == 1 && !jobs_cannot_saturate(B->mj + B->nmj, ts))
+         for (i = 0; i < ts; i++) B->mj[B->nmj + i].exact = 1;
+      if (ts == Q->nts && assign_walkers(B->mj + B->nmj, ts, opts)) {
+         fprintf(stderr, "flam3_amd: device error: %s\n", fl3d_error());
+         ts = -1;
       }
-      /* walkers per run of temporal samples with equal sample counts, as
-       * fl3h_render launches them */
-      if (ts == P[r].nts && slot[r] == -1)
-         for (ts = 0; ts < P[r].nts;) {
-            int e = ts;
-            long long w;
-            while (e < P[r].nts && e - ts < MAX_BATCH_JOBS && mj[nmj + e].nsamples == mj[nmj + ts].nsamples) e++;
-            w = fl3d_job_walkers(mj[nmj + ts].nsamples, e - ts, opts ? opts->nwalkers : 0);
-            if (w < 256) {
-               fprintf(stderr, "flam3_amd: device error: %s\n", fl3d_error());
-               break;
-            }
-            for (i = ts; i < e; i++) mj[nmj + i].walkers = (int)w;
-            ts = e;
+      if (ts != Q->nts) {
+         for (i = 0; i < Q->nts; i++) {
+            free((void *)B->mj[B->nmj + i].job);
+            B->mj[B->nmj + i].job = NULL;
          }
-      if (ts < P[r].nts || slot[r] == -2) {
-         for (i = 0; i < P[r].nts; i++) free((void *)mj[nmj + i].job);
-         fl3_free_de_filters(&de[r]);
-         plan_close(&P[r]);
+         q->frame->rc = rc0;
+         B->kind[r] = -1;
+         fl3_free_de_filters(&B->de[r]);
+         plan_close(Q);
          continue;
       }
-      nmj += P[r].nts;
-      bits = P[r].spec->bits;
-      slot[r] = nlive;
-      live[nlive++] = r;
+      B->njob[r] = Q->nts;
+      B->nmj += Q->nts;
+      if (B->kind[r] == 1) {
+         for (i = 0; i < Q->nts; i++) B->mj[B->job0[r] + i].render = B->nlive;
+         B->bits = Q->spec->bits;
+         B->live[B->nlive++] = r;
+      }
    }
+   return B;
+}
 
-   if (nlive > 0) {
-      widths = (int *)malloc(sizeof(int) * (size_t)nlive);
-      heights = (int *)malloc(sizeof(int) * (size_t)nlive);
-      bad = (double *)calloc((size_t)nlive, sizeof(double));
-      sat = (int *)calloc((size_t)nlive, sizeof(int));
-      dst = (fl3d_image_dst *)calloc((size_t)nlive, sizeof(fl3d_image_dst));
-      if (widths && heights && bad && sat && dst) {
-         for (i = 0; i < nlive; i++) {
-            const render_plan *Q = &P[live[i]];
-            widths[i] = Q->geo.width;
-            heights[i] = Q->geo.height;
-            dst[i].img_off = img_total;
-            dst[i].out = Q->out;
-            dst[i].out_stride = plan_out_stride(Q);
-            dst[i].row_bytes = (size_t)Q->geo.image_width * Q->nchan * Q->bpc;
-            dst[i].rows = Q->geo.image_width > 0 ? Q->geo.image_height : 0;
-            img_total += (dst[i].row_bytes * (size_t)Q->geo.image_height + 255) & ~(size_t)255;
-            filt_total += (sizeof(double) * Q->filter_width * Q->filter_width + 255) & ~(size_t)255;
-         }
-         devok = fl3d_begin_many(widths, heights, nlive, bits) == 0 &&
-                 fl3d_iterate_multi(mj, nmj, FUSE_WALKER, 0, nlive, bad, sat) == 0 &&
-                 fl3d_images_begin(img_total, filt_total) == 0;
-         filt_total = 0;
-         for (i = 0; devok && i < nlive; i++) {
-            render_plan *Q = &P[live[i]];
-            fl3d_tm_params tp;
-            Q->badvals = bad[i];
-            devok = fl3d_select(i) == 0;
-            if (devok && de[live[i]].max_filter_index == 0) {
-               double k1, k2;
-               plan_k1k2(Q, &k1, &k2);
-               devok = fl3d_logscale(k1, k2) == 0;
-            } else if (devok) {
-               fl3d_de_params dp;
-               plan_de_params(Q, &dp);
-               devok = fl3d_density(&dp, &de[live[i]]) == 0;
-            }
-            plan_tm(Q, &tp);
-            if (devok) devok = fl3d_tonemap_async(&tp, Q->filter, dst[i].img_off, filt_total) == 0;
-            filt_total += (sizeof(double) * Q->filter_width * Q->filter_width + 255) & ~(size_t)255;
-         }
-         if (devok) devok = fl3d_images_fetch(dst, nlive) == 0;
-         if (!devok) fprintf(stderr, "flam3_amd: device error: %s\n", fl3d_error());
+/* Density estimation (or log scaling) of the selected render. */
+static int many_density(render_plan *Q, fl3_de_filters *de) {
+   if (de->max_filter_index == 0) {
+      double k1, k2;
+      plan_k1k2(Q, &k1, &k2);
+      return fl3d_logscale(k1, k2);
+   } else {
+      fl3d_de_params dp;
+      plan_de_params(Q, &dp);
+      return fl3d_density(&dp, de);
+   }
+}
+
+/* Shared renders live[i0, i1): one histogram allocation, one walker launch
+ * (per kernel tier), their density estimation and tone maps, one image
+ * download. */
+static int many_group(fl3_many *B, int i0, int i1) {
+   int *widths, *heights, *sat, i, ok = 0;
+   double *bad;
+   fl3d_multi_job *mj;
+   fl3d_image_dst *dst;
+   size_t img_total = 0, filt_total = 0;
+   const int nl = i1 - i0;
+   int nm = 0;
+   if (nl <= 0) return 0;
+   widths = (int *)malloc(sizeof(int) * (size_t)nl);
+   heights = (int *)malloc(sizeof(int) * (size_t)nl);
+   bad = (double *)calloc((size_t)nl, sizeof(double));
+   sat = (int *)calloc((size_t)nl, sizeof(int));
+   dst = (fl3d_image_dst *)calloc((size_t)nl, sizeof(fl3d_image_dst));
+   mj = (fl3d_multi_job *)malloc(sizeof(fl3d_multi_job) * (size_t)(B->nmj > 0 ? B->nmj : 1));
+   if (widths && heights && bad && sat && dst && mj) {
+      for (i = 0; i < nl; i++) {
+         const int r = B->live[i0 + i];
+         const render_plan *Q = &B->P[r];
+         int k;
+         widths[i] = Q->geo.width;
+         heights[i] = Q->geo.height;
+         dst[i].img_off = img_total;
+         dst[i].out = Q->out;
+         dst[i].out_stride = plan_out_stride(Q);
+         dst[i].row_bytes = (size_t)Q->geo.image_width * Q->nchan * Q->bpc;
+         dst[i].rows = Q->geo.image_width > 0 ? Q->geo.image_height : 0;
+         img_total += (dst[i].row_bytes * (size_t)Q->geo.image_height + 255) & ~(size_t)255;
+         filt_total += (sizeof(double) * Q->filter_width * Q->filter_width + 255) & ~(size_t)255;
+         memcpy(mj + nm, B->mj + B->job0[r], sizeof(fl3d_multi_job) * (size_t)B->njob[r]);
+         for (k = 0; k < B->njob[r]; k++) mj[nm + k].render = i;
+         nm += B->njob[r];
       }
-      for (i = 0; i < nlive; i++) {
-         r = live[i];
-         if (devok) {
-            plan_finish(&P[r]);
-            reqs[r].rc = 0;
-         }
-         fl3_free_de_filters(&de[r]);
-         plan_close(&P[r]);
+      ok = fl3d_begin_many(widths, heights, nl, B->bits) == 0 &&
+           fl3d_iterate_multi(mj, nm, FUSE_WALKER, 0, nl, bad, sat) == 0 &&
+           fl3d_images_begin(img_total, filt_total) == 0;
+      filt_total = 0;
+      for (i = 0; ok && i < nl; i++) {
+         render_plan *Q = &B->P[B->live[i0 + i]];
+         fl3d_tm_params tp;
+         Q->badvals = bad[i];
+         ok = fl3d_select(i) == 0 && many_density(Q, &B->de[B->live[i0 + i]]) == 0;
+         plan_tm(Q, &tp);
+         if (ok) ok = fl3d_tonemap_async(&tp, Q->filter, dst[i].img_off, filt_total) == 0;
+         filt_total += (sizeof(double) * Q->filter_width * Q->filter_width + 255) & ~(size_t)255;
+      }
+      if (ok) ok = fl3d_images_fetch(dst, nl) == 0;
+      if (!ok) fprintf(stderr, "flam3_amd: device error: %s\n", fl3d_error());
+      for (i = 0; ok && i < nl; i++) {
+         plan_finish(&B->P[B->live[i0 + i]]);
+         B->reqs[B->live[i0 + i]].rc = 0;
       }
    }
-   /* the rest one at a time */
-   for (r = 0; r < n; r++)
-      if (slot[r] == -2)
-         reqs[r].rc = fl3h_render(reqs[r].frame, reqs[r].out, reqs[r].field, reqs[r].nchan, reqs[r].transp,
-                                  reqs[r].stats, opts);
-out:
-   for (i = 0; i < nmj; i++) free((void *)mj[i].job);
-   for (r = 0; r < n; r++) failures += reqs[r].rc != 0;
    free(mj);
    free(dst);
    free(sat);
    free(bad);
    free(heights);
    free(widths);
-   free(live);
-   free(slot);
-   free(de);
-   free(P);
+   return ok ? 0 : 1;
+}
+
+/* The shared renders in groups whose joint cell space fits the tile-binned
+ * deposit (at most FL3_SHARED_MAX_CELLS cells, 65535 jobs). */
+static int many_small(fl3_many *B) {
+   int i0 = 0, rc = 0;
+   while (i0 < B->nlive) {
+      long long cells = 0;
+      int jobs = 0, i1 = i0;
+      while (i1 < B->nlive) {
+         const render_plan *Q = &B->P[B->live[i1]];
+         const long long c = (long long)Q->geo.width * Q->geo.height;
+         const int nj = B->njob[B->live[i1]];
+         if (i1 > i0 && (cells + c > FL3_SHARED_MAX_CELLS || jobs + nj > 65535)) break;
+         cells += c;
+         jobs += nj;
+         i1++;
+      }
+      rc |= many_group(B, i0, i1);
+      i0 = i1;
+   }
+   return rc;
+}
+
+/* A large prepared render: fl3h_render's launches (runs of temporal samples
+ * with equal sample counts), density estimation and tone map. */
+static int many_large(fl3_many *B, int r) {
+   render_plan *Q = &B->P[r];
+   const fl3h_render_opts *opts = B->has_opts ? &B->opts : NULL;
+   fl3d_multi_job *m = B->mj + B->job0[r];
+   int exact, ts, e, i, saturated = 0;
+   fl3d_tm_params tp;
+   if (fl3d_begin(Q->geo.width, Q->geo.height, Q->spec->bits)) goto devfail;
+   for (exact = 0; exact < 2; exact++) {
+      if (exact) {
+         if (!saturated) break;
+         if (fl3d_clear_buckets()) goto devfail;
+      }
+      Q->badvals = 0.0;
+      for (ts = 0; ts < B->njob[r]; ts = e) {
+         const void *jobs[MAX_BATCH_JOBS];
+         size_t bytes[MAX_BATCH_JOBS];
+         uint64_t keys[2 * MAX_BATCH_JOBS];
+         fl3d_iter_params ip;
+         double bad = 0.0;
+         int sat = 0;
+         for (e = ts; e < B->njob[r] && e - ts < MAX_BATCH_JOBS && m[e].nsamples == m[ts].nsamples; e++) {
+            jobs[e - ts] = m[e].job;
+            bytes[e - ts] = m[e].bytes;
+            keys[2 * (e - ts)] = m[e].key[0];
+            keys[2 * (e - ts) + 1] = m[e].key[1];
+         }
+         memset(&ip, 0, sizeof(ip));
+         ip.fuse = FUSE_WALKER;
+         ip.nsamples = m[ts].nsamples;
+         ip.key[0] = keys[0];
+         ip.key[1] = keys[1];
+         ip.exact = exact;
+         ip.nwalkers = opts ? opts->nwalkers : 0;
+         ip.deposit = opts ? opts->deposit : 0;
+         ip.chunk = opts ? opts->chunk_samples : 0;
+         if (fl3d_iterate_batch(jobs, bytes, e - ts, &ip, keys, &bad, &sat)) goto devfail;
+         Q->badvals += bad;
+         if (sat) saturated = 1;
+      }
+   }
+   if (many_density(Q, &B->de[r])) goto devfail;
+   plan_tm(Q, &tp);
+   if (fl3d_tonemap(&tp, Q->filter, Q->out, plan_out_stride(Q))) goto devfail;
+   plan_finish(Q);
+   B->reqs[r].rc = 0;
+   (void)i;
+   return 0;
+devfail:
+   fprintf(stderr, "flam3_amd: device error: %s\n", fl3d_error());
+   return 1;
+}
+
+int fl3_many_execute(fl3_many *B) {
+   int r, failures = 0;
+   const fl3h_render_opts *opts;
+   if (!B) return 0;
+   opts = B->has_opts ? &B->opts : NULL;
+   many_small(B);
+   for (r = 0; r < B->n; r++) {
+      fl3h_request *q = &B->reqs[r];
+      if (B->kind[r] == 2)
+         many_large(B, r);
+      else if (B->kind[r] == 0)
+         q->rc = fl3h_render(q->frame, q->out, q->field, q->nchan, q->transp, q->stats, opts);
+   }
+   for (r = 0; r < B->n; r++) failures += B->reqs[r].rc != 0;
+   return failures;
+}
+
+int fl3h_render_many(fl3h_request *reqs, int n, const fl3h_render_opts *opts) {
+   fl3_many *B;
+   int failures, r;
+   if (n <= 0) return 0;
+   B = fl3_many_prepare(reqs, n, opts);
+   if (!B) {
+      for (r = 0; r < n; r++) reqs[r].rc = 1;
+      return n;
+   }
+   failures = fl3_many_execute(B);
+   fl3_many_free(B);
    return failures;
 }
 

@@ -20,7 +20,10 @@ template <int BITS, bool EXACT, int RECORDS, int TIER>
 __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
    unsigned char *lds = fl3_lds;
    GenL<float> G;
-   const JobDesc *dsc = RECORDS == FL3_REC_DIRECT && a.descs ? a.descs + a.blk_job[blockIdx.x] : nullptr;
+   // independent renders: direct deposits, or job records in one cell space
+   const bool dmode = (RECORDS == FL3_REC_DIRECT || RECORDS == FL3_REC_JOB) && a.descs;
+   const int djob = dmode ? a.blk_job[blockIdx.x] : 0;
+   const JobDesc *dsc = dmode ? a.descs + djob : nullptr;
    const int nj = !dsc && a.njobs > 1 ? a.njobs : 1;
    const int jb = nj > 1 ? (int)(blockIdx.x % nj) : 0;
    // record modes leave the colours to the tile accumulation: no palette in LDS
@@ -30,15 +33,20 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
    if (RECORDS)
       for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) bh[b] = 0;
    __syncthreads();
-   // W, wid: the record matrix and walker-state layout over all jobs;
-   // Wj, wj: this walker's job-local count and index
-   const long long W = (long long)gridDim.x * blockDim.x;
-   const long long wid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   // W, wid: the record matrix and walker-state layout over all jobs (with
+   // independent renders the launch may cover only the columns from
+   // workgroup blk0 on); Wj, wj: this walker's job-local count and index
+   const long long W = a.wtot ? a.wtot : (long long)gridDim.x * blockDim.x;
+   const long long blk = (long long)a.blk0 + blockIdx.x;
+   const long long wid = blk * blockDim.x + threadIdx.x;
    const long long Wj = dsc ? (long long)dsc->nblocks * blockDim.x : (long long)(gridDim.x / nj) * blockDim.x;
    const long long wj = dsc ? (long long)(blockIdx.x - dsc->block0) * blockDim.x + threadIdx.x
                             : (long long)(blockIdx.x / nj) * blockDim.x + threadIdx.x;
    const long long nsj = dsc ? dsc->nsamples : a.nsamples;
-   const long long n = nsj / Wj + (wj < nsj % Wj ? 1 : 0);
+   long long n = nsj / Wj + (wj < nsj % Wj ? 1 : 0);
+   // independent renders with records run in chunks of rows [row0, row0 + rows_out)
+   if (dsc && RECORDS) n = n - a.row0 < 0 ? 0 : (n - a.row0 < a.rows_out ? n - a.row0 : a.rows_out);
+   const unsigned long long cell0 = dsc ? (unsigned long long)dsc->cell0 : 0ull;
    // deposit target and counters: the job's render
    void *const buckets = !dsc ? a.buckets
                          : BITS == 64 ? (void *)((double *)a.buckets + dsc->cell0 * 5)
@@ -98,11 +106,15 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
             a.rec[i * W + wid] = v;
             if ((unsigned)v != FL3_REC_NONE) atomicAdd(&bh[(unsigned)v >> a.bin_shift], 1u);
          } else if (RECORDS == FL3_REC_JOB) {
+            // independent renders: the render's first cell moves the cell
+            // into the launch's cell space, the job is the descriptor's
             const unsigned long long v = make_record<float>(G, q, fn);
-            a.rec[i * W + wid] = (unsigned)v == FL3_REC_NONE
+            const unsigned cell = (unsigned)v == FL3_REC_NONE ? FL3_REC_NONE : (unsigned)(v + cell0);
+            a.rec[i * W + wid] = cell == FL3_REC_NONE
                                      ? (unsigned long long)FL3_REC_NONE
-                                     : ((v & 0xffffffffffffull) | ((unsigned long long)jb << 48));
-            if ((unsigned)v != FL3_REC_NONE) atomicAdd(&bh[(unsigned)v >> a.bin_shift], 1u);
+                                     : ((v & 0xffff00000000ull) | cell |
+                                        ((unsigned long long)(dsc ? djob : jb) << 48));
+            if (cell != FL3_REC_NONE) atomicAdd(&bh[cell >> a.bin_shift], 1u);
          } else if (RECORDS == FL3_REC_NARROW) {
             const unsigned long long v = make_record<float>(G, q, 0);
             const unsigned cell = (unsigned)v;
@@ -120,8 +132,7 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
       for (long long i = n; i < a.rows_out; i++) ((unsigned *)a.rec)[i * W + wid] = FL3_REC_NONE;
    if (RECORDS) {
       __syncthreads();
-      for (int b = threadIdx.x; b < a.nbins; b += blockDim.x)
-         a.blk_hist[(size_t)blockIdx.x * a.nbins + b] = bh[b];
+      for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) a.blk_hist[(size_t)blk * a.nbins + b] = bh[b];
    }
    if (ws) {
       for (int k = 0; k < 4; k++) ws[k * W + wid] = __float_as_uint(p[k]);

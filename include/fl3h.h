@@ -55,15 +55,48 @@ typedef struct {
 /* Renders n frames, each exactly as fl3h_render(frame, out, field, nchan,
  * transp, stats, opts) would (same image, stats and f->rc), with small
  * renders sharing the GPU: the walkers of all temporal samples of all
- * renders below 2^24 plotted samples (one batch, production mode, the bits
- * of the first such render, no progress callback) run in one launch, then
- * their density estimation and tone maps, and their images come back in one
- * copy.  Other renders run one after another.  Returns the number of
+ * renders below 2^24 plotted samples, or 2^26 with a stepped palette (one
+ * batch, production mode, the bits of the first such render, no progress
+ * callback) run in one launch, then their density estimation and tone
+ * maps, and their images come back in one copy.  Other renders run one
+ * after another.  Returns the number of
  * requests whose rc is not 0.  No counterpart in the reference: the
  * preview loop of flam3-genome (flam3-genome.c:1020-1116) and
  * flam3-animate's frame loop (flam3-animate.c:225-327) call flam3_render
  * once per image. */
 int fl3h_render_many(fl3h_request *reqs, int n, const fl3h_render_opts *opts);
+
+/* Render server (no counterpart in the reference; for callers that render
+ * many independent images: a genome-search preview loop like
+ * flam3-genome.c:1020-1116, or flam3-animate's frames): continuous batching
+ * on one GPU.  A planner thread admits pending requests in arrival order
+ * (up to max_batch renders whose histogram + accumulator bytes fit
+ * hist_budget_bytes; the first always) and does their host work while a
+ * runner thread has the previous batch on the GPU (fl3h_render_many's
+ * shared launches).  Each request's frame, out and stats must stay valid
+ * until fl3h_server_wait returns for it; each comes out as fl3h_render(...,
+ * &opts) makes it alone. */
+typedef struct {
+   double hist_budget_bytes;   /* per batch; 0 = 1.15e9 (the 32e6 cells one shared launch bins) */
+   int max_batch;              /* renders per batch; 0 = 512 */
+   fl3h_render_opts opts;      /* for every request */
+} fl3h_server_opts;
+typedef struct fl3h_server fl3h_server;
+fl3h_server *fl3h_server_create(const fl3h_server_opts *o);   /* NULL o: defaults */
+/* Queues a flam3_render(f, out, field, nchan, transp, stats); returns its
+ * ticket (>= 0) or -1. */
+long long fl3h_server_submit(fl3h_server *s, flam3_frame *f, void *out, int field, int nchan, int transp,
+                             stat_struct *stats);
+/* Blocks until the request is done; returns what flam3_render would return
+ * and, optionally, the seconds from submit to done. */
+int fl3h_server_wait(fl3h_server *s, long long ticket, double *latency_seconds);
+int fl3h_server_done(fl3h_server *s, long long ticket);   /* 1 once finished */
+/* Batches run, renders done, and the seconds the planner spent on host
+ * work and the runner on device work (each pointer may be NULL). */
+void fl3h_server_counts(fl3h_server *s, long long *batches, long long *renders, double *prep_seconds,
+                        double *gpu_seconds);
+/* Finishes every submitted request, then stops the threads. */
+void fl3h_server_destroy(fl3h_server *s);
 
 /* flam3_render with options; flam3_render(f, ...) == fl3h_render(f, ..., NULL). */
 int fl3h_render(flam3_frame *f, void *out, int field, int nchan, int transp, stat_struct *stats,

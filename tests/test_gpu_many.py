@@ -68,16 +68,21 @@ def test_result_is_the_same_alone_and_in_any_mix(prod, order):
 
 
 def test_requests_that_cannot_share_render_alone(prod):
-    """A render above 2^24 samples (binned deposit), one with bits 64 among
-    bits-33 ones and a replay-free request with several batches each take
-    fl3h_render's own path; the shared ones around them are unaffected."""
+    """A linear-palette render above 2^24 samples (binned deposit), one with
+    bits 64 among bits-33 ones and a request with several batches each take
+    fl3h_render's own path; a stepped-palette render above 2^24 samples
+    shares the launch with saturating deposits and still equals
+    fl3h_render's binned image; the shared ones around them are
+    unaffected."""
     big = G.synthetic_preview(401, size=512, quality=100)   # 2.6e7 samples
+    big_lin = G.synthetic_preview(406, size=512, quality=100)
+    big_lin.palette_mode = "linear"
     b64 = _preview(402, size=64, quality=10)
     b64["bits"] = 64
     multi = G.synthetic_preview(403, size=64, quality=20)
     multi.passes = 2
     items = [_preview(404), {"genomes": big, "seed": "big"}, b64, {"genomes": multi, "seed": "multi"},
-             _preview(405)]
+             {"genomes": big_lin, "seed": "big-linear"}, _preview(405)]
     got = prod.render_many(items)
     for it, (img, st) in zip(items, got):
         want, ws = _alone(prod, it)
