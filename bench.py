@@ -71,6 +71,10 @@ def parse_args():
     p.add_argument("--quality", type=float, default=500.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-quality", type=float, default=60.0)
+    p.add_argument("--no-animation", action="store_true", help="skip the 120-frame animation measurement")
+    p.add_argument("--no-preview", action="store_true", help="skip the preview-serving measurement")
+    p.add_argument("--anim-frames", type=int, default=120)
+    p.add_argument("--preview-requests", type=int, default=2000)
     p.add_argument("--copy-probe", action="store_true",
                    help="after the timed steps, run k_copy_probe over 1 GiB (a plain copy kernel of known traffic: "
                         "1 GiB read, 1 GiB written) to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -135,6 +139,156 @@ def gpu_bench(args):
     finally:
         for gp in owned:
             G.free_struct(gp)
+
+
+def animation_bench(args):
+    """The 1280x720 ss2 q200 animation (100 temporal samples per frame, two
+    5-xform control points with turning affines): frames rendered one
+    flam3_render call at a time, then the same frames through the render
+    server (the host work of frame k+1 overlaps the GPU work of frame k)."""
+    import numpy as np
+    import flam3_amd as F
+    from flam3_amd import abi
+    from flam3_amd import genome as G
+
+    L = F.lib()
+    gs = G.synthetic_animation(seed=3, frames=args.anim_frames)
+    arr, owned = G.genome_array(gs)
+    n = args.anim_frames
+    h, w = gs[0].height, gs[0].width
+    imgs = [np.zeros((h, w, 3), np.uint8) for _ in range(2)]
+    try:
+        def frame(t):
+            return F.make_frame(ctypes.addressof(arr), 2, time=float(t), seed="anim-%d" % t)
+        st = abi.stat_struct()
+        f = frame(0)
+        L.flam3_render(ctypes.byref(f), imgs[0].ctypes.data_as(ctypes.c_void_p), 0, 3, 0, ctypes.byref(st))
+        samples = 0.0
+        t0 = time.perf_counter()
+        for t in range(n):
+            f = frame(t)
+            if L.flam3_render(ctypes.byref(f), imgs[t & 1].ctypes.data_as(ctypes.c_void_p), 0, 3, 0,
+                              ctypes.byref(st)) != 0:
+                raise RuntimeError("flam3_render failed: " + F.last_error())
+            samples += st.num_iters
+        plain = time.perf_counter() - t0
+        log("animation: %d frames one call at a time: %.3f s" % (n, plain))
+        frames = [frame(t) for t in range(n)]
+        outs = [np.zeros((h, w, 3), np.uint8) for _ in range(n)]
+        stats = [abi.stat_struct() for _ in range(n)]
+        srv = F.Server()
+        t0 = time.perf_counter()
+        ids = [L.fl3h_server_submit(srv._s, ctypes.byref(frames[t]), outs[t].ctypes.data_as(ctypes.c_void_p), 0, 3,
+                                    0, ctypes.byref(stats[t])) for t in range(n)]
+        for i in ids:
+            if L.fl3h_server_wait(srv._s, i, None) != 0:
+                raise RuntimeError("served frame failed: " + F.last_error())
+        served = time.perf_counter() - t0
+        srv.close()
+        log("animation: %d frames through the server: %.3f s" % (n, served))
+        return {"workload": "1280x720 ss2 q200, 100 temporal samples per frame, 2 control points of 5 xforms, "
+                            "DE radius 7; %d frames" % n,
+                "samples_per_frame": samples / n,
+                "plain_frames_per_s": n / plain, "plain_ms_per_frame": 1e3 * plain / n,
+                "served_frames_per_s": n / served, "served_ms_per_frame": 1e3 * served / n,
+                "served_samples_per_s": samples / served,
+                "note": "plain: one flam3_render call per frame, each frame's 100 temporal samples in one walker "
+                        "launch; served: the render server, host work of the next frame overlapping the GPU"}
+    finally:
+        for gp in owned:
+            G.free_struct(gp)
+
+
+def preview_bench(args):
+    """Genome-search preview serving: requests drawn from 64 seeded preview
+    genomes (2-12 xforms of the 12 workload variations, 64^2..512^2, quality
+    U[10,100], ss1), Poisson arrivals at several rates into the render
+    server; latency from submit to image ready."""
+    import numpy as np
+    import flam3_amd as F
+    from flam3_amd import abi
+    from flam3_amd import genome as G
+
+    L = F.lib()
+    pool = [G.synthetic_preview(1000 + k) for k in range(64)]
+    arrs = [G.genome_array([g]) for g in pool]
+    n = args.preview_requests
+    rng = np.random.default_rng(4)
+    picks = rng.integers(0, len(pool), n)
+
+    def build(k):
+        g = pool[picks[k]]
+        f = F.make_frame(ctypes.addressof(arrs[picks[k]][0]), 1, seed="req-%d" % k)
+        return f, np.zeros((g.height, g.width, 3), np.uint8), abi.stat_struct()
+
+    try:
+        f, img, st = build(0)
+        L.flam3_render(ctypes.byref(f), img.ctypes.data_as(ctypes.c_void_p), 0, 3, 0, ctypes.byref(st))
+        nseq = min(n, 200)
+        t0 = time.perf_counter()
+        for k in range(nseq):
+            f, img, st = build(k)
+            L.flam3_render(ctypes.byref(f), img.ctypes.data_as(ctypes.c_void_p), 0, 3, 0, ctypes.byref(st))
+        alone = (time.perf_counter() - t0) / nseq
+        mean_samples = float(np.mean([pool[p].width * pool[p].height * pool[p].quality for p in picks]))
+        sweeps = []
+        for rate in (200.0, 1000.0, 2000.0, None):
+            arrive = np.cumsum(rng.exponential(1.0 / rate, n)) if rate else np.zeros(n)
+            inputs = [build(k) for k in range(n)]
+            srv = F.Server()
+            ids = []
+            t0 = time.perf_counter()
+            for k in range(n):
+                d = t0 + arrive[k] - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                f, img, st = inputs[k]
+                ids.append(L.fl3h_server_submit(srv._s, ctypes.byref(f), img.ctypes.data_as(ctypes.c_void_p), 0, 3,
+                                                0, ctypes.byref(st)))
+            lat = []
+            for i in ids:
+                lv = ctypes.c_double(0.0)
+                if L.fl3h_server_wait(srv._s, i, ctypes.byref(lv)) != 0:
+                    raise RuntimeError("preview request failed: " + F.last_error())
+                lat.append(lv.value)
+            wall = time.perf_counter() - t0
+            batches, _ = srv.counts()
+            srv.close()
+            lat = np.array(lat)
+            sweeps.append({"offered_per_s": rate if rate else "all at once", "requests": n,
+                           "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+                           "requests_per_s": n / wall, "samples_per_s": n * mean_samples / wall, "batches": batches})
+            log("preview rate %s: p50 %.2f ms p99 %.2f ms, %.0f req/s" % (rate, sweeps[-1]["p50_ms"],
+                                                                         sweeps[-1]["p99_ms"], n / wall))
+        return {"workload": "preview requests: 64 seeded genomes (2-12 xforms, 64^2..512^2, q U[10,100], ss1), "
+                            "Poisson arrivals", "mean_samples_per_request": mean_samples,
+                "alone_ms_per_request": 1e3 * alone, "alone_requests_per_s": 1.0 / alone, "sweep": sweeps}
+    finally:
+        for arr, owned in arrs:
+            for gp in owned:
+                G.free_struct(gp)
+
+
+def cpu_preview_baseline(args):
+    """Reference flam3_render (oracle/_ref fast build, all host threads) on
+    the first 20 preview requests, one after another."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyref
+    from flam3_amd import genome as G
+
+    if not pyref.available("fast"):
+        return None
+    threads = max(1, min(16, os.cpu_count() or 1))
+    pool = [G.synthetic_preview(1000 + k) for k in range(64)]
+    picks = np.random.default_rng(4).integers(0, len(pool), args.preview_requests)[:20]
+    docs = [G.flame_document([pool[p]]) for p in picks]
+    t0 = time.perf_counter()
+    for k, xml in enumerate(docs):
+        pyref.render(xml, seed="req-%d" % k, nthreads=threads, kind="fast")
+    dt = time.perf_counter() - t0
+    return {"requests_per_s": len(docs) / dt, "ms_per_request": 1e3 * dt / len(docs), "cores": threads,
+            "kind": "reference", "sample": "first %d preview requests, serial flam3_render calls" % len(docs)}
 
 
 def cpu_baseline(args):
@@ -203,9 +357,15 @@ def main():
             "density_GBps": (st.cells * DE_BYTES_PER_CELL / (de_ms * 1e-3) / 1e9) if de_ms > 0 else None,
             "tonemap_ms": st.tonemap_ms / max(1, st.tonemap_launches)},
     }
+    if not args.no_animation:
+        out["animation"] = animation_bench(args)
+    if not args.no_preview:
+        out["preview"] = preview_bench(args)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(args)
         out["cpu_baseline"] = cb
+        if not args.no_preview and cb is not None:
+            out["preview"]["cpu_baseline"] = cpu_preview_baseline(args)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

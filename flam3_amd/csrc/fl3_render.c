@@ -10,6 +10,7 @@
  * reference.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -27,10 +28,9 @@
 
 /* Job records carry the job index in 16 bits. */
 #define MAX_BATCH_JOBS 4096
-/* Renders below this many plotted samples may share walker launches, and
- * with a stepped palette (bits 32/33) up to the second size. */
+/* Renders with a linear palette below this many plotted samples share
+ * walker launches (with per-sample atomics). */
 #define FL3_SHARED_MAX_SAMPLES (1LL << 24)
-#define FL3_SHARED_MAX_STEP_SAMPLES (1LL << 26)
 /* Histogram cells of one shared launch: its renders' joint cell space in at
  * most 8192 tiles of 4096 cells. */
 #define FL3_SHARED_MAX_CELLS (32LL * 1000 * 1000)
@@ -583,21 +583,24 @@ cleanup:
 
 /* Production renders of one batch without a progress callback are
  * prepared here.  1 = shared (its walkers join the shared launch, with the
- * bits of the other shared renders): below the size from which fl3h_render
- * bins its deposit (per-sample atomics are then what fl3h_render uses too),
- * or, with a stepped palette, below a larger size; 2 = alone, with
- * fl3h_render's launches.  0: fl3h_render itself. */
+ * bits of the other shared renders); 2 = alone, with fl3h_render's
+ * launches.  0: fl3h_render itself. */
 static int plan_kind(const render_plan *P, const fl3h_render_opts *opts, int bits) {
    double est;
    if (P->nbatches != 1 || P->replay || P->spec->progress) return 0;
    if (opts && opts->deposit == 1) return 0;  /* forced atomics may saturate: fl3h_render's second pass */
    if (opts && opts->deposit == 2) return 2;
    if (bits && P->spec->bits != bits) return 2;
+   /* a stepped palette at bits 32/33: the shared launch bins job records,
+    * which sums the same integers as fl3h_render's deposit of any size (a
+    * linear palette's records would round the blend fraction) */
+   if (P->cp.palette_mode == 0 && P->spec->bits != 64 &&
+       (long long)P->geo.width * P->geo.height <= FL3_SHARED_MAX_CELLS)
+      return 1;
+   /* else small renders, whose shared deposits are per-sample atomics as
+    * fl3h_render's are */
    est = P->cp.sample_density * pow(2.0, 2.0 * P->cp.zoom) * P->geo.image_width * P->geo.image_height;
    if (est < (double)FL3_SHARED_MAX_SAMPLES) return 1;
-   /* above it fl3h_render bins the deposit, which sums the same integers as
-    * atomics for a stepped palette (a linear one rounds the blend fraction) */
-   if (P->cp.palette_mode == 0 && P->spec->bits != 64 && est < (double)FL3_SHARED_MAX_STEP_SAMPLES) return 1;
    return 2;
 }
 
@@ -629,12 +632,21 @@ struct fl3_many {
    int nlive, bits;
    fl3d_multi_job *mj;
    int nmj;
+   fl3d_multi_job **rj;    /* per request, before they go into mj */
 };
 
 void fl3_many_free(fl3_many *B) {
    int i;
    if (!B) return;
    for (i = 0; i < B->nmj; i++) free((void *)B->mj[i].job);
+   if (B->rj)
+      for (i = 0; i < B->n; i++)
+         if (B->rj[i]) {
+            int k;
+            for (k = 0; k < B->njob[i]; k++) free((void *)B->rj[i][k].job);
+            free(B->rj[i]);
+         }
+   free(B->rj);
    if (B->P)
       for (i = 0; i < B->n; i++) {
          if (B->de) fl3_free_de_filters(&B->de[i]);
@@ -666,9 +678,81 @@ static int assign_walkers(fl3d_multi_job *m, int n, const fl3h_render_opts *opts
    return 0;
 }
 
+/* Host work of request r: plan, DE filters, temporal-sample jobs with their
+ * keys (the request's frame stream advances here) and walker counts.
+ * Leaves kind[r] -1 (failed), 0 (fl3h_render later) or 1 / 2 with the jobs
+ * in rj[r]. */
+static void prep_one(fl3_many *B, int r, const fl3h_render_opts *opts) {
+   fl3h_request *q = &B->reqs[r];
+   render_plan *Q = &B->P[r];
+   fl3d_multi_job *m;
+   randctx rc0;
+   int ts, i;
+   q->rc = 1;
+   B->kind[r] = -1;
+   if (plan_open(Q, q->frame, q->out, q->field, q->nchan, q->transp, q->stats, opts)) return;
+   B->kind[r] = plan_kind(Q, opts, 0);
+   if (B->kind[r] == 0) {
+      plan_close(Q);
+      return;
+   }
+   rc0 = q->frame->rc;
+   m = (fl3d_multi_job *)calloc((size_t)Q->nts, sizeof(fl3d_multi_job));
+   if (!m || plan_batch_de(Q, 0, &B->de[r])) {
+      free(m);
+      B->kind[r] = -1;
+      plan_close(Q);
+      return;
+   }
+   for (ts = 0; ts < Q->nts; ts++) {
+      double batch_size;
+      void *job;
+      plan_keys(Q, ts, ts);
+      if (plan_ts_job(Q, ts, 1, &job, &m[ts].bytes, &batch_size)) break;
+      m[ts].job = job;
+      m[ts].nsamples = plan_nsamples(Q, batch_size);
+      m[ts].key[0] = Q->keys[2 * ts];
+      m[ts].key[1] = Q->keys[2 * ts + 1];
+   }
+   if (ts == Q->nts && !jobs_cannot_saturate(m, ts))
+      for (i = 0; i < ts; i++) m[i].exact = 1;
+   if (ts == Q->nts && assign_walkers(m, ts, opts)) {
+      fprintf(stderr, "flam3_amd: device error: %s\n", fl3d_error());
+      ts = -1;
+   }
+   if (ts != Q->nts) {
+      for (i = 0; i < Q->nts; i++) free((void *)m[i].job);
+      free(m);
+      q->frame->rc = rc0;
+      B->kind[r] = -1;
+      fl3_free_de_filters(&B->de[r]);
+      plan_close(Q);
+      return;
+   }
+   B->rj[r] = m;
+   B->njob[r] = Q->nts;
+}
+
+typedef struct {
+   fl3_many *B;
+   const fl3h_render_opts *opts;
+   int r0, step;
+} prep_arg;
+
+static void *prep_thread(void *p) {
+   prep_arg *a = (prep_arg *)p;
+   int r;
+   for (r = a->r0; r < a->B->n; r += a->step) prep_one(a->B, r, a->opts);
+   return NULL;
+}
+
+/* Host threads for preparing requests (each request's work is independent:
+ * its own frame, frame stream and plan). */
+#define FL3_PREP_THREADS 8
+
 fl3_many *fl3_many_prepare(fl3h_request *reqs, int n, const fl3h_render_opts *opts) {
    fl3_many *B = (fl3_many *)calloc(1, sizeof(fl3_many));
-   int r, ts, i;
+   int r, i, nt;
    if (!B) return NULL;
    B->reqs = reqs;
    B->n = n;
@@ -683,72 +767,59 @@ fl3_many *fl3_many_prepare(fl3h_request *reqs, int n, const fl3h_render_opts *op
    B->job0 = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
    B->njob = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
    B->live = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
-   if (!B->P || !B->de || !B->kind || !B->job0 || !B->njob || !B->live) {
+   B->rj = (fl3d_multi_job **)calloc((size_t)(n > 0 ? n : 1), sizeof(fl3d_multi_job *));
+   if (!B->P || !B->de || !B->kind || !B->job0 || !B->njob || !B->live || !B->rj) {
       fl3_many_free(B);
       return NULL;
    }
+   nt = n < FL3_PREP_THREADS ? n : FL3_PREP_THREADS;
+   if (nt > 1) {
+      pthread_t th[FL3_PREP_THREADS];
+      prep_arg pa[FL3_PREP_THREADS];
+      int started = 0;
+      for (i = 0; i < nt; i++) {
+         pa[i].B = B;
+         pa[i].opts = opts;
+         pa[i].r0 = i;
+         pa[i].step = nt;
+      }
+      /* thread 0's share runs here; a thread that fails to start leaves
+       * its share to this thread too */
+      for (i = 1; i < nt; i++)
+         if (pthread_create(&th[i], NULL, prep_thread, &pa[i]) == 0)
+            started |= 1 << i;
+         else
+            prep_thread(&pa[i]);
+      prep_thread(&pa[0]);
+      for (i = 1; i < nt; i++)
+         if (started & (1 << i)) pthread_join(th[i], NULL);
+   } else
+      for (r = 0; r < n; r++) prep_one(B, r, opts);
+   /* in request order: the shared renders take the bits of the first of
+    * them (others run alone), and the jobs go into one array */
    for (r = 0; r < n; r++) {
-      fl3h_request *q = &reqs[r];
-      randctx rc0;
+      const int nj = B->njob[r];
       fl3d_multi_job *t;
-      render_plan *Q = &B->P[r];
-      q->rc = 1;
-      B->kind[r] = -1;
-      if (plan_open(Q, q->frame, q->out, q->field, q->nchan, q->transp, q->stats, opts)) continue;
-      B->kind[r] = plan_kind(Q, opts, B->bits);
-      if (B->kind[r] == 0) {
-         plan_close(Q);
-         continue;
+      if (B->kind[r] < 1) continue;
+      if (B->kind[r] == 1) {
+         if (!B->bits)
+            B->bits = B->P[r].spec->bits;
+         else if (B->P[r].spec->bits != B->bits)
+            B->kind[r] = 2;
       }
-      rc0 = q->frame->rc;
-      if (plan_batch_de(Q, 0, &B->de[r])) {
-         B->kind[r] = -1;
-         plan_close(Q);
-         continue;
-      }
-      t = (fl3d_multi_job *)realloc(B->mj, sizeof(fl3d_multi_job) * (size_t)(B->nmj + Q->nts));
+      t = (fl3d_multi_job *)realloc(B->mj, sizeof(fl3d_multi_job) * (size_t)(B->nmj + nj));
       if (!t) {
          B->kind[r] = -1;
-         fl3_free_de_filters(&B->de[r]);
-         plan_close(Q);
          continue;
       }
       B->mj = t;
+      memcpy(B->mj + B->nmj, B->rj[r], sizeof(fl3d_multi_job) * (size_t)nj);
+      free(B->rj[r]);
+      B->rj[r] = NULL;
       B->job0[r] = B->nmj;
-      memset(B->mj + B->nmj, 0, sizeof(fl3d_multi_job) * (size_t)Q->nts);
-      for (ts = 0; ts < Q->nts; ts++) {
-         fl3d_multi_job *m = &B->mj[B->nmj + ts];
-         double batch_size;
-         void *job;
-         plan_keys(Q, ts, ts);
-         if (plan_ts_job(Q, ts, 1, &job, &m->bytes, &batch_size)) break;
-         m->job = job;
-         m->nsamples = plan_nsamples(Q, batch_size);
-         m->key[0] = Q->keys[2 * ts];
-         m->key[1] = Q->keys[2 * ts + 1];
-      }
-      if (ts == Q->nts && B->kind[r] == 1 && !jobs_cannot_saturate(B->mj + B->nmj, ts))
-         for (i = 0; i < ts; i++) B->mj[B->nmj + i].exact = 1;
-      if (ts == Q->nts && assign_walkers(B->mj + B->nmj, ts, opts)) {
-         fprintf(stderr, "flam3_amd: device error: %s\n", fl3d_error());
-         ts = -1;
-      }
-      if (ts != Q->nts) {
-         for (i = 0; i < Q->nts; i++) {
-            free((void *)B->mj[B->nmj + i].job);
-            B->mj[B->nmj + i].job = NULL;
-         }
-         q->frame->rc = rc0;
-         B->kind[r] = -1;
-         fl3_free_de_filters(&B->de[r]);
-         plan_close(Q);
-         continue;
-      }
-      B->njob[r] = Q->nts;
-      B->nmj += Q->nts;
+      B->nmj += nj;
       if (B->kind[r] == 1) {
-         for (i = 0; i < Q->nts; i++) B->mj[B->job0[r] + i].render = B->nlive;
-         B->bits = Q->spec->bits;
+         for (i = 0; i < nj; i++) B->mj[B->job0[r] + i].render = B->nlive;
          B->live[B->nlive++] = r;
       }
    }

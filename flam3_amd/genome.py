@@ -299,6 +299,30 @@ def synthetic_preview(seed, size=None, quality=None):
                   palette=hsv_palette(rng), xforms=xs)
 
 
+def synthetic_animation(seed=3, width=1280, height=720, quality=200.0, supersample=2, temporal_samples=100,
+                        frames=120, turn=90.0):
+    """Two seeded 5-xform control points (3 of the 12 workload variations per
+    xform) at times 0 and frames-1; the second's affines are the first's
+    rotated by `turn` degrees, so log interpolation turns them a little per
+    frame.  1280x720 ss2 q200, 100 temporal samples, DE radius 7."""
+    rng = np.random.default_rng(seed)
+    xs = [_mixed_xform(rng, WORKLOAD_VARS, 3, -1.2, 1.2) for _ in range(5)]
+    pal = hsv_palette(rng)
+    c, s_ = math.cos(math.radians(turn)), math.sin(math.radians(turn))
+    out = []
+    for k in range(2):
+        xk = []
+        for x in xs:
+            a, b, cc, d, e, f = x.coefs
+            if k:  # rotate the linear part: [a c; b d] -> R [a c; b d]
+                a, b, cc, d = c * a - s_ * b, s_ * a + c * b, c * cc - s_ * d, s_ * cc + c * d
+            xk.append(Xform(weight=x.weight, color=x.color, coefs=(a, b, cc, d, e, f), variations=dict(x.variations)))
+        out.append(Genome(width=width, height=height, scale=height / 3.0, quality=quality, supersample=supersample,
+                          estimator_radius=7.0, temporal_samples=temporal_samples, palette=pal, xforms=xk,
+                          time=float(k * (frames - 1))))
+    return out
+
+
 def synthetic_8k(seed=5, width=7680, height=4320, quality=2000.0, supersample=3):
     """8 xforms + final xform + xaos matrix U[0,2]; 7680x4320 ss3 q2000,
     DE radius 15 curve 0.4 (render with bits=64)."""

@@ -54,12 +54,13 @@ typedef struct {
 
 /* Renders n frames, each exactly as fl3h_render(frame, out, field, nchan,
  * transp, stats, opts) would (same image, stats and f->rc), with small
- * renders sharing the GPU: the walkers of all temporal samples of all
- * renders below 2^24 plotted samples, or 2^26 with a stepped palette (one
- * batch, production mode, the bits of the first such render, no progress
- * callback) run in one launch, then their density estimation and tone
- * maps, and their images come back in one copy.  Other renders run one
- * after another.  Returns the number of
+ * renders sharing the GPU: the walkers of all temporal samples of the
+ * renders with a stepped palette at bits 32/33 (up to 32e6 histogram cells
+ * per launch), and of those with a linear palette below 2^24 plotted
+ * samples (one batch, production mode, the bits of the first such render,
+ * no progress callback) run in one launch, then their density estimation
+ * and tone maps, and their images come back in one copy.  Other renders run
+ * one after another.  Returns the number of
  * requests whose rc is not 0.  No counterpart in the reference: the
  * preview loop of flam3-genome (flam3-genome.c:1020-1116) and
  * flam3-animate's frame loop (flam3-animate.c:225-327) call flam3_render
