@@ -173,6 +173,18 @@ def animation_bench(args):
             samples += st.num_iters
         plain = time.perf_counter() - t0
         log("animation: %d frames one call at a time: %.3f s" % (n, plain))
+        # the reference's loop structure: one walker launch per temporal sample
+        from flam3_amd._lib import fl3h_render_opts
+        o = fl3h_render_opts(0, 0, 0, 0, 1)
+        nper = min(n, 6)
+        t0 = time.perf_counter()
+        for t in range(nper):
+            f = frame(t)
+            if L.fl3h_render(ctypes.byref(f), imgs[t & 1].ctypes.data_as(ctypes.c_void_p), 0, 3, 0, ctypes.byref(st),
+                             ctypes.byref(o)) != 0:
+                raise RuntimeError("fl3h_render failed: " + F.last_error())
+        per_sample = (time.perf_counter() - t0) / nper
+        log("animation: one launch per temporal sample: %.1f ms/frame (%d frames)" % (1e3 * per_sample, nper))
         frames = [frame(t) for t in range(n)]
         outs = [np.zeros((h, w, 3), np.uint8) for _ in range(n)]
         stats = [abi.stat_struct() for _ in range(n)]
@@ -189,11 +201,14 @@ def animation_bench(args):
         return {"workload": "1280x720 ss2 q200, 100 temporal samples per frame, 2 control points of 5 xforms, "
                             "DE radius 7; %d frames" % n,
                 "samples_per_frame": samples / n,
+                "per_sample_launch_ms_per_frame": 1e3 * per_sample,
                 "plain_frames_per_s": n / plain, "plain_ms_per_frame": 1e3 * plain / n,
                 "served_frames_per_s": n / served, "served_ms_per_frame": 1e3 * served / n,
                 "served_samples_per_s": samples / served,
-                "note": "plain: one flam3_render call per frame, each frame's 100 temporal samples in one walker "
-                        "launch; served: the render server, host work of the next frame overlapping the GPU"}
+                "note": "per_sample_launch: one walker launch per temporal sample (the reference's loop, first "
+                        "frames only); plain: one flam3_render call per frame, its 100 temporal samples in one "
+                        "walker launch; served: the render server, ~8 frames per shared launch with the host work "
+                        "of the next batch overlapping the GPU"}
     finally:
         for gp in owned:
             G.free_struct(gp)

@@ -25,7 +25,7 @@ EXPORTS = [
 
 class fl3h_render_opts(ctypes.Structure):
     _fields_ = [("replay", ctypes.c_int), ("nwalkers", ctypes.c_int), ("deposit", ctypes.c_int),
-                ("chunk_samples", ctypes.c_longlong)]
+                ("chunk_samples", ctypes.c_longlong), ("temporal_batch", ctypes.c_int)]
 
 
 class fl3h_request(ctypes.Structure):

@@ -56,6 +56,8 @@ typedef struct {
    int render;
    int walkers;
    int exact;       /* saturating deposits (a render that could reach 2^32 in a cell) */
+   int run;         /* jobs of a render with equal run share their kernel tier, as
+                       one fl3d_iterate_batch launch of that render alone would */
 } fl3d_multi_job;
 long long fl3d_job_walkers(int64_t nsamples, int njobs, int requested);
 int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fuse, int exact, int nrenders,

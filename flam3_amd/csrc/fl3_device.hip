@@ -89,7 +89,7 @@ struct Ctx {
    hipStream_t bin_stream = nullptr;
    hipEvent_t ev_walk[2] = {nullptr, nullptr}, ev_bin[2] = {nullptr, nullptr}, ev_join = nullptr;
    DevBuf buckets, accum, job, table, image, isaac, points, misc, findex;
-   DevBuf rec, rec2, sorted, wstate, bins, blkhist, blkhist2, jobtab;   // tile-binned deposit
+   DevBuf rec, rec2, sorted, wstate, bins, blkhist, blkhist2, jobtab, soff;   // tile-binned deposit
    DevBuf srcv, srcl;                  // density estimation per-source values
    unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
    void *staging = nullptr;                   // pinned host buffer for the image download
@@ -451,8 +451,9 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    if (g_ctx.wstate.ensure(sizeof(uint32_t) * 10 * (size_t)W)) return 1;
    if (g_ctx.blkhist.ensure(sizeof(unsigned) * (size_t)nblk * nbins)) return 1;
    if (nchunks > 1 && g_ctx.blkhist2.ensure(sizeof(unsigned) * (size_t)nblk * nbins)) return 1;
-   const size_t bin_words = 3 * (size_t)nbins + 8;
-   if (g_ctx.bins.ensure(sizeof(unsigned) * bin_words)) return 1;
+   const int nsuper = (nbins + (1 << FL3_SUPER_SHIFT) - 1) >> FL3_SUPER_SHIFT;
+   if (g_ctx.bins.ensure(sizeof(unsigned) * (4 * (size_t)nbins + 8))) return 1;
+   if (g_ctx.soff.ensure(sizeof(unsigned) * (size_t)nblk * nsuper)) return 1;
    unsigned *bw = (unsigned *)g_ctx.bins.p;
    BinArgs b;
    memset(&b, 0, sizeof(b));
@@ -467,6 +468,9 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    b.bin_start = bw + nbins;
    b.item_start = bw + 2 * nbins + 1;
    b.item_counter = bw + 3 * nbins + 2;
+   b.cursor = bw + 3 * nbins + 4;
+   b.soff = (unsigned *)g_ctx.soff.p;
+   b.nsuper = nsuper;
    b.shift = shift;
    b.nbins = nbins;
    b.width = g_ctx.width;
@@ -506,6 +510,7 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
       HIPCHK(hipStreamWaitEvent(bs, g_ctx.ev_walk[buf], 0));
       HIPCHK(hipMemsetAsync(b.item_counter, 0, sizeof(unsigned), bs));
       b.rec = a.rec;
+      b.final = a.rec;   // the walker buffer takes the tile-grouped records
       b.blk_hist = a.blk_hist;
       b.rows = a.rows_out;
       pk = prof_begin(ST_BIN, bs);
@@ -628,7 +633,7 @@ extern "C" long long fl3d_job_walkers(int64_t nsamples, int njobs, int requested
 }
 
 // Jobs of several renders (begun with fl3d_begin_many) in one launch.  A
-// run of consecutive jobs of one render with equal sample counts is what
+// run of consecutive jobs of one render (equal `run`) is what
 // fl3d_iterate_batch would launch for that render alone: it gets that
 // launch's kernel tier, so its arithmetic is the same.  Jobs with a stepped
 // palette write job records into the renders' joint cell space, which the
@@ -657,7 +662,7 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
    }
    for (int j = 0; j < njobs;) {
       int e = j, t = 0;
-      while (e < njobs && e - j < 4096 && jobs[e].render == jobs[j].render && jobs[e].nsamples == jobs[j].nsamples) {
+      while (e < njobs && jobs[e].render == jobs[j].render && jobs[e].run == jobs[j].run) {
          const int te = pick_tier(hs[e]);
          t = te > t ? te : t;
          e++;
@@ -775,7 +780,9 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
       if (g_ctx.wstate.ensure(sizeof(uint32_t) * 10 * (size_t)Wrec)) return 1;
       if (g_ctx.blkhist.ensure(sizeof(unsigned) * (size_t)nblk * nbins)) return 1;
       if (nchunks > 1 && g_ctx.blkhist2.ensure(sizeof(unsigned) * (size_t)nblk * nbins)) return 1;
-      if (g_ctx.bins.ensure(sizeof(unsigned) * (3 * (size_t)nbins + 8))) return 1;
+      const int nsuper = (nbins + (1 << FL3_SUPER_SHIFT) - 1) >> FL3_SUPER_SHIFT;
+      if (g_ctx.bins.ensure(sizeof(unsigned) * (4 * (size_t)nbins + 8))) return 1;
+      if (g_ctx.soff.ensure(sizeof(unsigned) * (size_t)nblk * nsuper)) return 1;
       unsigned *bw = (unsigned *)g_ctx.bins.p;
       BinArgs b;
       memset(&b, 0, sizeof(b));
@@ -788,6 +795,9 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
       b.bin_start = bw + nbins;
       b.item_start = bw + 2 * nbins + 1;
       b.item_counter = bw + 3 * nbins + 2;
+      b.cursor = bw + 3 * nbins + 4;
+      b.soff = (unsigned *)g_ctx.soff.p;
+      b.nsuper = nsuper;
       b.shift = shift;
       b.nbins = nbins;
       b.width = (int)(tcells < 0x7fffffffLL ? tcells : 0x7fffffffLL);
@@ -829,6 +839,7 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
          HIPCHK(hipStreamWaitEvent(bs, g_ctx.ev_walk[buf], 0));
          HIPCHK(hipMemsetAsync(b.item_counter, 0, sizeof(unsigned), bs));
          b.rec = ar.rec;
+         b.final = ar.rec;
          b.blk_hist = ar.blk_hist;
          b.rows = ar.rows_out;
          const int pk = prof_begin(ST_BIN, bs);

@@ -61,6 +61,7 @@ struct IterArgs {
 // item (bounds a tile cell's LDS partial sum by 2^20 * 255 < 2^32) and the
 // batch size from which the library prefers binning to atomics.
 #define FL3_MAX_BINS 8192
+#define FL3_SUPER_SHIFT 5           // tiles per super-tile = 32
 
 #define FL3_BIN_CHUNK_SAMPLES (1LL << 28)
 #define FL3_BIN_ITEM_RECORDS (1 << 20)
@@ -77,6 +78,12 @@ struct BinArgs {
    long long walkers;              // row stride of rec
    long long rows;                 // record rows in this chunk
    unsigned *bin_total, *bin_start, *item_start, *item_counter;
+   // two-pass sort: walker order -> super-tiles of FL3_SUPER tiles (sorted)
+   // -> tiles (final, the record buffer reused)
+   unsigned *soff;                 // [walker workgroup][super-tile] first slot in sorted
+   unsigned *cursor;               // [tile] next free slot of the tile in final
+   void *final;
+   int nsuper;
    int shift;                      // cells per tile = 1 << shift
    int nbins;                      // tiles
    int chunk;                      // records per accumulation work item

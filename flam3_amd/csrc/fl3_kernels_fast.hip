@@ -54,11 +54,9 @@ extern "C" int fl3k_launch_apply_fp32(const unsigned char *job, int xi, const do
 #define FL3_SCATTER_UNROLL 16
 #endif
 
-// Tile totals over the walker workgroups, and blk_hist turned into each
-// workgroup's exclusive offset within its tile.  A workgroup covers 64
-// tiles; its 16 wavefronts split the walker workgroups into 16 ranges (rows
-// of 64 consecutive tiles are read coalesced), sum their ranges, exchange
-// the range sums in LDS, then write the running offsets.
+// Tile totals over the walker workgroups.  A workgroup covers 64 tiles; its
+// 16 wavefronts split the walker workgroups into 16 ranges (rows of 64
+// consecutive tiles are read coalesced) and add their range sums in LDS.
 __global__ __launch_bounds__(1024) void k_bin_sum(BinArgs a) {
    __shared__ unsigned part[16][64];
    const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
@@ -77,27 +75,10 @@ __global__ __launch_bounds__(1024) void k_bin_sum(BinArgs a) {
    }
    part[grp][lane] = s;
    __syncthreads();
-   if (b >= a.nbins) return;
-   unsigned run = 0, total = 0;
-   for (int g = 0; g < 16; g++) {
-      if (g < grp) run += part[g][lane];
-      total += part[g][lane];
-   }
-   if (grp == 0) a.bin_total[b] = total;
-   int k = k0;
-   for (; k + 8 <= k1; k += 8) {
-      unsigned v[8];
-      for (int u = 0; u < 8; u++) v[u] = a.blk_hist[(size_t)(k + u) * a.nbins + b];
-      for (int u = 0; u < 8; u++) {
-         a.blk_hist[(size_t)(k + u) * a.nbins + b] = run;
-         run += v[u];
-      }
-   }
-   for (; k < k1; k++) {
-      unsigned *h = a.blk_hist + (size_t)k * a.nbins + b;
-      const unsigned c = *h;
-      *h = run;
-      run += c;
+   if (grp == 0 && b < a.nbins) {
+      unsigned total = 0;
+      for (int g = 0; g < 16; g++) total += part[g][lane];
+      a.bin_total[b] = total;
    }
 }
 
@@ -125,6 +106,7 @@ __global__ __launch_bounds__(1024) void k_bin_scan(BinArgs a) {
    unsigned long long run = t ? part[t - 1] : 0, irun = t ? ipart[t - 1] : 0;
    for (int b = b0; b < b1; b++) {
       a.bin_start[b] = (unsigned)run;
+      a.cursor[b] = (unsigned)run;
       a.item_start[b] = (unsigned)irun;
       run += a.bin_total[b];
       irun += (a.bin_total[b] + a.chunk - 1) / a.chunk;
@@ -144,19 +126,48 @@ __device__ __forceinline__ bool rec_none(RT v) {
    return (unsigned)v == FL3_REC_NONE;
 }
 
-// Workgroup k moves the records of walkers [256k, 256k + 256) (one column
-// block of the record matrix) to its reserved range of each tile.
+// First slot of each walker workgroup's records of each super-tile in the
+// super-tile-grouped array: workgroup s scans super-tile s's counts over the
+// walker workgroups (each row of FL3_SUPER tile counts read as one line).
+__global__ __launch_bounds__(256) void k_super_off(BinArgs a) {
+   __shared__ unsigned part[256];
+   const int s = blockIdx.x, t = threadIdx.x;
+   const int b0 = s << FL3_SUPER_SHIFT, b1 = min(a.nbins, b0 + (1 << FL3_SUPER_SHIFT));
+   const int per = (a.nblk + 255) / 256;
+   const int k0 = t * per, k1 = min(a.nblk, k0 + per);
+   unsigned sum = 0;
+   for (int k = k0; k < k1; k++)
+      for (int b = b0; b < b1; b++) sum += a.blk_hist[(size_t)k * a.nbins + b];
+   part[t] = sum;
+   __syncthreads();
+   for (int off = 1; off < 256; off <<= 1) {
+      const unsigned v = t >= off ? part[t - off] : 0;
+      __syncthreads();
+      part[t] += v;
+      __syncthreads();
+   }
+   unsigned run = a.bin_start[b0] + (t ? part[t - 1] : 0);
+   for (int k = k0; k < k1; k++) {
+      a.soff[(size_t)k * a.nsuper + s] = run;
+      for (int b = b0; b < b1; b++) run += a.blk_hist[(size_t)k * a.nbins + b];
+   }
+}
+
+// Pass 1: workgroup k moves the records of walkers [256k, 256k + 256) (one
+// column block of the record matrix) to its range of each super-tile.  With
+// at most 256 super-tiles per workgroup the partly written lines stay in L2
+// until they fill.
 template <typename RT>
-__global__ __launch_bounds__(256) void k_bin_scatter(BinArgs a) {
-   unsigned *base = (unsigned *)fl3_lds, *cnt = base + a.nbins;
-   const unsigned *off = a.blk_hist + (size_t)blockIdx.x * a.nbins;
-   for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) {
-      base[b] = a.bin_start[b] + off[b];
-      cnt[b] = 0;
+__global__ __launch_bounds__(256) void k_bin_part1(BinArgs a) {
+   __shared__ unsigned base[256], cnt[256];
+   const int ss = a.shift + FL3_SUPER_SHIFT;
+   for (int s = threadIdx.x; s < a.nsuper; s += blockDim.x) {
+      base[s] = a.soff[(size_t)blockIdx.x * a.nsuper + s];
+      cnt[s] = 0;
    }
    __syncthreads();
    const RT *rec = (const RT *)a.rec;
-   RT *sorted = (RT *)a.sorted;
+   RT *out = (RT *)a.sorted;
    const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    long long i = 0;
    for (; i + FL3_SCATTER_UNROLL <= a.rows; i += FL3_SCATTER_UNROLL) {
@@ -164,16 +175,48 @@ __global__ __launch_bounds__(256) void k_bin_scatter(BinArgs a) {
       for (int k = 0; k < FL3_SCATTER_UNROLL; k++) v[k] = rec[(i + k) * a.walkers + col];
       for (int k = 0; k < FL3_SCATTER_UNROLL; k++) {
          if (rec_none(v[k])) continue;
-         const unsigned b = rec_cell(v[k]) >> a.shift;
-         sorted[base[b] + atomicAdd(&cnt[b], 1u)] = v[k];
+         const unsigned s = rec_cell(v[k]) >> ss;
+         out[base[s] + atomicAdd(&cnt[s], 1u)] = v[k];
       }
    }
    for (; i < a.rows; i++) {
       const RT v = rec[i * a.walkers + col];
       if (rec_none(v)) continue;
-      const unsigned b = rec_cell(v) >> a.shift;
-      sorted[base[b] + atomicAdd(&cnt[b], 1u)] = v;
+      const unsigned s = rec_cell(v) >> ss;
+      out[base[s] + atomicAdd(&cnt[s], 1u)] = v;
    }
+}
+
+// Pass 2: workgroup c moves the super-tile-grouped records
+// [c * FL3_PART2_RECORDS, ...) to their tiles: ranks within the workgroup
+// from LDS counters, each tile's slots reserved with one global atomic.  The
+// order within a tile is arbitrary (the accumulation adds integers).
+#define FL3_PART2_PER 32
+#define FL3_PART2_RECORDS (256 * FL3_PART2_PER)
+template <typename RT>
+__global__ __launch_bounds__(256) void k_bin_part2(BinArgs a) {
+   unsigned *cnt = (unsigned *)fl3_lds;   // [nbins]
+   const long long total = a.bin_start[a.nbins];
+   const long long r0 = (long long)blockIdx.x * FL3_PART2_RECORDS;
+   if (r0 >= total) return;
+   for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) cnt[b] = 0;
+   __syncthreads();
+   const RT *in = (const RT *)a.sorted;
+   RT *out = (RT *)a.final;
+   RT v[FL3_PART2_PER];
+   unsigned rank[FL3_PART2_PER];
+   for (int k = 0; k < FL3_PART2_PER; k++) {
+      const long long r = r0 + k * 256 + threadIdx.x;
+      v[k] = r < total ? in[r] : (RT)FL3_REC_NONE;
+   }
+   for (int k = 0; k < FL3_PART2_PER; k++)
+      if (!rec_none(v[k])) rank[k] = atomicAdd(&cnt[rec_cell(v[k]) >> a.shift], 1u);
+   __syncthreads();
+   for (int b = threadIdx.x; b < a.nbins; b += blockDim.x)
+      if (cnt[b]) cnt[b] = atomicAdd(&a.cursor[b], cnt[b]);
+   __syncthreads();
+   for (int k = 0; k < FL3_PART2_PER; k++)
+      if (!rec_none(v[k])) out[cnt[rec_cell(v[k]) >> a.shift] + rank[k]] = v[k];
 }
 
 // Persistent workgroups take work items (tile, chunk of its records) from a
@@ -201,7 +244,7 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum(BinArgs a) {
    for (int e = threadIdx.x; e < h->nxf && e < 256; e += blockDim.x) vis[e] = (float)px[e].vis;
    const int linear = h->palette_mode == 1;
    const long long total_items = a.item_start[a.nbins];
-   const RT *sorted = (const RT *)a.sorted;
+   const RT *sorted = (const RT *)a.final;
    for (;;) {
       __syncthreads();
       if (threadIdx.x == 0) item_sh = (int)atomicAdd(a.item_counter, 1u);
@@ -299,18 +342,22 @@ extern "C" size_t fl3k_bin_lds_bytes(int shift) {
 extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_t s) {
    hipLaunchKernelGGL(k_bin_sum, dim3((a->nbins + 63) / 64), dim3(1024), 0, s, *a);
    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, *a);
-   const size_t lds_s = 2 * sizeof(unsigned) * a->nbins;
+   hipLaunchKernelGGL(k_super_off, dim3(a->nsuper), dim3(256), 0, s, *a);
+   const unsigned g2 = (unsigned)(((long long)a->walkers * a->rows + FL3_PART2_RECORDS - 1) / FL3_PART2_RECORDS);
+   const size_t lds2 = sizeof(unsigned) * a->nbins;
    const size_t lds = fl3k_bin_lds_bytes(a->shift);
    const unsigned gacc = (unsigned)ncu * (lds > 65536 ? 1 : 3);
    const int big = lds > 65536;
    if (format == 1) {
-      hipLaunchKernelGGL(k_bin_scatter<unsigned>, dim3(a->nblk), dim3(256), lds_s, s, *a);
+      hipLaunchKernelGGL(k_bin_part1<unsigned>, dim3(a->nblk), dim3(256), 0, s, *a);
+      if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned>, dim3(g2), dim3(256), lds2, s, *a);
       if (big)
          (void)hipFuncSetAttribute((const void *)k_bin_accum<unsigned, false>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       hipLaunchKernelGGL((k_bin_accum<unsigned, false>), dim3(gacc), dim3(FL3_ACC_THREADS), lds, s, *a);
    } else {
-      hipLaunchKernelGGL(k_bin_scatter<unsigned long long>, dim3(a->nblk), dim3(256), lds_s, s, *a);
+      hipLaunchKernelGGL(k_bin_part1<unsigned long long>, dim3(a->nblk), dim3(256), 0, s, *a);
+      if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned long long>, dim3(g2), dim3(256), lds2, s, *a);
       if (format == 2) {
          if (big)
             (void)hipFuncSetAttribute((const void *)k_bin_accum<unsigned long long, true>,

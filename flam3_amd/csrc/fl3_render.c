@@ -26,6 +26,9 @@
 #define FUSE_WALKER 100
 #define FUSE_EARLYCLIP 100
 
+/* Host threads that prepare requests or temporal samples (each one's work
+ * is independent: its own frame or genome and camera). */
+#define FL3_PREP_THREADS 8
 /* Job records carry the job index in 16 bits. */
 #define MAX_BATCH_JOBS 4096
 /* Renders with a linear palette below this many plotted samples share
@@ -61,6 +64,12 @@ static int flush_batch(void **jobs, size_t *bytes, int *n, int64_t nsamples, con
    }
    *n = 0;
    return rc;
+}
+
+/* Temporal samples per walker launch. */
+static int max_batch_jobs(const fl3h_render_opts *opts) {
+   if (opts && opts->temporal_batch > 0 && opts->temporal_batch < MAX_BATCH_JOBS) return opts->temporal_batch;
+   return MAX_BATCH_JOBS;
 }
 
 static int env_int(const char *name, int dflt) {
@@ -427,6 +436,72 @@ static void plan_finish(render_plan *P) {
    P->stats->render_seconds = (int)(time(NULL) - P->tstart);
 }
 
+/* Jobs of temporal samples [t0, t0 + n) built on several host threads,
+ * each with its own genome and camera scratch: what plan_ts_job(P, t0 + k,
+ * 0, ...) returns for each, and the genome values the first pass adds to
+ * the tone-map averages.  P itself is not changed. */
+typedef struct {
+   void *job;
+   size_t bytes;
+   double batch_size, vibrancy, gamma, background[3];
+   int ok;
+} ts_result;
+
+typedef struct {
+   const render_plan *P;
+   int t0, n, k0, step;
+   ts_result *res;
+} ts_arg;
+
+static void *ts_thread(void *p) {
+   ts_arg *a = (ts_arg *)p;
+   render_plan T = *a->P;   /* shallow: shared read-only tables, private scratch below */
+   stat_struct st;
+   int k;
+   memset(&T.cp, 0, sizeof(T.cp));
+   T.cam = (fl3h_camera *)calloc(1, sizeof(fl3h_camera));
+   memset(&st, 0, sizeof(st));
+   T.stats = &st;
+   for (k = a->k0; k < a->n; k += a->step) {
+      ts_result *r = &a->res[k];
+      r->ok = T.cam && plan_ts_job(&T, a->t0 + k, 0, &r->job, &r->bytes, &r->batch_size) == 0;
+      if (!r->ok) continue;
+      r->vibrancy = T.cp.vibrancy;
+      r->gamma = T.cp.gamma;
+      r->background[0] = T.cp.background[0];
+      r->background[1] = T.cp.background[1];
+      r->background[2] = T.cp.background[2];
+   }
+   free(T.cam);
+   fl3_genome_free(&T.cp);
+   return NULL;
+}
+
+static int build_ts_jobs(render_plan *P, int t0, int n, ts_result *res) {
+   pthread_t th[FL3_PREP_THREADS];
+   ts_arg ta[FL3_PREP_THREADS];
+   int nt = n < FL3_PREP_THREADS ? n : FL3_PREP_THREADS, i, started = 0;
+   for (i = 0; i < nt; i++) {
+      ta[i].P = P;
+      ta[i].t0 = t0;
+      ta[i].n = n;
+      ta[i].k0 = i;
+      ta[i].step = nt;
+      ta[i].res = res;
+   }
+   for (i = 1; i < nt; i++)
+      if (pthread_create(&th[i], NULL, ts_thread, &ta[i]) == 0)
+         started |= 1 << i;
+      else
+         ts_thread(&ta[i]);
+   if (nt > 0) ts_thread(&ta[0]);
+   for (i = 1; i < nt; i++)
+      if (started & (1 << i)) pthread_join(th[i], NULL);
+   for (i = 0; i < n; i++)
+      if (!res[i].ok) return 1;
+   return 0;
+}
+
 int fl3h_render(flam3_frame *spec, void *out, int field, int nchan, int transp, stat_struct *stats,
                 const fl3h_render_opts *opts) {
    render_plan P;
@@ -464,7 +539,65 @@ int fl3h_render(flam3_frame *spec, void *out, int field, int nchan, int transp, 
             batch_bad = 0.0;
             if (fl3d_clear_buckets()) goto devfail_de;
          }
-         for (ts = 0; ts < nts; ts++) {
+         ts = 0;
+         if (pass == 0 && !P.replay && !spec->progress && nts >= 4) {
+            /* temporal samples built on several threads, then launched in
+             * order exactly as below */
+            ts_result *res = (ts_result *)calloc((size_t)nts, sizeof(ts_result));
+            int k, failed;
+            void *last = NULL;
+            size_t lb;
+            double lbs;
+            if (!res) {
+               fl3_free_de_filters(&de);
+               goto fail;
+            }
+            failed = build_ts_jobs(&P, batch_num * nts, nts, res);
+            /* the plan's genome, camera and geometry as the last sample left them */
+            if (!failed) failed = plan_ts_job(&P, batch_num * nts + nts - 1, 0, &last, &lb, &lbs);
+            free(last);
+            for (ts = 0; !failed && ts < nts; ts++) {
+               const ts_result *r = &res[ts];
+               double bad = 0.0;
+               int sat = 0;
+               int64_t nsamp;
+               stats->num_iters += r->batch_size;
+               P.vibrancy += r->vibrancy;
+               P.gamma += r->gamma;
+               P.background[0] += r->background[0];
+               P.background[1] += r->background[1];
+               P.background[2] += r->background[2];
+               P.vib_gam_n++;
+               plan_keys(&P, ts, batch_num * nts + ts);
+               nsamp = plan_nsamples(&P, r->batch_size);
+               if (bn > 0 && (nsamp != bns || bn == max_batch_jobs(opts))) {
+                  if (flush_batch(bjobs, bbytes, &bn, bns, P.keys + 2 * bts0, exact, opts, &bad, &sat)) {
+                     failed = 2;
+                     break;
+                  }
+                  if (sat) saturated = 1;
+               }
+               if (bn == 0) {
+                  bts0 = ts;
+                  bns = nsamp;
+               }
+               bjobs[bn] = r->job;
+               bbytes[bn] = r->bytes;
+               bn++;
+               res[ts].job = NULL;
+               P.badvals += bad;
+               batch_bad += bad;
+            }
+            for (k = 0; k < nts; k++) free(res[k].job);
+            free(res);
+            if (failed == 2) goto devfail_de;
+            if (failed) {
+               fl3_free_de_filters(&de);
+               goto fail;
+            }
+            ts = nts;
+         }
+         for (; ts < nts; ts++) {
             const int tidx = batch_num * nts + ts;
             void *job;
             size_t job_bytes;
@@ -496,7 +629,7 @@ int fl3h_render(flam3_frame *spec, void *out, int field, int nchan, int transp, 
                /* Temporal samples with the same sample count go to the GPU
                 * together, one launch for the whole run of them. */
                nsamp = plan_nsamples(&P, batch_size);
-               if (bn > 0 && (nsamp != bns || bn == MAX_BATCH_JOBS)) {
+               if (bn > 0 && (nsamp != bns || bn == max_batch_jobs(opts))) {
                   if (flush_batch(bjobs, bbytes, &bn, bns, P.keys + 2 * bts0, exact, opts, &bad, &sat))
                      goto devfail_de;
                   if (sat) saturated = 1;
@@ -669,10 +802,13 @@ static int assign_walkers(fl3d_multi_job *m, int n, const fl3h_render_opts *opts
    while (ts < n) {
       int e = ts;
       long long w;
-      while (e < n && e - ts < MAX_BATCH_JOBS && m[e].nsamples == m[ts].nsamples) e++;
+      while (e < n && e - ts < max_batch_jobs(opts) && m[e].nsamples == m[ts].nsamples) e++;
       w = fl3d_job_walkers(m[ts].nsamples, e - ts, opts ? opts->nwalkers : 0);
       if (w < 256) return 1;
-      for (i = ts; i < e; i++) m[i].walkers = (int)w;
+      for (i = ts; i < e; i++) {
+         m[i].walkers = (int)w;
+         m[i].run = ts;
+      }
       ts = e;
    }
    return 0;
@@ -745,10 +881,6 @@ static void *prep_thread(void *p) {
    for (r = a->r0; r < a->B->n; r += a->step) prep_one(a->B, r, a->opts);
    return NULL;
 }
-
-/* Host threads for preparing requests (each request's work is independent:
- * its own frame, frame stream and plan). */
-#define FL3_PREP_THREADS 8
 
 fl3_many *fl3_many_prepare(fl3h_request *reqs, int n, const fl3h_render_opts *opts) {
    fl3_many *B = (fl3_many *)calloc(1, sizeof(fl3_many));
@@ -948,7 +1080,7 @@ static int many_large(fl3_many *B, int r) {
          fl3d_iter_params ip;
          double bad = 0.0;
          int sat = 0;
-         for (e = ts; e < B->njob[r] && e - ts < MAX_BATCH_JOBS && m[e].nsamples == m[ts].nsamples; e++) {
+         for (e = ts; e < B->njob[r] && e - ts < max_batch_jobs(opts) && m[e].nsamples == m[ts].nsamples; e++) {
             jobs[e - ts] = m[e].job;
             bytes[e - ts] = m[e].bytes;
             keys[2 * (e - ts)] = m[e].key[0];

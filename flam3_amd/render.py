@@ -35,13 +35,13 @@ def make_frame(genome_ptr, ngenomes, *, bits=33, bpc=1, earlyclip=0, time=0.0, n
 
 
 def render_frame(frame, width, height, *, nchan=3, transp=0, field=0, replay=False, nwalkers=0, deposit=0,
-                 chunk_samples=0):
+                 chunk_samples=0, temporal_batch=0):
     """Run fl3h_render on a prepared frame; returns (image, stats)."""
     bpc = frame.bytes_per_channel
     dtype = np.uint16 if bpc == 2 else np.uint8
     img = np.zeros((height, width, nchan), dtype=dtype)
     stats = abi.stat_struct()
-    opts = fl3h_render_opts(1 if replay else 0, int(nwalkers), int(deposit), int(chunk_samples))
+    opts = fl3h_render_opts(1 if replay else 0, int(nwalkers), int(deposit), int(chunk_samples), int(temporal_batch))
     rc = lib().fl3h_render(ctypes.byref(frame), img.ctypes.data_as(ctypes.c_void_p), field, nchan, transp,
                            ctypes.byref(stats), ctypes.byref(opts))
     check(rc, "flam3_render")
@@ -50,7 +50,8 @@ def render_frame(frame, width, height, *, nchan=3, transp=0, field=0, replay=Fal
 
 
 def render(genomes, *, nchan=3, transp=0, bits=33, bpc=1, earlyclip=0, time=0.0, nthreads=1,
-           sub_batch_size=10000, seed="flam3", replay=False, nwalkers=0, field=0, deposit=0, chunk_samples=0):
+           sub_batch_size=10000, seed="flam3", replay=False, nwalkers=0, field=0, deposit=0, chunk_samples=0,
+           temporal_batch=0):
     """Render Genome descriptions (one still, or an animation frame at
     ``time`` when several control points are given)."""
     if isinstance(genomes, Genome):
@@ -61,13 +62,13 @@ def render(genomes, *, nchan=3, transp=0, bits=33, bpc=1, earlyclip=0, time=0.0,
                        time=time, nthreads=nthreads, sub_batch_size=sub_batch_size, seed=seed)
         return render_frame(f, genomes[0].width, genomes[0].height, nchan=nchan, transp=transp,
                             field=field, replay=replay, nwalkers=nwalkers, deposit=deposit,
-                            chunk_samples=chunk_samples)
+                            chunk_samples=chunk_samples, temporal_batch=temporal_batch)
     finally:
         for gp in owned:
             lib().fl3h_genome_free(gp)
 
 
-def render_many(items, *, nwalkers=0, deposit=0):
+def render_many(items, *, nwalkers=0, deposit=0, temporal_batch=0):
     """Render several independent images with fl3h_render_many (small ones
     share the GPU).  items: dicts with the keys of ``render`` (``genomes``
     required).  Returns [(image, stats)] in order; raises if any failed."""
@@ -98,7 +99,7 @@ def render_many(items, *, nwalkers=0, deposit=0):
             reqs[i].nchan = nchan
             reqs[i].transp = it.get("transp", 0)
             reqs[i].stats = ctypes.cast(ctypes.byref(st), ctypes.c_void_p)
-        opts = fl3h_render_opts(0, int(nwalkers), int(deposit), 0)
+        opts = fl3h_render_opts(0, int(nwalkers), int(deposit), 0, int(temporal_batch))
         nfail = lib().fl3h_render_many(reqs, len(items), ctypes.byref(opts))
         if nfail:
             raise RuntimeError("fl3h_render_many: %d of %d renders failed: %s" % (nfail, len(items), last_error()))

@@ -23,9 +23,9 @@ class Ticket:
 
 
 class Server:
-    def __init__(self, hist_budget_bytes=0.0, max_batch=0, nwalkers=0, deposit=0):
+    def __init__(self, hist_budget_bytes=0.0, max_batch=0, nwalkers=0, deposit=0, temporal_batch=0):
         o = fl3h_server_opts(float(hist_budget_bytes), int(max_batch),
-                             fl3h_render_opts(0, int(nwalkers), int(deposit), 0))
+                             fl3h_render_opts(0, int(nwalkers), int(deposit), 0, int(temporal_batch)))
         self._s = lib().fl3h_server_create(ctypes.byref(o))
         if not self._s:
             raise RuntimeError("fl3h_server_create failed")

@@ -40,6 +40,9 @@ typedef struct {
                       for large batches), 1 per-sample atomics, 2 tile-binned */
    long long chunk_samples; /* binned deposit: samples per chunk (rounded up to
                       a multiple of the walker count), 0 = library choice */
+   int temporal_batch; /* production: temporal samples per walker launch, 0 = all
+                      of a batch (1 = one launch per temporal sample, the
+                      reference's rect.c:754-931 loop) */
 } fl3h_render_opts;
 
 /* One render of fl3h_render_many: flam3_render's arguments and its return
