@@ -410,6 +410,22 @@ static int upload_job_tables(const fl3_job_hdr *const *h, int n, int nxf_max, Bi
    return 0;
 }
 
+// Records per LDS round of the packed 4-byte-record accumulation: the
+// count (16 bits) and the R and G sums (24 bits) must not overflow; a
+// multiple of the records one pass of the workgroup loads; 0 if the palette
+// is too bright for a useful round.
+static int narrow_round(const fl3_job_hdr *const *h, int n) {
+   double maxd = 255.0;
+   for (int j = 0; j < n; j++) {
+      const double *dm = (const double *)((const unsigned char *)h[j] + h[j]->off_dmap);
+      for (int e = 0; e < 1024; e++) maxd = dm[e] > maxd ? dm[e] : maxd;
+   }
+   const long long pass = fl3k_bin_pass_records();
+   long long lim = (long long)(16777215.0 / (maxd + 1.0));
+   if (lim > 65535) lim = 65535;
+   return (int)(lim / pass * pass);
+}
+
 // Largest records per accumulation work item: a cell's sums must fit 32
 // bits (and the packed pairs must not carry): at most 2^20 records, fewer
 // if a palette entry exceeds 255.
@@ -476,6 +492,7 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    b.width = g_ctx.width;
    b.height = g_ctx.height;
    b.chunk = item_records(js.h, js.n);
+   b.round = narrow ? narrow_round(js.h, js.n) : 0;
    if (format == 2 && upload_job_tables(js.h, js.n, js.nxf_max, &b)) return 1;
    a.rec_mode = narrow ? 2 : (format == 2 ? 3 : 1);
    a.wstate = (uint32_t *)g_ctx.wstate.p;

@@ -87,6 +87,7 @@ struct BinArgs {
    int shift;                      // cells per tile = 1 << shift
    int nbins;                      // tiles
    int chunk;                      // records per accumulation work item
+   int round;                      // 4-byte records: records per LDS round (0: not packed)
    int width, height;
    // job records: palette (floor and float) and xform visibility per job
    const uint32_t *dmapu_all;      // [job][1024]

@@ -335,6 +335,105 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum(BinArgs a) {
    }
 }
 
+// Accumulation of 4-byte records (visibility 1, stepped palette: a record
+// adds the palette entry and 255 hits).  The tile's cells live in LDS as two
+// packed 64-bit words, R:24 | G:24 | count:16 and B:32 | A:32, so a record
+// costs two LDS atomics; a round of at most `round` records keeps every
+// field below its width, after which each thread folds the CPT cells it
+// owns into registers and clears them.  The tile is then added to the
+// histogram from the registers (non-atomic saturating add if the tile has
+// one work item).
+template <int CPT>
+__global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_narrow(BinArgs a) {
+   const fl3_job_hdr *h = (const fl3_job_hdr *)a.job;
+   const int tile_cells = CPT * FL3_ACC_THREADS;
+   unsigned long long *p0 = (unsigned long long *)fl3_lds;   // tile_cells
+   unsigned long long *p1 = p0 + tile_cells;                  // tile_cells
+   uint32_t *dmapu = (uint32_t *)(p1 + tile_cells);           // 1024
+   __shared__ int item_sh;
+   const double *pd = (const double *)(a.job + h->off_dmap);
+   for (int e = threadIdx.x; e < 1024; e += blockDim.x) dmapu[e] = (uint32_t)pd[e];
+   for (int e = threadIdx.x; e < tile_cells; e += blockDim.x) {
+      p0[e] = 0;
+      p1[e] = 0;
+   }
+   const long long total_items = a.item_start[a.nbins];
+   const unsigned *sorted = (const unsigned *)a.final;
+   const long long step = (long long)FL3_ACC_UNROLL * FL3_ACC_THREADS;
+   for (;;) {
+      __syncthreads();
+      if (threadIdx.x == 0) item_sh = (int)atomicAdd(a.item_counter, 1u);
+      __syncthreads();
+      const long long item = item_sh;
+      if (item >= total_items) break;
+      int lo = 0, hi = a.nbins - 1;
+      while (lo < hi) {  // last tile whose first item is <= item
+         const int mid = (lo + hi + 1) >> 1;
+         if ((long long)a.item_start[mid] <= item) lo = mid;
+         else hi = mid - 1;
+      }
+      const int b = lo;
+      const long long k = item - a.item_start[b];
+      const long long r0 = (long long)a.bin_start[b] + k * a.chunk;
+      const long long r1 = min((long long)a.bin_start[b + 1], r0 + a.chunk);
+      const bool single = (a.item_start[b + 1] - a.item_start[b]) == 1;
+      const unsigned cell0 = (unsigned)b << a.shift;
+      uint32_t acc[CPT][5];
+      for (int j = 0; j < CPT; j++)
+         for (int c = 0; c < 5; c++) acc[j][c] = 0;
+      for (long long rs = r0; rs < r1; rs += a.round) {
+         const long long re = min(rs + a.round, r1);
+         for (long long rb = rs; rb < re; rb += step) {
+            unsigned v[FL3_ACC_UNROLL];
+            for (int u = 0; u < FL3_ACC_UNROLL; u++) {
+               const long long r = rb + u * FL3_ACC_THREADS + threadIdx.x;
+               v[u] = r < re ? sorted[r] : FL3_REC_NONE;
+            }
+            for (int u = 0; u < FL3_ACC_UNROLL; u++) {
+               if (v[u] == FL3_REC_NONE) continue;
+               const unsigned local = (v[u] & 0xffffffu) - cell0;
+               const uint32_t *du = dmapu + (v[u] >> 24) * 4;
+               atomicAdd(&p0[local], (unsigned long long)du[0] | ((unsigned long long)du[1] << 24) | (1ull << 48));
+               atomicAdd(&p1[local], (unsigned long long)du[2] | ((unsigned long long)du[3] << 32));
+            }
+         }
+         __syncthreads();
+         for (int j = 0; j < CPT; j++) {
+            const int e = threadIdx.x + j * FL3_ACC_THREADS;
+            const unsigned long long x0 = p0[e], x1 = p1[e];
+            p0[e] = 0;
+            p1[e] = 0;
+            acc[j][0] += (uint32_t)(x0 & 0xffffffu);
+            acc[j][1] += (uint32_t)((x0 >> 24) & 0xffffffu);
+            acc[j][2] += (uint32_t)x1;
+            acc[j][3] += (uint32_t)(x1 >> 32);
+            acc[j][4] += 255u * (uint32_t)(x0 >> 48);
+         }
+         __syncthreads();
+      }
+      uint32_t *bk = (uint32_t *)a.buckets + (size_t)cell0 * 5;
+      const long long ncells = (long long)a.width * a.height;
+      const int nvalid = (int)min((long long)tile_cells, ncells - (long long)cell0);
+      for (int j = 0; j < CPT; j++) {
+         const int e = threadIdx.x + j * FL3_ACC_THREADS;
+         if (e >= nvalid) continue;
+         for (int c = 0; c < 5; c++) {
+            const uint32_t v = acc[j][c];
+            if (!v) continue;
+            uint32_t *dst = bk + (size_t)e * 5 + c;
+            if (single) {
+               const unsigned long long s2 = (unsigned long long)*dst + v;
+               *dst = s2 > 0xffffffffull ? 0xffffffffu : (uint32_t)s2;
+            } else
+               sat_add(dst, v);
+         }
+      }
+   }
+}
+
+extern "C" size_t fl3k_bin_narrow_lds_bytes(int shift) { return (size_t)16 * ((size_t)1 << shift) + 4 * 1024 + 64; }
+extern "C" int fl3k_bin_pass_records(void) { return FL3_ACC_UNROLL * FL3_ACC_THREADS; }
+
 extern "C" size_t fl3k_bin_lds_bytes(int shift) {
    return (size_t)20 * ((size_t)1 << shift) + 8 * 1024 + 4 * 256 + 64;
 }
@@ -348,7 +447,19 @@ extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_
    const size_t lds = fl3k_bin_lds_bytes(a->shift);
    const unsigned gacc = (unsigned)ncu * (lds > 65536 ? 1 : 3);
    const int big = lds > 65536;
-   if (format == 1) {
+   const int cpt = (1 << a->shift) / FL3_ACC_THREADS;
+   if (format == 1 && a->round > 0 && (cpt == 2 || cpt == 4)) {
+      hipLaunchKernelGGL(k_bin_part1<unsigned>, dim3(a->nblk), dim3(256), 0, s, *a);
+      if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned>, dim3(g2), dim3(256), lds2, s, *a);
+      const size_t ln = fl3k_bin_narrow_lds_bytes(a->shift);
+      const unsigned gn = (unsigned)ncu * (ln > 65536 ? 2 : 3);
+      const void *kf = cpt == 4 ? (const void *)k_bin_accum_narrow<4> : (const void *)k_bin_accum_narrow<2>;
+      if (ln > 65536) (void)hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ln);
+      if (cpt == 4)
+         hipLaunchKernelGGL(k_bin_accum_narrow<4>, dim3(gn), dim3(FL3_ACC_THREADS), ln, s, *a);
+      else
+         hipLaunchKernelGGL(k_bin_accum_narrow<2>, dim3(gn), dim3(FL3_ACC_THREADS), ln, s, *a);
+   } else if (format == 1) {
       hipLaunchKernelGGL(k_bin_part1<unsigned>, dim3(a->nblk), dim3(256), 0, s, *a);
       if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned>, dim3(g2), dim3(256), lds2, s, *a);
       if (big)

@@ -19,6 +19,7 @@ int fl3k_launch_apply_fp32_tier(const unsigned char *job, int xi, const double *
                                 unsigned long long key, size_t lds, hipStream_t s, int tier);
 /* format: 0 8-byte records, 1 4-byte records, 2 8-byte job records */
 int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_t s);
+int fl3k_bin_pass_records(void);   // records one accumulation workgroup loads per pass
 int fl3k_launch_copy_probe(const void *src, void *dst, long long n16, hipStream_t s);
 size_t fl3k_bin_lds_bytes(int shift);
 int fl3k_launch_replay(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int bits);
