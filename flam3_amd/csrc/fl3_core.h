@@ -38,7 +38,7 @@ __device__ __forceinline__ float m_tan(float x) {
 __device__ __forceinline__ float m_atan2(float y, float x) {
    const float ax = fabsf(x), ay = fabsf(y);
    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
-   const float t = mx > 0.0f ? mn / mx : 0.0f;
+   const float t = mx > 0.0f ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f;
    const float s = t * t;
    float p = 0.00681174332f;
    p = p * s - 0.0336040774f;
@@ -346,6 +346,18 @@ __device__ __forceinline__ void fl3_var_tier(int v, R w, const XfL<R> &X, Pre<R>
    }
 }
 
+// The variation inputs made opaque at each step of a variation loop: they
+// do not change inside the loop, and without this the compiler hoists the
+// invariant part of every case of the switch (sines, cosines, roots of all
+// the tier's variations) out of the loop and evaluates them all at every
+// iteration of the walker, used or not.
+template <typename R>
+__device__ __forceinline__ void opaque_pre(Pre<R> &f) {
+   if constexpr (sizeof(R) == 4)
+      asm volatile("" : "+v"(f.tx), "+v"(f.ty), "+v"(f.sumsq), "+v"(f.sq), "+v"(f.atan), "+v"(f.sina),
+                   "+v"(f.cosa), "+v"(f.atanyx));
+}
+
 // UNI: fn is the same in every lane of the wavefront (the final xform), so
 // the loop runs over that xform's own variations only.
 template <typename R, class RNG, int TIER = FL3_TIER_FULL, bool UNI = false>
@@ -391,6 +403,7 @@ __device__ bool apply_xform(const GenL<R> &G, int fn, const R *p, R *q, RNG &rng
          const R w = wrow[u];
          uint32_t o = xoff;
          asm volatile("" : "+v"(o));
+         opaque_pre(f);
          fl3_var_tier<R, RNG, TIER>(v, w, *(const XfL<R> *)(fl3_lds + o), f, rng);
       }
    } else {
@@ -399,6 +412,7 @@ __device__ bool apply_xform(const GenL<R> &G, int fn, const R *p, R *q, RNG &rng
          const R w = wrow[u];
          uint32_t o = xoff;
          asm volatile("" : "+v"(o));
+         opaque_pre(f);
          fl3_var_tier<R, RNG, TIER>(v, w, *(const XfL<R> *)(fl3_lds + o), f, rng);
       }
    }
