@@ -413,7 +413,12 @@ __device__ bool apply_xform(const GenL<R> &G, int fn, const R *p, R *q, RNG &rng
          uint32_t o = xoff;
          asm volatile("" : "+v"(o));
          opaque_pre(f);
+         const R s0 = f.p0, s1 = f.p1;
          fl3_var_tier<R, RNG, TIER>(v, w, *(const XfL<R> *)(fl3_lds + o), f, rng);
+         if (sizeof(R) == 4) {  // FP32 cases ran in every lane: keep the lanes that use the variation
+            f.p0 = w != R(0) ? f.p0 : s0;
+            f.p1 = w != R(0) ? f.p1 : s1;
+         }
       }
    }
    if (flags & FL3_HAS_POST) {

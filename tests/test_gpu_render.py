@@ -119,8 +119,11 @@ def test_production_render_within_noise_floor(prod, ref, mk, kw):
     floor = rms(a, b)
     err = 0.5 * (rms(got, a) + rms(got, b))
     assert err <= 1.15 * floor, (err, floor)
+    # brightness: two reference seeds can agree closely by chance; the
+    # seed-to-seed spread of the mean is ~1.5% for these small images (measured
+    # over 8 GPU seeds of xaos_bits64: sd 0.2 at a mean of 13.3)
     m = 0.5 * (a.mean() + b.mean())
-    assert abs(got.mean() - m) <= 3 * abs(a.mean() - b.mean()) + 0.01 * m + 0.05, (got.mean(), a.mean(), b.mean())
+    assert abs(got.mean() - m) <= 3 * abs(a.mean() - b.mean()) + 0.03 * m + 0.05, (got.mean(), a.mean(), b.mean())
     g.estimator_radius = 0.0
     hq = mk()
     hq.estimator_radius = 0.0
@@ -133,7 +136,7 @@ def test_production_render_within_noise_floor(prod, ref, mk, kw):
     # the log density is concave, so a low-quality image is darker than a
     # high-quality one on average: compare brightness at the same quality
     m0 = 0.5 * (a0.mean() + b0.mean())
-    assert abs(got0.mean() - m0) <= 3 * abs(a0.mean() - b0.mean()) + 0.01 * m0 + 0.05, (got0.mean(), a0.mean(),
+    assert abs(got0.mean() - m0) <= 3 * abs(a0.mean() - b0.mean()) + 0.03 * m0 + 0.05, (got0.mean(), a0.mean(),
                                                                                          b0.mean())
 
 
