@@ -89,7 +89,7 @@ struct Ctx {
    hipStream_t bin_stream = nullptr;
    hipEvent_t ev_walk[2] = {nullptr, nullptr}, ev_bin[2] = {nullptr, nullptr}, ev_join = nullptr;
    DevBuf buckets, accum, job, table, image, isaac, points, misc, findex;
-   DevBuf rec, rec2, sorted, wstate, bins, blkhist, blkhist2, jobtab, soff;   // tile-binned deposit
+   DevBuf rec, rec2, sorted, wstate, bins, blkhist, blkhist2, jobtab, soff, bintot, bintot2;   // tile-binned deposit
    DevBuf srcv, srcl;                  // density estimation per-source values
    unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
    void *staging = nullptr;                   // pinned host buffer for the image download
@@ -467,6 +467,8 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    if (g_ctx.wstate.ensure(sizeof(uint32_t) * 10 * (size_t)W)) return 1;
    if (g_ctx.blkhist.ensure(sizeof(unsigned) * (size_t)nblk * nbins)) return 1;
    if (nchunks > 1 && g_ctx.blkhist2.ensure(sizeof(unsigned) * (size_t)nblk * nbins)) return 1;
+   if (g_ctx.bintot.ensure(sizeof(unsigned) * (size_t)nbins) || g_ctx.bintot2.ensure(sizeof(unsigned) * (size_t)nbins))
+      return 1;
    const int nsuper = (nbins + (1 << FL3_SUPER_SHIFT) - 1) >> FL3_SUPER_SHIFT;
    if (g_ctx.bins.ensure(sizeof(unsigned) * (4 * (size_t)nbins + 8))) return 1;
    if (g_ctx.soff.ensure(sizeof(unsigned) * (size_t)nblk * nsuper)) return 1;
@@ -515,7 +517,9 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
       a.resume = done > 0;
       a.rec = (unsigned long long *)(buf ? g_ctx.rec2.p : g_ctx.rec.p);
       a.blk_hist = (unsigned *)(buf ? g_ctx.blkhist2.p : g_ctx.blkhist.p);
+      a.bin_total = (unsigned *)(buf ? g_ctx.bintot2.p : g_ctx.bintot.p);
       if (k >= 2) HIPCHK(hipStreamWaitEvent(ws, g_ctx.ev_bin[buf], 0));
+      HIPCHK(hipMemsetAsync(a.bin_total, 0, sizeof(unsigned) * (size_t)nbins, ws));
       int pk = prof_begin(ST_ITER, ws);
       int lrc = fl3k_launch_iter_fast(&a, (unsigned)nblk, lds_rec, ws, g_ctx.bits, 1, js.tier);
       prof_end(pk, ws);
@@ -529,6 +533,8 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
       b.rec = a.rec;
       b.final = a.rec;   // the walker buffer takes the tile-grouped records
       b.blk_hist = a.blk_hist;
+      b.bin_total = a.bin_total;
+      b.totals_in = 1;
       b.rows = a.rows_out;
       pk = prof_begin(ST_BIN, bs);
       lrc = fl3k_launch_bin(&b, g_ctx.ncu, format, bs);
@@ -797,6 +803,9 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
       if (g_ctx.wstate.ensure(sizeof(uint32_t) * 10 * (size_t)Wrec)) return 1;
       if (g_ctx.blkhist.ensure(sizeof(unsigned) * (size_t)nblk * nbins)) return 1;
       if (nchunks > 1 && g_ctx.blkhist2.ensure(sizeof(unsigned) * (size_t)nblk * nbins)) return 1;
+      if (g_ctx.bintot.ensure(sizeof(unsigned) * (size_t)nbins) ||
+          g_ctx.bintot2.ensure(sizeof(unsigned) * (size_t)nbins))
+         return 1;
       const int nsuper = (nbins + (1 << FL3_SUPER_SHIFT) - 1) >> FL3_SUPER_SHIFT;
       if (g_ctx.bins.ensure(sizeof(unsigned) * (4 * (size_t)nbins + 8))) return 1;
       if (g_ctx.soff.ensure(sizeof(unsigned) * (size_t)nblk * nsuper)) return 1;
@@ -839,7 +848,9 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
          ar.resume = k > 0;
          ar.rec = (unsigned long long *)(buf ? g_ctx.rec2.p : g_ctx.rec.p);
          ar.blk_hist = (unsigned *)(buf ? g_ctx.blkhist2.p : g_ctx.blkhist.p);
+         ar.bin_total = (unsigned *)(buf ? g_ctx.bintot2.p : g_ctx.bintot.p);
          if (k >= 2) HIPCHK(hipStreamWaitEvent(ws, g_ctx.ev_bin[buf], 0));
+         HIPCHK(hipMemsetAsync(ar.bin_total, 0, sizeof(unsigned) * (size_t)nbins, ws));
          for (int g = MG_REC; g < MG_N; g++) {
             if (!gblocks[g]) continue;
             ar.blk_job = blk_dev + gstart[g];
@@ -858,6 +869,8 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
          b.rec = ar.rec;
          b.final = ar.rec;
          b.blk_hist = ar.blk_hist;
+         b.bin_total = ar.bin_total;
+         b.totals_in = 1;
          b.rows = ar.rows_out;
          const int pk = prof_begin(ST_BIN, bs);
          const int lrc = fl3k_launch_bin(&b, g_ctx.ncu, 2, bs);

@@ -48,6 +48,7 @@ struct IterArgs {
    unsigned long long *rec;   // 8-byte records, or 4-byte ones in the same buffer
    long long rows_out;
    unsigned *blk_hist;        // [workgroup][tile] record counts
+   unsigned *bin_total;       // [tile] record counts over all workgroups (added atomically)
    int bin_shift, nbins;
    unsigned hist_off;         // LDS byte offset of the workgroup's tile counts
    // walker state kept between launches of one batch (10 words per walker,
@@ -71,7 +72,7 @@ struct IterArgs {
 struct BinArgs {
    const unsigned char *job;       // for palette / visibility tables
    void *buckets;
-   const void *rec;                // records in walker order (4 or 8 bytes)
+   const void *rec;                // records in walker order, one block of rows x 256 per workgroup
    void *sorted;                   // records grouped by tile
    unsigned *blk_hist;             // [walker workgroup][tile] counts
    int nblk;                       // walker workgroups (256 walkers each)
@@ -84,6 +85,7 @@ struct BinArgs {
    unsigned *cursor;               // [tile] next free slot of the tile in final
    void *final;
    int nsuper;
+   int totals_in;                  // bin_total already summed (by the walkers)
    int shift;                      // cells per tile = 1 << shift
    int nbins;                      // tiles
    int chunk;                      // records per accumulation work item

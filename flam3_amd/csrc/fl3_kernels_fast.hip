@@ -168,11 +168,10 @@ __global__ __launch_bounds__(256) void k_bin_part1(BinArgs a) {
    __syncthreads();
    const RT *rec = (const RT *)a.rec;
    RT *out = (RT *)a.sorted;
-   const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    long long i = 0;
    for (; i + FL3_SCATTER_UNROLL <= a.rows; i += FL3_SCATTER_UNROLL) {
       RT v[FL3_SCATTER_UNROLL];
-      for (int k = 0; k < FL3_SCATTER_UNROLL; k++) v[k] = rec[(i + k) * a.walkers + col];
+      for (int k = 0; k < FL3_SCATTER_UNROLL; k++) v[k] = rec[(blockIdx.x * a.rows + i + k) * 256 + threadIdx.x];
       for (int k = 0; k < FL3_SCATTER_UNROLL; k++) {
          if (rec_none(v[k])) continue;
          const unsigned s = rec_cell(v[k]) >> ss;
@@ -180,7 +179,7 @@ __global__ __launch_bounds__(256) void k_bin_part1(BinArgs a) {
       }
    }
    for (; i < a.rows; i++) {
-      const RT v = rec[i * a.walkers + col];
+      const RT v = rec[(blockIdx.x * a.rows + i) * 256 + threadIdx.x];
       if (rec_none(v)) continue;
       const unsigned s = rec_cell(v) >> ss;
       out[base[s] + atomicAdd(&cnt[s], 1u)] = v;
@@ -439,7 +438,7 @@ extern "C" size_t fl3k_bin_lds_bytes(int shift) {
 }
 
 extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_t s) {
-   hipLaunchKernelGGL(k_bin_sum, dim3((a->nbins + 63) / 64), dim3(1024), 0, s, *a);
+   if (!a->totals_in) hipLaunchKernelGGL(k_bin_sum, dim3((a->nbins + 63) / 64), dim3(1024), 0, s, *a);
    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, *a);
    hipLaunchKernelGGL(k_super_off, dim3(a->nsuper), dim3(256), 0, s, *a);
    const unsigned g2 = (unsigned)(((long long)a->walkers * a->rows + FL3_PART2_RECORDS - 1) / FL3_PART2_RECORDS);

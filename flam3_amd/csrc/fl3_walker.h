@@ -11,6 +11,8 @@
 #define FL3_REC_NARROW 2   // 4-byte records: cell | palette index << 24
 #define FL3_REC_JOB 3      // 8-byte records: cell, palette index, xform, job
 
+// Records: row i of walker (workgroup b, thread t) at (b * rows_out + i) *
+// 256 + t, so a workgroup's records are one contiguous block.
 // Production walkers.  Walker w plots nsamples/W (+1 for the first
 // nsamples%W walkers) points after `fuse` unplotted steps from a random
 // start, exactly the per-sub-batch procedure of rect.c:393-400 with one long
@@ -103,14 +105,14 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
       if (i >= 0) {
          if (RECORDS == FL3_REC_WIDE) {
             const unsigned long long v = make_record<float>(G, q, fn);
-            a.rec[i * W + wid] = v;
+            a.rec[(blk * a.rows_out + i) * 256 + threadIdx.x] = v;
             if ((unsigned)v != FL3_REC_NONE) atomicAdd(&bh[(unsigned)v >> a.bin_shift], 1u);
          } else if (RECORDS == FL3_REC_JOB) {
             // independent renders: the render's first cell moves the cell
             // into the launch's cell space, the job is the descriptor's
             const unsigned long long v = make_record<float>(G, q, fn);
             const unsigned cell = (unsigned)v == FL3_REC_NONE ? FL3_REC_NONE : (unsigned)(v + cell0);
-            a.rec[i * W + wid] = cell == FL3_REC_NONE
+            a.rec[(blk * a.rows_out + i) * 256 + threadIdx.x] = cell == FL3_REC_NONE
                                      ? (unsigned long long)FL3_REC_NONE
                                      : ((v & 0xffff00000000ull) | cell |
                                         ((unsigned long long)(dsc ? djob : jb) << 48));
@@ -118,8 +120,8 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
          } else if (RECORDS == FL3_REC_NARROW) {
             const unsigned long long v = make_record<float>(G, q, 0);
             const unsigned cell = (unsigned)v;
-            ((unsigned *)a.rec)[i * W + wid] = cell == FL3_REC_NONE ? FL3_REC_NONE
-                                                                     : (cell | ((unsigned)(v >> 32) & 0xffu) << 24);
+            ((unsigned *)a.rec)[(blk * a.rows_out + i) * 256 + threadIdx.x] =
+                cell == FL3_REC_NONE ? FL3_REC_NONE : (cell | ((unsigned)(v >> 32) & 0xffu) << 24);
             if (cell != FL3_REC_NONE) atomicAdd(&bh[cell >> a.bin_shift], 1u);
          } else
             deposit<float, BITS, EXACT>(G, q, buckets, satp);
@@ -127,12 +129,18 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
       i++;
    }
    if (RECORDS == FL3_REC_WIDE || RECORDS == FL3_REC_JOB)
-      for (long long i = n; i < a.rows_out; i++) a.rec[i * W + wid] = FL3_REC_NONE;
+      for (long long i = n; i < a.rows_out; i++) a.rec[(blk * a.rows_out + i) * 256 + threadIdx.x] = FL3_REC_NONE;
    if (RECORDS == FL3_REC_NARROW)
-      for (long long i = n; i < a.rows_out; i++) ((unsigned *)a.rec)[i * W + wid] = FL3_REC_NONE;
+      for (long long i = n; i < a.rows_out; i++)
+         ((unsigned *)a.rec)[(blk * a.rows_out + i) * 256 + threadIdx.x] = FL3_REC_NONE;
    if (RECORDS) {
+      // the workgroup's tile counts, and their sum over workgroups
       __syncthreads();
-      for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) a.blk_hist[(size_t)blk * a.nbins + b] = bh[b];
+      for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) {
+         const unsigned c = bh[b];
+         a.blk_hist[(size_t)blk * a.nbins + b] = c;
+         if (c) atomicAdd(&a.bin_total[b], c);
+      }
    }
    if (ws) {
       for (int k = 0; k < 4; k++) ws[k * W + wid] = __float_as_uint(p[k]);
