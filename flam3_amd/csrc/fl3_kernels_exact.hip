@@ -232,10 +232,15 @@ __global__ __launch_bounds__(256) void k_de_prep(DEArgs a) {
 #define DG_CHUNK 256
 template <typename AT, bool EXACT>
 __global__ __launch_bounds__(DG_CHUNK) void k_de_gather(DEArgs a) {
+   // per source: packed position, kernel index and values -- doubles
+   // (4 channels and the log term) in the exact mode, the accumulator type
+   // (4 pre-scaled channels) otherwise
+   typedef typename std::conditional<EXACT, double, AT>::type VT;
+   constexpr int NV = EXACT ? 5 : 4;
    __shared__ int radius[1024];
    __shared__ unsigned s_pos[DG_CHUNK];
    __shared__ int s_fi[DG_CHUNK];
-   __shared__ double s_val[DG_CHUNK][5];
+   __shared__ VT s_val[DG_CHUNK][NV];
    __shared__ unsigned wave_cnt[DG_CHUNK / 64];
    const int W = a.width, H = a.height, R = a.half;
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -282,11 +287,11 @@ __global__ __launch_bounds__(DG_CHUNK) void k_de_gather(DEArgs a) {
          const long cell = sx + (long)sy * W;
          s_pos[idx] = (unsigned)sx | ((unsigned)sy << 16);
          s_fi[idx] = fi;
-         if (EXACT) {
+         if constexpr (EXACT) {
             for (int c = 0; c < 4; c++) s_val[idx][c] = bucket_value(a.buckets, cell * 5 + c, (AT)0);
             s_val[idx][4] = a.srcl[cell];
          } else {
-            for (int c = 0; c < 4; c++) s_val[idx][c] = (double)srcv[cell * 4 + c];
+            for (int c = 0; c < 4; c++) s_val[idx][c] = srcv[cell * 4 + c];
          }
       }
       __syncthreads();
@@ -300,7 +305,7 @@ __global__ __launch_bounds__(DG_CHUNK) void k_de_gather(DEArgs a) {
          const int hi = max(dx, dy), lo = min(dx, dy);
          if (!live || hi > r) continue;
          const long ci = (long)cfi * a.kernel_size + hi * (hi + 1) / 2 + lo;
-         if (EXACT) {
+         if constexpr (EXACT) {
             const double coef = a.coefs[ci];
             if (coef == 0.0) continue;
             const double c3 = s_val[k][3];
@@ -309,7 +314,7 @@ __global__ __launch_bounds__(DG_CHUNK) void k_de_gather(DEArgs a) {
          } else {
             const AT coef = ((const AT *)a.coefs_t)[ci];
             if (coef == (AT)0) continue;
-            for (int c = 0; c < 4; c++) acc[c] += coef * (AT)s_val[k][c];
+            for (int c = 0; c < 4; c++) acc[c] += coef * s_val[k][c];
          }
       }
       __syncthreads();
