@@ -48,6 +48,22 @@ DE_BYTES_PER_CELL = 20.0 + 32.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_iter_fast_1080p_q500_hbm_bytes.txt")
 
 
+VALU_FILE = os.path.join(ROOT, "profiles", "k_iter_fast_1080p_q500_valu.txt")
+CLOCK_HZ = 2.4e9            # MI355X peak engine clock
+SIMDS = 1024                # 256 CUs x 4 SIMDs
+
+
+def profile_value(path, key):
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith(key + ":"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def measured_traffic():
     try:
         with open(TRAFFIC_FILE) as fh:
@@ -57,6 +73,16 @@ def measured_traffic():
     except OSError:
         pass
     return None
+
+
+def valu_issue_frac(samples_per_launch, kernel_ms):
+    """Fraction of the VALU issue slots the walker kernel used: its VALU
+    wavefront instructions per sample (rocprofv3 SQ_INSTS_VALU, committed
+    profile) x 4 cycles each / (SIMDs x clock x the live launch time)."""
+    ips = profile_value(VALU_FILE, "valu_instructions_per_sample")
+    if ips is None or kernel_ms <= 0:
+        return None
+    return ips * samples_per_launch / 64.0 * 4.0 / (SIMDS * CLOCK_HZ * kernel_ms * 1e-3)
 
 
 def log(*a):
@@ -362,8 +388,12 @@ def main():
                      "kernel": "k_iter_fast", "kernel_ms": it_ms, "samples_per_launch": per_launch,
                      "ops_per_sample": ops,
                      "algorithmic_bytes_per_launch": 4.0 * per_launch,
+                     "valu_issue_frac": valu_issue_frac(per_launch, it_ms) if args.quality == 500.0 else None,
                      "note": "FP32 vector-ALU bound (no matrix work on this path); peak is MI355X FP32, "
-                             "157.3 TFLOP/s for vector and dense matrix alike"},
+                             "157.3 TFLOP/s for vector and dense matrix alike. achieved counts only the formula "
+                             "ops of each sample's chosen xform; the kernel issues ~517 VALU instructions per "
+                             "sample (every xform's variations present in a wavefront, RNG, selection, trig "
+                             "reduction, record), valu_issue_frac is the share of VALU issue slots it uses"},
         "stages": {
             "iterate_ms_per_launch": it_ms,
             "bin_ms_per_chunk": bin_ms,
