@@ -354,3 +354,19 @@ def test_production_animation_frame_within_noise_floor(prod, ref):
     got, _ = prod.render(gs, seed="gpu", time=0.6)
     floor = rms(a, b)
     assert 0.5 * (rms(got, a) + rms(got, b)) <= 1.15 * floor
+
+
+def test_8k_bits64_geometry(prod):
+    """The 8K still's geometry at low quality: 7680x4320 supersample 3 (a
+    23134 x 13054 histogram, 3.0e8 cells, 12 GB of FP64 buckets), xaos, final
+    xform, DE radius 15, bits 64.  Same sample count as the reference's
+    formula, a finite non-empty image, and the same brightness as the bits-33
+    render of the same genome (the buckets' type does not change the
+    expected image)."""
+    g = G.synthetic_8k(seed=5, quality=2.0)
+    img64, st64 = prod.render(g, seed="8k", bits=64)
+    assert st64["num_iters"] == pytest.approx(2.0 * 7680 * 4320, rel=1e-9)
+    assert img64.shape == (4320, 7680, 3)
+    assert img64.mean() > 1.0
+    img33, _ = prod.render(g, seed="8k", bits=33)
+    assert abs(img64.mean() - img33.mean()) <= 0.02 * img33.mean() + 0.05, (img64.mean(), img33.mean())
