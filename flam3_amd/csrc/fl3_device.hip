@@ -534,7 +534,6 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
       b.final = a.rec;   // the walker buffer takes the tile-grouped records
       b.blk_hist = a.blk_hist;
       b.bin_total = a.bin_total;
-      b.totals_in = 1;
       b.rows = a.rows_out;
       pk = prof_begin(ST_BIN, bs);
       lrc = fl3k_launch_bin(&b, g_ctx.ncu, format, bs);
@@ -870,8 +869,7 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
          b.final = ar.rec;
          b.blk_hist = ar.blk_hist;
          b.bin_total = ar.bin_total;
-         b.totals_in = 1;
-         b.rows = ar.rows_out;
+            b.rows = ar.rows_out;
          const int pk = prof_begin(ST_BIN, bs);
          const int lrc = fl3k_launch_bin(&b, g_ctx.ncu, 2, bs);
          prof_end(pk, bs);

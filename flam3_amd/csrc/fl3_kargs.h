@@ -78,14 +78,14 @@ struct BinArgs {
    int nblk;                       // walker workgroups (256 walkers each)
    long long walkers;              // row stride of rec
    long long rows;                 // record rows in this chunk
-   unsigned *bin_total, *bin_start, *item_start, *item_counter;
+   unsigned *bin_total;            // [tile] record counts (summed by the walkers)
+   unsigned *bin_start, *item_start, *item_counter;
    // two-pass sort: walker order -> super-tiles of FL3_SUPER tiles (sorted)
    // -> tiles (final, the record buffer reused)
    unsigned *soff;                 // [walker workgroup][super-tile] first slot in sorted
    unsigned *cursor;               // [tile] next free slot of the tile in final
    void *final;
    int nsuper;
-   int totals_in;                  // bin_total already summed (by the walkers)
    int shift;                      // cells per tile = 1 << shift
    int nbins;                      // tiles
    int chunk;                      // records per accumulation work item

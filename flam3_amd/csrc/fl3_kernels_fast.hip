@@ -54,34 +54,6 @@ extern "C" int fl3k_launch_apply_fp32(const unsigned char *job, int xi, const do
 #define FL3_SCATTER_UNROLL 16
 #endif
 
-// Tile totals over the walker workgroups.  A workgroup covers 64 tiles; its
-// 16 wavefronts split the walker workgroups into 16 ranges (rows of 64
-// consecutive tiles are read coalesced) and add their range sums in LDS.
-__global__ __launch_bounds__(1024) void k_bin_sum(BinArgs a) {
-   __shared__ unsigned part[16][64];
-   const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
-   const int b = blockIdx.x * 64 + lane;
-   const int per = (a.nblk + 15) / 16;
-   const int k0 = grp * per, k1 = min(a.nblk, k0 + per);
-   unsigned s = 0;
-   if (b < a.nbins) {
-      int k = k0;
-      for (; k + 8 <= k1; k += 8) {
-         unsigned v[8];
-         for (int u = 0; u < 8; u++) v[u] = a.blk_hist[(size_t)(k + u) * a.nbins + b];
-         for (int u = 0; u < 8; u++) s += v[u];
-      }
-      for (; k < k1; k++) s += a.blk_hist[(size_t)k * a.nbins + b];
-   }
-   part[grp][lane] = s;
-   __syncthreads();
-   if (grp == 0 && b < a.nbins) {
-      unsigned total = 0;
-      for (int g = 0; g < 16; g++) total += part[g][lane];
-      a.bin_total[b] = total;
-   }
-}
-
 // One workgroup: exclusive scans of the tile counts and of the work items
 // (ceil(count / chunk) per tile).
 __global__ __launch_bounds__(1024) void k_bin_scan(BinArgs a) {
@@ -438,7 +410,6 @@ extern "C" size_t fl3k_bin_lds_bytes(int shift) {
 }
 
 extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_t s) {
-   if (!a->totals_in) hipLaunchKernelGGL(k_bin_sum, dim3((a->nbins + 63) / 64), dim3(1024), 0, s, *a);
    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, *a);
    hipLaunchKernelGGL(k_super_off, dim3(a->nsuper), dim3(256), 0, s, *a);
    const unsigned g2 = (unsigned)(((long long)a->walkers * a->rows + FL3_PART2_RECORDS - 1) / FL3_PART2_RECORDS);
