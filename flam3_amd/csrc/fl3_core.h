@@ -546,10 +546,14 @@ __device__ __forceinline__ unsigned long long make_record(const GenL<R> &G, cons
       y = x0 * G.rot10 + y0 * G.rot11 + G.rc1;
    }
    if (!(x >= G.b0 && y >= G.b1 && x <= G.b2 && y <= G.b3) || q[3] == R(0)) return FL3_REC_NONE;
-   const long ix = (long)(int)(G.ws0 * x - G.wb0s0);
-   const long iy = (long)(int)(G.hs1 * y - G.hb1s1);
-   const long cell = ix + (long)G.width * iy;
-   if (cell < 0 || cell >= (long)G.width * G.height) return FL3_REC_NONE;
+   // the reference's linear index ix + width * iy (rect.c:440-441) in 32-bit
+   // unsigned arithmetic: histograms here stay below 2^31 cells, so a
+   // negative index wraps above the cell count and is dropped like one past
+   // the end
+   const int ix = (int)(G.ws0 * x - G.wb0s0);
+   const int iy = (int)(G.hs1 * y - G.hb1s1);
+   const unsigned cell = (unsigned)ix + (unsigned)G.width * (unsigned)iy;
+   if (cell >= (unsigned)G.width * (unsigned)G.height) return FL3_REC_NONE;
    int ci;
    R frac;
    point_color(G, q[2], ci, frac);
