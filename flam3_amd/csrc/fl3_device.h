@@ -77,6 +77,10 @@ typedef struct {
 int fl3d_density(const fl3d_de_params *p, const fl3_de_filters *de);
 /* Log scaling without DE (rect.c:947-974). */
 int fl3d_logscale(double k1, double k2);
+/* Density estimation (or log scaling where de[i] is NULL or has no filters)
+ * of renders 0..n-1 of fl3d_begin_many, batched into one launch pair where
+ * possible. */
+int fl3d_density_many(const fl3d_de_params *p, const fl3_de_filters *const *de, int n);
 
 typedef struct {
    int image_width, image_height;  /* output pixels (height halved for fields) */

@@ -22,6 +22,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <algorithm>
 #include <atomic>
 #include <mutex>
 #include <type_traits>
@@ -91,6 +92,7 @@ struct Ctx {
    DevBuf buckets, accum, job, table, image, isaac, points, misc, findex;
    DevBuf rec, rec2, sorted, wstate, bins, blkhist, blkhist2, jobtab, soff, bintot, bintot2;   // tile-binned deposit
    DevBuf srcv, srcl;                  // density estimation per-source values
+   DevBuf detab, deargs;               // batched density estimation: tables, per-render arguments
    unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
    void *staging = nullptr;                   // pinned host buffer for the image download
    size_t staging_cap = 0;
@@ -228,6 +230,7 @@ extern "C" int fl3h_copy_probe(long long bytes) {
 }
 
 static size_t bucket_bytes(int bits) { return bits == 64 ? 40 : 20; }
+#define DG_REGION_HOST 16   // cells per side of a density-estimation workgroup (DG_REGION)
 static size_t accum_bytes(int bits);
 // the selected render's histogram and accumulator
 static void *cur_buckets() { return (unsigned char *)g_ctx.buckets.p + g_ctx.cell0 * bucket_bytes(g_ctx.bits); }
@@ -465,8 +468,9 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    if (g_ctx.rec.ensure(rec_bytes) || g_ctx.sorted.ensure(rec_bytes)) return 1;
    if (nchunks > 1 && g_ctx.rec2.ensure(rec_bytes)) return 1;
    if (g_ctx.wstate.ensure(sizeof(uint32_t) * 10 * (size_t)W)) return 1;
-   if (g_ctx.blkhist.ensure(sizeof(unsigned) * (size_t)nblk * nbins)) return 1;
-   if (nchunks > 1 && g_ctx.blkhist2.ensure(sizeof(unsigned) * (size_t)nblk * nbins)) return 1;
+   const size_t nsup = (size_t)((nbins + (1 << FL3_SUPER_SHIFT) - 1) >> FL3_SUPER_SHIFT);
+   if (g_ctx.blkhist.ensure(sizeof(unsigned) * (size_t)nblk * nsup)) return 1;
+   if (nchunks > 1 && g_ctx.blkhist2.ensure(sizeof(unsigned) * (size_t)nblk * nsup)) return 1;
    if (g_ctx.bintot.ensure(sizeof(unsigned) * (size_t)nbins) || g_ctx.bintot2.ensure(sizeof(unsigned) * (size_t)nbins))
       return 1;
    const int nsuper = (nbins + (1 << FL3_SUPER_SHIFT) - 1) >> FL3_SUPER_SHIFT;
@@ -800,8 +804,9 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
       if (g_ctx.rec.ensure(rec_bytes) || g_ctx.sorted.ensure(rec_bytes)) return 1;
       if (nchunks > 1 && g_ctx.rec2.ensure(rec_bytes)) return 1;
       if (g_ctx.wstate.ensure(sizeof(uint32_t) * 10 * (size_t)Wrec)) return 1;
-      if (g_ctx.blkhist.ensure(sizeof(unsigned) * (size_t)nblk * nbins)) return 1;
-      if (nchunks > 1 && g_ctx.blkhist2.ensure(sizeof(unsigned) * (size_t)nblk * nbins)) return 1;
+      const size_t nsup = (size_t)((nbins + (1 << FL3_SUPER_SHIFT) - 1) >> FL3_SUPER_SHIFT);
+      if (g_ctx.blkhist.ensure(sizeof(unsigned) * (size_t)nblk * nsup)) return 1;
+      if (nchunks > 1 && g_ctx.blkhist2.ensure(sizeof(unsigned) * (size_t)nblk * nsup)) return 1;
       if (g_ctx.bintot.ensure(sizeof(unsigned) * (size_t)nbins) ||
           g_ctx.bintot2.ensure(sizeof(unsigned) * (size_t)nbins))
          return 1;
@@ -996,6 +1001,136 @@ extern "C" int fl3d_density(const fl3d_de_params *p, const fl3_de_filters *de) {
       set_err("density estimation launch failed");
       return 1;
    }
+   return 0;
+}
+
+static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *de, const std::vector<int> &dej,
+                         size_t tab, long long tcells);
+
+// Density estimation of renders 0..n-1 of the begun histograms in one
+// launch pair (FP32 accumulators; renders without DE filters, or with bits
+// 64 / exact sums, take the per-render calls).  de[i] NULL or
+// max_filter_index 0: log scaling with p[i].k1, p[i].k2.
+extern "C" int fl3d_density_many(const fl3d_de_params *p, const fl3_de_filters *const *de, int n) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   if (n <= 0) return 0;
+   if (n > (int)g_ctx.rcell0.size()) {
+      set_err("fl3d_density_many: more renders than begun");
+      return 1;
+   }
+   const bool batched = g_ctx.bits != 64;
+   std::vector<int> dej;
+   size_t tab = 0;
+   long long tcells = 0;
+   for (int i = 0; i < n; i++) {
+      tcells += (long long)g_ctx.rwidth[i] * g_ctx.rheight[i];
+      const bool has = de[i] && de[i]->max_filter_index > 0;
+      if (has && batched && !p[i].exact && de[i]->max_filter_index < 1024 && g_ctx.rwidth[i] < 65536 &&
+          g_ctx.rheight[i] < 65536) {
+         dej.push_back(i);
+         const size_t nc = (size_t)de[i]->nfilters * de[i]->kernel_size, nw = de[i]->nfilters;
+         tab += (nc + nw) * sizeof(double) + ((nc * sizeof(float) + 15) & ~(size_t)15);
+      }
+   }
+   const int nd = (int)dej.size();
+   if (nd > 0 && de_many_batch(p, de, dej, tab, tcells)) return 1;
+   // then the renders outside the batch, one by one
+   for (int i = 0; i < n; i++) {
+      const bool has = de[i] && de[i]->max_filter_index > 0;
+      if (std::find(dej.begin(), dej.end(), i) != dej.end()) continue;
+      if (fl3d_select(i)) return 1;
+      if (has) {
+         if (fl3d_density(&p[i], de[i])) return 1;
+      } else if (fl3d_logscale(p[i].k1, p[i].k2))
+         return 1;
+   }
+   return 0;
+}
+
+// The batched part of fl3d_density_many: renders dej of the begun
+// histograms, tab bytes of coefficient tables, tcells cells in all.
+static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *de, const std::vector<int> &dej,
+                         size_t tab, long long tcells) {
+   const int nd = (int)dej.size();
+   // coefficient tables (double, then the float copy) of every render
+   if (g_ctx.detab.ensure(tab)) return 1;
+   if (g_ctx.findex.ensure(sizeof(uint16_t) * (size_t)tcells) || g_ctx.srcv.ensure(16 * (size_t)tcells) ||
+       g_ctx.srcl.ensure(sizeof(double) * (size_t)tcells))
+      return 1;
+   std::vector<DEArgs> as((size_t)nd);
+   std::vector<unsigned> first(2 * (size_t)(nd + 1));
+   unsigned *pf = first.data(), *gf = first.data() + nd + 1;
+   unsigned long long pb = 0, gb = 0;
+   std::vector<unsigned char> hostt(tab);
+   size_t off = 0;
+   for (int k = 0; k < nd; k++) {
+      const int i = dej[k];
+      const fl3_de_filters *d = de[i];
+      const size_t nc = (size_t)d->nfilters * d->kernel_size, nw = d->nfilters;
+      unsigned char *tb = (unsigned char *)g_ctx.detab.p + off;
+      memcpy(hostt.data() + off, d->filter_coefs, nc * sizeof(double));
+      memcpy(hostt.data() + off + nc * sizeof(double), d->filter_widths, nw * sizeof(double));
+      DEArgs &a = as[k];
+      memset(&a, 0, sizeof(a));
+      const long long c0 = g_ctx.rcell0[i];
+      a.buckets = (unsigned char *)g_ctx.buckets.p + c0 * bucket_bytes(g_ctx.bits);
+      a.accum = (unsigned char *)g_ctx.accum.p + c0 * accum_bytes(g_ctx.bits);
+      a.coefs = (const double *)tb;
+      a.widths = (const double *)(tb + nc * sizeof(double));
+      a.coefs_t = tb + (nc + nw) * sizeof(double);
+      a.findex = (uint16_t *)g_ctx.findex.p + c0;
+      a.srcv = (unsigned char *)g_ctx.srcv.p + c0 * 16;
+      a.srcl = (double *)g_ctx.srcl.p + c0;
+      a.exact = 0;
+      a.width = g_ctx.rwidth[i];
+      a.height = g_ctx.rheight[i];
+      a.oversample = p[i].oversample;
+      a.max_filtered_counts = d->max_filtered_counts;
+      a.max_filter_index = d->max_filter_index;
+      a.kernel_size = d->kernel_size;
+      a.k1 = p[i].k1;
+      a.k2 = p[i].k2;
+      a.curve = p[i].curve;
+      int half = 0;
+      while ((half + 1) * (half + 2) / 2 < d->kernel_size) half++;
+      a.half = half;
+      pf[k] = (unsigned)pb;
+      gf[k] = (unsigned)gb;
+      pb += (unsigned long long)((a.width + 255) / 256) * a.height;
+      gb += (unsigned long long)((a.width + DG_REGION_HOST - 1) / DG_REGION_HOST) *
+            ((a.height + DG_REGION_HOST - 1) / DG_REGION_HOST);
+      off += (nc + nw) * sizeof(double) + ((nc * sizeof(float) + 15) & ~(size_t)15);
+   }
+   if (pb > 0x7fffffffULL || gb > 0x7fffffffULL) {
+      set_err("fl3d_density_many: too many workgroups");
+      return 1;
+   }
+   pf[nd] = (unsigned)pb;
+   gf[nd] = (unsigned)gb;
+   const size_t ab = sizeof(DEArgs) * nd, fb = sizeof(unsigned) * first.size();
+   if (g_ctx.deargs.ensure(ab + fb + 16)) return 1;
+   HIPCHK(hipMemcpyAsync(g_ctx.detab.p, hostt.data(), tab, hipMemcpyHostToDevice, g_ctx.stream));
+   HIPCHK(hipMemcpyAsync(g_ctx.deargs.p, as.data(), ab, hipMemcpyHostToDevice, g_ctx.stream));
+   HIPCHK(hipMemcpyAsync((unsigned char *)g_ctx.deargs.p + ab, first.data(), fb, hipMemcpyHostToDevice, g_ctx.stream));
+   for (int k = 0; k < nd; k++) {
+      const fl3_de_filters *d = de[dej[k]];
+      const long nc = (long)d->nfilters * d->kernel_size;
+      if (fl3k_launch_f64_to_f32(as[k].coefs, (float *)as[k].coefs_t, nc, g_ctx.stream)) {
+         set_err("k_f64_to_f32 launch failed");
+         return 1;
+      }
+   }
+   const int pk = prof_begin(ST_DE);
+   const unsigned *dfirst = (const unsigned *)((unsigned char *)g_ctx.deargs.p + ab);
+   const int lrc = fl3k_launch_de_many((const DEArgs *)g_ctx.deargs.p, dfirst, (unsigned)pb, dfirst + nd + 1,
+                                       (unsigned)gb, nd, g_ctx.stream);
+   prof_end(pk);
+   if (lrc) {
+      set_err("density estimation launch failed");
+      return 1;
+   }
+   // the host vectors go out of scope
+   HIPCHK(hipStreamSynchronize(g_ctx.stream));
    return 0;
 }
 

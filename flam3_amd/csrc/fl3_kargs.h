@@ -47,7 +47,7 @@ struct IterArgs {
                               // 3 8-byte records carrying the job instead of the palette fraction
    unsigned long long *rec;   // 8-byte records, or 4-byte ones in the same buffer
    long long rows_out;
-   unsigned *blk_hist;        // [workgroup][tile] record counts
+   unsigned *blk_hist;        // [workgroup][super-tile] record counts
    unsigned *bin_total;       // [tile] record counts over all workgroups (added atomically)
    int bin_shift, nbins;
    unsigned hist_off;         // LDS byte offset of the workgroup's tile counts
@@ -74,7 +74,7 @@ struct BinArgs {
    void *buckets;
    const void *rec;                // records in walker order, one block of rows x 256 per workgroup
    void *sorted;                   // records grouped by tile
-   unsigned *blk_hist;             // [walker workgroup][tile] counts
+   unsigned *blk_hist;             // [walker workgroup][super-tile] counts
    int nblk;                       // walker workgroups (256 walkers each)
    long long walkers;              // row stride of rec
    long long rows;                 // record rows in this chunk

@@ -168,14 +168,14 @@ __device__ __forceinline__ double bucket_value(const void *b, long i, float) { r
 __device__ __forceinline__ double bucket_value(const void *b, long i, double) { return ((const double *)b)[i]; }
 
 template <typename BT, typename AT>
-__global__ __launch_bounds__(256) void k_de_prep(DEArgs a) {
+__device__ __forceinline__ void de_prep_body(const DEArgs &a, int bx, int by) {
    const int os = a.oversample;
    const int ss = os / 2;
    const bool scf = !(os & 1);
    const double scfact = pow(os / (os + 1.0), 2.0);
    const int W = a.width, H = a.height;
-   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-   const int j = blockIdx.y;
+   const int i = bx * blockDim.x + threadIdx.x;
+   const int j = by;
    if (i >= W) return;
    const long cell = i + (long)j * W;
    uint16_t out = DE_SKIP;
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(256) void k_de_prep(DEArgs a) {
 #define DG_REGION 16
 #define DG_CHUNK 256
 template <typename AT, bool EXACT>
-__global__ __launch_bounds__(DG_CHUNK) void k_de_gather(DEArgs a) {
+__device__ __forceinline__ void de_gather_body(const DEArgs &a, int bx, int by) {
    // per source: packed position, kernel index and values -- doubles
    // (4 channels and the log term) in the exact mode, the accumulator type
    // (4 pre-scaled channels) otherwise
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(DG_CHUNK) void k_de_gather(DEArgs a) {
    __shared__ unsigned wave_cnt[DG_CHUNK / 64];
    const int W = a.width, H = a.height, R = a.half;
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-   const int rx0 = blockIdx.x * DG_REGION, ry0 = blockIdx.y * DG_REGION;
+   const int rx0 = bx * DG_REGION, ry0 = by * DG_REGION;
    const int rx1 = min(rx0 + DG_REGION, W) - 1, ry1 = min(ry0 + DG_REGION, H) - 1;
    const int mx0 = rx0 + 8 * (wave & 1), my0 = ry0 + 8 * (wave >> 1);
    const int x = mx0 + (lane & 7), y = my0 + (lane >> 3);
@@ -321,6 +321,58 @@ __global__ __launch_bounds__(DG_CHUNK) void k_de_gather(DEArgs a) {
    }
    if (live)
       for (int c = 0; c < 4; c++) accp[c] = acc[c];
+}
+
+template <typename BT, typename AT>
+__global__ __launch_bounds__(256) void k_de_prep(DEArgs a) {
+   de_prep_body<BT, AT>(a, blockIdx.x, blockIdx.y);
+}
+
+template <typename AT, bool EXACT>
+__global__ __launch_bounds__(DG_CHUNK) void k_de_gather(DEArgs a) {
+   de_gather_body<AT, EXACT>(a, blockIdx.x, blockIdx.y);
+}
+
+// Several renders in one launch: workgroup b belongs to the render j with
+// first[j] <= b < first[j + 1] and is that render's workgroup b - first[j]
+// of a grid `cols` wide.
+__device__ __forceinline__ int de_job(const unsigned *first, int n, unsigned b) {
+   int lo = 0, hi = n - 1;
+   while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (first[mid] <= b) lo = mid;
+      else hi = mid - 1;
+   }
+   return lo;
+}
+
+template <typename BT, typename AT>
+__global__ __launch_bounds__(256) void k_de_prep_many(const DEArgs *as, const unsigned *first, int n) {
+   const int j = de_job(first, n, blockIdx.x);
+   const DEArgs a = as[j];
+   const unsigned local = blockIdx.x - first[j], cols = (unsigned)(a.width + 255) / 256;
+   de_prep_body<BT, AT>(a, (int)(local % cols), (int)(local / cols));
+}
+
+template <typename AT, bool EXACT>
+__global__ __launch_bounds__(DG_CHUNK) void k_de_gather_many(const DEArgs *as, const unsigned *first, int n) {
+   const int j = de_job(first, n, blockIdx.x);
+   const DEArgs a = as[j];
+   const unsigned local = blockIdx.x - first[j], cols = (unsigned)(a.width + DG_REGION - 1) / DG_REGION;
+   de_gather_body<AT, EXACT>(a, (int)(local % cols), (int)(local / cols));
+}
+
+// Density estimation of n renders: per-render arguments in device memory,
+// workgroup prefixes prep_first[n + 1] (rows of 256 cells) and
+// gather_first[n + 1] (16x16 regions).  bits 32/33 with the FP32
+// accumulator (exact = 0) only.
+extern "C" int fl3k_launch_de_many(const DEArgs *as, const unsigned *prep_first, unsigned prep_blocks,
+                                   const unsigned *gather_first, unsigned gather_blocks, int n, hipStream_t s) {
+   if (n <= 0) return 0;
+   hipLaunchKernelGGL((k_de_prep_many<uint32_t, float>), dim3(prep_blocks), dim3(256), 0, s, as, prep_first, n);
+   hipLaunchKernelGGL((k_de_gather_many<float, false>), dim3(gather_blocks), dim3(DG_CHUNK), 0, s, as, gather_first,
+                      n);
+   return hipGetLastError() != hipSuccess;
 }
 
 // Log scaling of every cell without DE (rect.c:947-974).

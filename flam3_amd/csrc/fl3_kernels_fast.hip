@@ -100,16 +100,15 @@ __device__ __forceinline__ bool rec_none(RT v) {
 
 // First slot of each walker workgroup's records of each super-tile in the
 // super-tile-grouped array: workgroup s scans super-tile s's counts over the
-// walker workgroups (each row of FL3_SUPER tile counts read as one line).
+// walker workgroups.
 __global__ __launch_bounds__(256) void k_super_off(BinArgs a) {
    __shared__ unsigned part[256];
    const int s = blockIdx.x, t = threadIdx.x;
-   const int b0 = s << FL3_SUPER_SHIFT, b1 = min(a.nbins, b0 + (1 << FL3_SUPER_SHIFT));
+   const int b0 = s << FL3_SUPER_SHIFT;
    const int per = (a.nblk + 255) / 256;
    const int k0 = t * per, k1 = min(a.nblk, k0 + per);
    unsigned sum = 0;
-   for (int k = k0; k < k1; k++)
-      for (int b = b0; b < b1; b++) sum += a.blk_hist[(size_t)k * a.nbins + b];
+   for (int k = k0; k < k1; k++) sum += a.blk_hist[(size_t)k * a.nsuper + s];
    part[t] = sum;
    __syncthreads();
    for (int off = 1; off < 256; off <<= 1) {
@@ -121,7 +120,7 @@ __global__ __launch_bounds__(256) void k_super_off(BinArgs a) {
    unsigned run = a.bin_start[b0] + (t ? part[t - 1] : 0);
    for (int k = k0; k < k1; k++) {
       a.soff[(size_t)k * a.nsuper + s] = run;
-      for (int b = b0; b < b1; b++) run += a.blk_hist[(size_t)k * a.nbins + b];
+      run += a.blk_hist[(size_t)k * a.nsuper + s];
    }
 }
 

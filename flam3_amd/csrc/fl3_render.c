@@ -1008,14 +1008,29 @@ static int many_group(fl3_many *B, int i0, int i1) {
          nm += B->njob[r];
       }
       ok = fl3d_begin_many(widths, heights, nl, B->bits) == 0 &&
-           fl3d_iterate_multi(mj, nm, FUSE_WALKER, 0, nl, bad, sat) == 0 &&
-           fl3d_images_begin(img_total, filt_total) == 0;
+           fl3d_iterate_multi(mj, nm, FUSE_WALKER, 0, nl, bad, sat) == 0;
+      if (ok) {
+         /* density estimation (or log scaling) of the group in one launch pair */
+         fl3d_de_params *dps = (fl3d_de_params *)calloc((size_t)nl, sizeof(fl3d_de_params));
+         const fl3_de_filters **des = (const fl3_de_filters **)calloc((size_t)nl, sizeof(fl3_de_filters *));
+         ok = dps && des;
+         for (i = 0; ok && i < nl; i++) {
+            render_plan *Q = &B->P[B->live[i0 + i]];
+            const fl3_de_filters *d = &B->de[B->live[i0 + i]];
+            Q->badvals = bad[i];
+            plan_de_params(Q, &dps[i]);
+            des[i] = d->max_filter_index > 0 ? d : NULL;
+         }
+         if (ok) ok = fl3d_density_many(dps, des, nl) == 0;
+         free(dps);
+         free((void *)des);
+      }
+      if (ok) ok = fl3d_images_begin(img_total, filt_total) == 0;
       filt_total = 0;
       for (i = 0; ok && i < nl; i++) {
          render_plan *Q = &B->P[B->live[i0 + i]];
          fl3d_tm_params tp;
-         Q->badvals = bad[i];
-         ok = fl3d_select(i) == 0 && many_density(Q, &B->de[B->live[i0 + i]]) == 0;
+         ok = fl3d_select(i) == 0;
          plan_tm(Q, &tp);
          if (ok) ok = fl3d_tonemap_async(&tp, Q->filter, dst[i].img_off, filt_total) == 0;
          filt_total += (sizeof(double) * Q->filter_width * Q->filter_width + 255) & ~(size_t)255;

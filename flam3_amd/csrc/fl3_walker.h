@@ -17,7 +17,8 @@
 // nsamples%W walkers) points after `fuse` unplotted steps from a random
 // start, exactly the per-sub-batch procedure of rect.c:393-400 with one long
 // sub-batch per walker.  In record mode each workgroup also counts its
-// records per tile in LDS and stores the counts at blk_hist[block][tile].
+// records per tile in LDS and stores them per super-tile at
+// blk_hist[block][super-tile].
 template <int BITS, bool EXACT, int RECORDS, int TIER>
 __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
    unsigned char *lds = fl3_lds;
@@ -134,12 +135,19 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
       for (long long i = n; i < a.rows_out; i++)
          ((unsigned *)a.rec)[(blk * a.rows_out + i) * 256 + threadIdx.x] = FL3_REC_NONE;
    if (RECORDS) {
-      // the workgroup's tile counts, and their sum over workgroups
+      // tile counts summed over workgroups, and this workgroup's counts per
+      // super-tile (FL3_SUPER tiles) for the first sort pass
       __syncthreads();
       for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) {
          const unsigned c = bh[b];
-         a.blk_hist[(size_t)blk * a.nbins + b] = c;
          if (c) atomicAdd(&a.bin_total[b], c);
+      }
+      const int nsuper = (a.nbins + (1 << FL3_SUPER_SHIFT) - 1) >> FL3_SUPER_SHIFT;
+      for (int sp = threadIdx.x; sp < nsuper; sp += blockDim.x) {
+         const int b0 = sp << FL3_SUPER_SHIFT, b1 = min(a.nbins, b0 + (1 << FL3_SUPER_SHIFT));
+         unsigned c = 0;
+         for (int b = b0; b < b1; b++) c += bh[b];
+         a.blk_hist[(size_t)blk * nsuper + sp] = c;
       }
    }
    if (ws) {
