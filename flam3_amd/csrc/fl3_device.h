@@ -105,6 +105,9 @@ typedef struct {
 int fl3d_images_begin(size_t image_bytes, size_t filter_bytes);
 int fl3d_tonemap_async(const fl3d_tm_params *p, const double *filter, size_t img_off, size_t filt_off);
 int fl3d_images_fetch(const fl3d_image_dst *d, int n);
+/* Tone maps of renders 0..n-1 of fl3d_begin_many in one launch (after
+ * fl3d_images_begin; render i's image at img_off[i]). */
+int fl3d_tonemap_many(const fl3d_tm_params *p, const double *const *filters, const size_t *img_off, int n);
 
 /* Replay iteration: nwalkers FP64 walkers, walker w being reference thread w
  * with ISAAC state isaac[36w..36w+35] (count, 16 results, 16 memory words,

@@ -1206,6 +1206,86 @@ extern "C" int fl3d_tonemap_async(const fl3d_tm_params *p, const double *filter,
    return 0;
 }
 
+// Tone maps of renders 0..n-1 of the begun histograms into the image buffer
+// of fl3d_images_begin (render i at img_off[i]): earlyclip pre-passes per
+// render, then one launch for all.
+extern "C" int fl3d_tonemap_many(const fl3d_tm_params *p, const double *const *filters, const size_t *img_off,
+                                 int n) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   if (n <= 0) return 0;
+   if (n > (int)g_ctx.rcell0.size()) {
+      set_err("fl3d_tonemap_many: more renders than begun");
+      return 1;
+   }
+   size_t fb = 0;
+   for (int i = 0; i < n; i++) fb += sizeof(double) * p[i].filter_width * p[i].filter_width;
+   std::vector<TMArgs> as((size_t)n);
+   std::vector<unsigned> first((size_t)n + 1);
+   std::vector<double> hf(fb / sizeof(double) + 1);
+   if (g_ctx.detab.ensure(fb + 16) || g_ctx.deargs.ensure(sizeof(TMArgs) * n + sizeof(unsigned) * (n + 1) + 16))
+      return 1;
+   size_t fo = 0;
+   unsigned long long blocks = 0;
+   for (int i = 0; i < n; i++) {
+      const size_t row = (size_t)p[i].image_width * p[i].nchan * p[i].bytes_per_channel;
+      if (img_off[i] + row * p[i].image_height > g_ctx.image.cap) {
+         set_err("fl3d_tonemap_many: image outside the buffer of fl3d_images_begin");
+         return 1;
+      }
+      const size_t nf = (size_t)p[i].filter_width * p[i].filter_width;
+      memcpy(hf.data() + fo, filters[i], sizeof(double) * nf);
+      TMArgs &a = as[i];
+      memset(&a, 0, sizeof(a));
+      a.accum = (unsigned char *)g_ctx.accum.p + g_ctx.rcell0[i] * accum_bytes(g_ctx.bits);
+      a.filter = (const double *)g_ctx.detab.p + fo;
+      a.out = (unsigned char *)g_ctx.image.p + img_off[i];
+      a.W = g_ctx.rwidth[i];
+      a.iw = p[i].image_width;
+      a.ih = p[i].image_height;
+      a.os = p[i].oversample;
+      a.de_offset = p[i].de_offset;
+      a.fw = p[i].filter_width;
+      a.nchan = p[i].nchan;
+      a.transp = p[i].transp;
+      a.bpc = p[i].bytes_per_channel;
+      a.earlyclip = p[i].earlyclip;
+      a.g = p[i].g;
+      a.linrange = p[i].linrange;
+      a.vibrancy = p[i].vibrancy;
+      a.highpow = p[i].highpow;
+      for (int k = 0; k < 3; k++) a.bg[k] = p[i].background[k];
+      first[i] = (unsigned)blocks;
+      if (a.iw > 0 && a.ih > 0) blocks += (unsigned long long)((a.iw + 255) / 256) * a.ih;
+      fo += nf;
+   }
+   first[n] = (unsigned)blocks;
+   if (blocks > 0x7fffffffULL) {
+      set_err("fl3d_tonemap_many: too many workgroups");
+      return 1;
+   }
+   HIPCHK(hipMemcpyAsync(g_ctx.detab.p, hf.data(), fb, hipMemcpyHostToDevice, g_ctx.stream));
+   HIPCHK(hipMemcpyAsync(g_ctx.deargs.p, as.data(), sizeof(TMArgs) * n, hipMemcpyHostToDevice, g_ctx.stream));
+   unsigned *dfirst = (unsigned *)((unsigned char *)g_ctx.deargs.p + sizeof(TMArgs) * n);
+   HIPCHK(hipMemcpyAsync(dfirst, first.data(), sizeof(unsigned) * (n + 1), hipMemcpyHostToDevice, g_ctx.stream));
+   for (int i = 0; i < n; i++)
+      if (as[i].earlyclip &&
+          fl3k_launch_earlyclip(const_cast<void *>(as[i].accum), (long)g_ctx.rwidth[i] * g_ctx.rheight[i], &as[i],
+                                g_ctx.stream, g_ctx.bits)) {
+         set_err("k_earlyclip launch failed");
+         return 1;
+      }
+   const int pk = prof_begin(ST_TONE);
+   const int lrc = fl3k_launch_tonemap_many((const TMArgs *)g_ctx.deargs.p, dfirst, (unsigned)blocks, n,
+                                            g_ctx.stream, g_ctx.bits);
+   prof_end(pk);
+   if (lrc) {
+      set_err("k_tonemap launch failed");
+      return 1;
+   }
+   HIPCHK(hipStreamSynchronize(g_ctx.stream));   // the host vectors go out of scope
+   return 0;
+}
+
 extern "C" int fl3d_images_fetch(const fl3d_image_dst *d, int n) {
    std::lock_guard<std::recursive_mutex> lk(g_mu);
    size_t end = 0;

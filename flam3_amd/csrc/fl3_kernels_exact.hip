@@ -438,9 +438,9 @@ __global__ __launch_bounds__(256) void k_earlyclip(AT *acc, long ncells, TMArgs 
 }
 
 template <typename AT>
-__global__ __launch_bounds__(256) void k_tonemap(TMArgs a) {
-   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-   const int j = blockIdx.y;
+__device__ __forceinline__ void tonemap_body(const TMArgs &a, int bx, int by) {
+   const int i = bx * blockDim.x + threadIdx.x;
+   const int j = by;
    if (i >= a.iw || j >= a.ih) return;
    const AT *acc = (const AT *)a.accum;
    const int x = a.de_offset + i * a.os, y = a.de_offset + j * a.os;
@@ -553,6 +553,33 @@ extern "C" int fl3k_launch_logscale(const void *b, void *acc, long n, double k1,
    else
       hipLaunchKernelGGL((k_logscale<uint32_t, uint32_t>), grid, block, 0, s, (const uint32_t *)b, (uint32_t *)acc,
                          n, k1, k2);
+   return hipGetLastError() != hipSuccess;
+}
+
+template <typename AT>
+__global__ __launch_bounds__(256) void k_tonemap(TMArgs a) {
+   tonemap_body<AT>(a, blockIdx.x, blockIdx.y);
+}
+
+// Tone maps of several renders in one launch (workgroup prefixes as for
+// k_de_gather_many; rows of 256 pixels).
+template <typename AT>
+__global__ __launch_bounds__(256) void k_tonemap_many(const TMArgs *as, const unsigned *first, int n) {
+   const int j = de_job(first, n, blockIdx.x);
+   const TMArgs a = as[j];
+   const unsigned local = blockIdx.x - first[j], cols = (unsigned)(a.iw + 255) / 256;
+   tonemap_body<AT>(a, (int)(local % cols), (int)(local / cols));
+}
+
+extern "C" int fl3k_launch_tonemap_many(const TMArgs *as, const unsigned *first, unsigned blocks, int n,
+                                        hipStream_t s, int bits) {
+   if (n <= 0 || blocks == 0) return 0;
+   if (bits == 64)
+      hipLaunchKernelGGL((k_tonemap_many<double>), dim3(blocks), dim3(256), 0, s, as, first, n);
+   else if (bits == 33)
+      hipLaunchKernelGGL((k_tonemap_many<float>), dim3(blocks), dim3(256), 0, s, as, first, n);
+   else
+      hipLaunchKernelGGL((k_tonemap_many<uint32_t>), dim3(blocks), dim3(256), 0, s, as, first, n);
    return hipGetLastError() != hipSuccess;
 }
 

@@ -35,5 +35,7 @@ int fl3k_launch_f64_to_f32(const double *src, float *dst, long n, hipStream_t s)
 int fl3k_launch_logscale(const void *b, void *acc, long n, double k1, double k2, hipStream_t s, int bits);
 int fl3k_launch_earlyclip(void *acc, long n, const TMArgs *a, hipStream_t s, int bits);
 int fl3k_launch_tonemap(const TMArgs *a, hipStream_t s, int bits);
+int fl3k_launch_tonemap_many(const TMArgs *as, const unsigned *first, unsigned blocks, int n, hipStream_t s,
+                             int bits);
 }
 #endif

@@ -1026,14 +1026,22 @@ static int many_group(fl3_many *B, int i0, int i1) {
          free((void *)des);
       }
       if (ok) ok = fl3d_images_begin(img_total, filt_total) == 0;
-      filt_total = 0;
-      for (i = 0; ok && i < nl; i++) {
-         render_plan *Q = &B->P[B->live[i0 + i]];
-         fl3d_tm_params tp;
-         ok = fl3d_select(i) == 0;
-         plan_tm(Q, &tp);
-         if (ok) ok = fl3d_tonemap_async(&tp, Q->filter, dst[i].img_off, filt_total) == 0;
-         filt_total += (sizeof(double) * Q->filter_width * Q->filter_width + 255) & ~(size_t)255;
+      if (ok) {
+         /* the group's tone maps in one launch, the images in one copy */
+         fl3d_tm_params *tps = (fl3d_tm_params *)calloc((size_t)nl, sizeof(fl3d_tm_params));
+         const double **fls = (const double **)calloc((size_t)nl, sizeof(double *));
+         size_t *offs = (size_t *)calloc((size_t)nl, sizeof(size_t));
+         ok = tps && fls && offs;
+         for (i = 0; ok && i < nl; i++) {
+            render_plan *Q = &B->P[B->live[i0 + i]];
+            plan_tm(Q, &tps[i]);
+            fls[i] = Q->filter;
+            offs[i] = dst[i].img_off;
+         }
+         if (ok) ok = fl3d_tonemap_many(tps, fls, offs, nl) == 0;
+         free(offs);
+         free((void *)fls);
+         free(tps);
       }
       if (ok) ok = fl3d_images_fetch(dst, nl) == 0;
       if (!ok) fprintf(stderr, "flam3_amd: device error: %s\n", fl3d_error());
