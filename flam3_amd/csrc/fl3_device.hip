@@ -356,6 +356,19 @@ static bool narrow_records_ok(const fl3_job_hdr *h, long long ncells) {
    return true;
 }
 
+// Job records take the packed accumulation when every job has a stepped
+// palette and full visibility (a record then adds its job's palette entry
+// and 255 hits, whatever its xform).
+static bool job_records_packed(const fl3_job_hdr *const *h, int n) {
+   for (int j = 0; j < n; j++) {
+      if (h[j]->palette_mode != 0) return false;
+      const fl3_xform_pack *px = (const fl3_xform_pack *)((const unsigned char *)h[j] + h[j]->off_xf);
+      for (int e = 0; e < h[j]->nxf; e++)
+         if (px[e].vis != 1.0) return false;
+   }
+   return true;
+}
+
 // The jobs of one launch (one per temporal sample of a batch) as the
 // device sees them.
 struct JobSet {
@@ -498,7 +511,7 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    b.width = g_ctx.width;
    b.height = g_ctx.height;
    b.chunk = item_records(js.h, js.n);
-   b.round = narrow ? narrow_round(js.h, js.n) : 0;
+   b.round = narrow || (format == 2 && job_records_packed(js.h, js.n)) ? narrow_round(js.h, js.n) : 0;
    if (format == 2 && upload_job_tables(js.h, js.n, js.nxf_max, &b)) return 1;
    a.rec_mode = narrow ? 2 : (format == 2 ? 3 : 1);
    a.wstate = (uint32_t *)g_ctx.wstate.p;
@@ -833,6 +846,12 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
       b.width = (int)(tcells < 0x7fffffffLL ? tcells : 0x7fffffffLL);
       b.height = 1;
       b.chunk = item_records(hs.data(), njobs);
+      {
+         std::vector<const fl3_job_hdr *> rh;   // the jobs that write records
+         for (int j = 0; j < njobs; j++)
+            if (grp[j] >= MG_REC) rh.push_back(hs[j]);
+         b.round = job_records_packed(rh.data(), (int)rh.size()) ? narrow_round(hs.data(), njobs) : 0;
+      }
       if (upload_job_tables(hs.data(), njobs, nxf_max, &b)) return 1;
       IterArgs ar = a;
       ar.rec_mode = 3;   // job records
@@ -1070,6 +1089,9 @@ static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *d
       unsigned char *tb = (unsigned char *)g_ctx.detab.p + off;
       memcpy(hostt.data() + off, d->filter_coefs, nc * sizeof(double));
       memcpy(hostt.data() + off + nc * sizeof(double), d->filter_widths, nw * sizeof(double));
+      // the float copy, rounded to nearest as a device conversion would
+      float *ft = (float *)(hostt.data() + off + (nc + nw) * sizeof(double));
+      for (size_t e = 0; e < nc; e++) ft[e] = (float)d->filter_coefs[e];
       DEArgs &a = as[k];
       memset(&a, 0, sizeof(a));
       const long long c0 = g_ctx.rcell0[i];
@@ -1112,14 +1134,6 @@ static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *d
    HIPCHK(hipMemcpyAsync(g_ctx.detab.p, hostt.data(), tab, hipMemcpyHostToDevice, g_ctx.stream));
    HIPCHK(hipMemcpyAsync(g_ctx.deargs.p, as.data(), ab, hipMemcpyHostToDevice, g_ctx.stream));
    HIPCHK(hipMemcpyAsync((unsigned char *)g_ctx.deargs.p + ab, first.data(), fb, hipMemcpyHostToDevice, g_ctx.stream));
-   for (int k = 0; k < nd; k++) {
-      const fl3_de_filters *d = de[dej[k]];
-      const long nc = (long)d->nfilters * d->kernel_size;
-      if (fl3k_launch_f64_to_f32(as[k].coefs, (float *)as[k].coefs_t, nc, g_ctx.stream)) {
-         set_err("k_f64_to_f32 launch failed");
-         return 1;
-      }
-   }
    const int pk = prof_begin(ST_DE);
    const unsigned *dfirst = (const unsigned *)((unsigned char *)g_ctx.deargs.p + ab);
    const int lrc = fl3k_launch_de_many((const DEArgs *)g_ctx.deargs.p, dfirst, (unsigned)pb, dfirst + nd + 1,

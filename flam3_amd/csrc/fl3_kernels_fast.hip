@@ -305,16 +305,18 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum(BinArgs a) {
    }
 }
 
-// Accumulation of 4-byte records (visibility 1, stepped palette: a record
-// adds the palette entry and 255 hits).  The tile's cells live in LDS as two
+// Accumulation of records of visibility 1 with a stepped palette (a record
+// adds its palette entry and 255 hits): 4-byte records, or job records whose
+// palette is their job's.  The tile's cells live in LDS as two
 // packed 64-bit words, R:24 | G:24 | count:16 and B:32 | A:32, so a record
 // costs two LDS atomics; a round of at most `round` records keeps every
 // field below its width, after which each thread folds the CPT cells it
 // owns into registers and clears them.  The tile is then added to the
 // histogram from the registers (non-atomic saturating add if the tile has
 // one work item).
-template <int CPT>
+template <int CPT, bool JOB>
 __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_narrow(BinArgs a) {
+   typedef typename std::conditional<JOB, unsigned long long, unsigned>::type RT;
    const fl3_job_hdr *h = (const fl3_job_hdr *)a.job;
    const int tile_cells = CPT * FL3_ACC_THREADS;
    unsigned long long *p0 = (unsigned long long *)fl3_lds;   // tile_cells
@@ -322,13 +324,14 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_narrow(BinArgs a)
    uint32_t *dmapu = (uint32_t *)(p1 + tile_cells);           // 1024
    __shared__ int item_sh;
    const double *pd = (const double *)(a.job + h->off_dmap);
-   for (int e = threadIdx.x; e < 1024; e += blockDim.x) dmapu[e] = (uint32_t)pd[e];
+   if (!JOB)
+      for (int e = threadIdx.x; e < 1024; e += blockDim.x) dmapu[e] = (uint32_t)pd[e];
    for (int e = threadIdx.x; e < tile_cells; e += blockDim.x) {
       p0[e] = 0;
       p1[e] = 0;
    }
    const long long total_items = a.item_start[a.nbins];
-   const unsigned *sorted = (const unsigned *)a.final;
+   const RT *sorted = (const RT *)a.final;
    const long long step = (long long)FL3_ACC_UNROLL * FL3_ACC_THREADS;
    for (;;) {
       __syncthreads();
@@ -354,15 +357,18 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_narrow(BinArgs a)
       for (long long rs = r0; rs < r1; rs += a.round) {
          const long long re = min(rs + a.round, r1);
          for (long long rb = rs; rb < re; rb += step) {
-            unsigned v[FL3_ACC_UNROLL];
+            RT v[FL3_ACC_UNROLL];
             for (int u = 0; u < FL3_ACC_UNROLL; u++) {
                const long long r = rb + u * FL3_ACC_THREADS + threadIdx.x;
-               v[u] = r < re ? sorted[r] : FL3_REC_NONE;
+               v[u] = r < re ? sorted[r] : (RT)FL3_REC_NONE;
             }
             for (int u = 0; u < FL3_ACC_UNROLL; u++) {
-               if (v[u] == FL3_REC_NONE) continue;
-               const unsigned local = (v[u] & 0xffffffu) - cell0;
-               const uint32_t *du = dmapu + (v[u] >> 24) * 4;
+               if ((unsigned)v[u] == FL3_REC_NONE) continue;
+               // 4-byte record: cell | palette index << 24; job record: cell,
+               // palette index at bit 32, job at bit 48 (its own palette)
+               const unsigned local = (JOB ? (unsigned)v[u] : ((unsigned)v[u] & 0xffffffu)) - cell0;
+               const uint32_t *du = JOB ? a.dmapu_all + (size_t)(v[u] >> 48) * 1024 + ((v[u] >> 32) & 0xff) * 4
+                                        : dmapu + ((unsigned)v[u] >> 24) * 4;
                atomicAdd(&p0[local], (unsigned long long)du[0] | ((unsigned long long)du[1] << 24) | (1ull << 48));
                atomicAdd(&p1[local], (unsigned long long)du[2] | ((unsigned long long)du[3] << 32));
             }
@@ -417,17 +423,28 @@ extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_
    const unsigned gacc = (unsigned)ncu * (lds > 65536 ? 1 : 3);
    const int big = lds > 65536;
    const int cpt = (1 << a->shift) / FL3_ACC_THREADS;
-   if (format == 1 && a->round > 0 && (cpt == 2 || cpt == 4)) {
-      hipLaunchKernelGGL(k_bin_part1<unsigned>, dim3(a->nblk), dim3(256), 0, s, *a);
-      if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned>, dim3(g2), dim3(256), lds2, s, *a);
+   if ((format == 1 || format == 2) && a->round > 0 && (cpt == 2 || cpt == 4)) {
+      const bool job = format == 2;
+      if (job) {
+         hipLaunchKernelGGL(k_bin_part1<unsigned long long>, dim3(a->nblk), dim3(256), 0, s, *a);
+         if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned long long>, dim3(g2), dim3(256), lds2, s, *a);
+      } else {
+         hipLaunchKernelGGL(k_bin_part1<unsigned>, dim3(a->nblk), dim3(256), 0, s, *a);
+         if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned>, dim3(g2), dim3(256), lds2, s, *a);
+      }
       const size_t ln = fl3k_bin_narrow_lds_bytes(a->shift);
       const unsigned gn = (unsigned)ncu * (ln > 65536 ? 2 : 3);
-      const void *kf = cpt == 4 ? (const void *)k_bin_accum_narrow<4> : (const void *)k_bin_accum_narrow<2>;
+      const void *kf = job ? (cpt == 4 ? (const void *)k_bin_accum_narrow<4, true> : (const void *)k_bin_accum_narrow<2, true>)
+                           : (cpt == 4 ? (const void *)k_bin_accum_narrow<4, false> : (const void *)k_bin_accum_narrow<2, false>);
       if (ln > 65536) (void)hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ln);
-      if (cpt == 4)
-         hipLaunchKernelGGL(k_bin_accum_narrow<4>, dim3(gn), dim3(FL3_ACC_THREADS), ln, s, *a);
+      if (job && cpt == 4)
+         hipLaunchKernelGGL((k_bin_accum_narrow<4, true>), dim3(gn), dim3(FL3_ACC_THREADS), ln, s, *a);
+      else if (job)
+         hipLaunchKernelGGL((k_bin_accum_narrow<2, true>), dim3(gn), dim3(FL3_ACC_THREADS), ln, s, *a);
+      else if (cpt == 4)
+         hipLaunchKernelGGL((k_bin_accum_narrow<4, false>), dim3(gn), dim3(FL3_ACC_THREADS), ln, s, *a);
       else
-         hipLaunchKernelGGL(k_bin_accum_narrow<2>, dim3(gn), dim3(FL3_ACC_THREADS), ln, s, *a);
+         hipLaunchKernelGGL((k_bin_accum_narrow<2, false>), dim3(gn), dim3(FL3_ACC_THREADS), ln, s, *a);
    } else if (format == 1) {
       hipLaunchKernelGGL(k_bin_part1<unsigned>, dim3(a->nblk), dim3(256), 0, s, *a);
       if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned>, dim3(g2), dim3(256), lds2, s, *a);
