@@ -105,6 +105,9 @@ typedef struct {
 int fl3d_images_begin(size_t image_bytes, size_t filter_bytes);
 int fl3d_tonemap_async(const fl3d_tm_params *p, const double *filter, size_t img_off, size_t filt_off);
 int fl3d_images_fetch(const fl3d_image_dst *d, int n);
+/* The first `bytes` of the image arena into pinned host buffer `slot` (0 or
+ * 1, kept until the next download into the same slot); *host points at it. */
+int fl3d_images_download(int slot, size_t bytes, const unsigned char **host);
 /* Tone maps of renders 0..n-1 of fl3d_begin_many in one launch (after
  * fl3d_images_begin; render i's image at img_off[i]). */
 int fl3d_tonemap_many(const fl3d_tm_params *p, const double *const *filters, const size_t *img_off, int n);

@@ -96,6 +96,8 @@ struct Ctx {
    unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
    void *staging = nullptr;                   // pinned host buffer for the image download
    size_t staging_cap = 0;
+   void *slot[2] = {nullptr, nullptr};        // pinned buffers of fl3d_images_download
+   size_t slot_cap[2] = {0, 0};
    int width = 0, height = 0, bits = 33;
    // Several renders may share the histogram buffers (fl3d_begin_many):
    // render r's cells start at rcell0[r]; width/height/cell0 are those of
@@ -1082,16 +1084,35 @@ static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *d
    unsigned long long pb = 0, gb = 0;
    std::vector<unsigned char> hostt(tab);
    size_t off = 0;
+   // renders with equal filter banks (the same estimator parameters and
+   // oversampling) share one uploaded table
+   std::vector<std::pair<const fl3_de_filters *, size_t>> uniq;
    for (int k = 0; k < nd; k++) {
       const int i = dej[k];
       const fl3_de_filters *d = de[i];
       const size_t nc = (size_t)d->nfilters * d->kernel_size, nw = d->nfilters;
-      unsigned char *tb = (unsigned char *)g_ctx.detab.p + off;
-      memcpy(hostt.data() + off, d->filter_coefs, nc * sizeof(double));
-      memcpy(hostt.data() + off + nc * sizeof(double), d->filter_widths, nw * sizeof(double));
-      // the float copy, rounded to nearest as a device conversion would
-      float *ft = (float *)(hostt.data() + off + (nc + nw) * sizeof(double));
-      for (size_t e = 0; e < nc; e++) ft[e] = (float)d->filter_coefs[e];
+      size_t toff = off;
+      bool found = false;
+      for (const auto &u : uniq) {
+         const fl3_de_filters *e = u.first;
+         if (e->nfilters == d->nfilters && e->kernel_size == d->kernel_size &&
+             !memcmp(e->filter_coefs, d->filter_coefs, nc * sizeof(double)) &&
+             !memcmp(e->filter_widths, d->filter_widths, nw * sizeof(double))) {
+            toff = u.second;
+            found = true;
+            break;
+         }
+      }
+      if (!found) {
+         memcpy(hostt.data() + off, d->filter_coefs, nc * sizeof(double));
+         memcpy(hostt.data() + off + nc * sizeof(double), d->filter_widths, nw * sizeof(double));
+         // the float copy, rounded to nearest as a device conversion would
+         float *ft = (float *)(hostt.data() + off + (nc + nw) * sizeof(double));
+         for (size_t e = 0; e < nc; e++) ft[e] = (float)d->filter_coefs[e];
+         uniq.emplace_back(d, off);
+         off += (nc + nw) * sizeof(double) + ((nc * sizeof(float) + 15) & ~(size_t)15);
+      }
+      unsigned char *tb = (unsigned char *)g_ctx.detab.p + toff;
       DEArgs &a = as[k];
       memset(&a, 0, sizeof(a));
       const long long c0 = g_ctx.rcell0[i];
@@ -1121,7 +1142,6 @@ static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *d
       pb += (unsigned long long)((a.width + 255) / 256) * a.height;
       gb += (unsigned long long)((a.width + DG_REGION_HOST - 1) / DG_REGION_HOST) *
             ((a.height + DG_REGION_HOST - 1) / DG_REGION_HOST);
-      off += (nc + nw) * sizeof(double) + ((nc * sizeof(float) + 15) & ~(size_t)15);
    }
    if (pb > 0x7fffffffULL || gb > 0x7fffffffULL) {
       set_err("fl3d_density_many: too many workgroups");
@@ -1131,7 +1151,7 @@ static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *d
    gf[nd] = (unsigned)gb;
    const size_t ab = sizeof(DEArgs) * nd, fb = sizeof(unsigned) * first.size();
    if (g_ctx.deargs.ensure(ab + fb + 16)) return 1;
-   HIPCHK(hipMemcpyAsync(g_ctx.detab.p, hostt.data(), tab, hipMemcpyHostToDevice, g_ctx.stream));
+   HIPCHK(hipMemcpyAsync(g_ctx.detab.p, hostt.data(), off, hipMemcpyHostToDevice, g_ctx.stream));
    HIPCHK(hipMemcpyAsync(g_ctx.deargs.p, as.data(), ab, hipMemcpyHostToDevice, g_ctx.stream));
    HIPCHK(hipMemcpyAsync((unsigned char *)g_ctx.deargs.p + ab, first.data(), fb, hipMemcpyHostToDevice, g_ctx.stream));
    const int pk = prof_begin(ST_DE);
@@ -1331,6 +1351,30 @@ extern "C" int fl3d_images_fetch(const fl3d_image_dst *d, int n) {
          for (int j = 0; j < d[i].rows; j++)
             memcpy(dst + (size_t)j * d[i].out_stride, src + (size_t)j * d[i].row_bytes, d[i].row_bytes);
    }
+   return 0;
+}
+
+extern "C" int fl3d_images_download(int slot, size_t bytes, const unsigned char **host) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   if (slot < 0 || slot > 1 || !host) {
+      set_err("fl3d_images_download: bad slot");
+      return 1;
+   }
+   *host = nullptr;
+   if (bytes > g_ctx.image.cap) {
+      set_err("fl3d_images_download: %s%ld bytes beyond the image arena", "", (long)bytes);
+      return 1;
+   }
+   if (g_ctx.slot_cap[slot] < bytes) {
+      if (g_ctx.slot[slot]) (void)hipHostFree(g_ctx.slot[slot]);
+      g_ctx.slot[slot] = nullptr;
+      g_ctx.slot_cap[slot] = 0;
+      HIPCHK(hipHostMalloc(&g_ctx.slot[slot], bytes, hipHostMallocDefault));
+      g_ctx.slot_cap[slot] = bytes;
+   }
+   if (bytes) HIPCHK(hipMemcpyAsync(g_ctx.slot[slot], g_ctx.image.p, bytes, hipMemcpyDeviceToHost, g_ctx.stream));
+   HIPCHK(hipStreamSynchronize(g_ctx.stream));
+   *host = (const unsigned char *)g_ctx.slot[slot];
    return 0;
 }
 

@@ -96,6 +96,24 @@ fl3_many *fl3_many_prepare(fl3h_request *reqs, int n, const fl3h_render_opts *op
 int fl3_many_execute(fl3_many *B);
 void fl3_many_free(fl3_many *B);
 
+/* Downloaded images of a group of shared renders, not yet copied into the
+ * callers' buffers.  By default fl3_many_execute copies them itself; with a
+ * delivery set it downloads each group into a pinned slot it acquires and
+ * hands the group over, so that the copies (and the renders' completion)
+ * run on another thread while the device goes on with the next group. */
+typedef struct fl3_images fl3_images;
+typedef struct {
+   void *ctx;
+   int (*acquire)(void *ctx);                     /* a free slot (0 or 1); may block */
+   void (*hand_over)(void *ctx, fl3_images *im);  /* later: fl3_images_deliver(im) */
+} fl3_delivery;
+void fl3_many_set_delivery(fl3_many *B, const fl3_delivery *d);
+/* Copy the group's images into the callers' buffers, complete their stats
+ * and set their rc to 0 (rc stays 1 for a group whose download failed); frees
+ * im and returns the slot it held, now free again.  B must not be freed
+ * before every group handed over from it has been delivered. */
+int fl3_images_deliver(fl3_images *im);
+
 #ifdef __cplusplus
 }
 #endif

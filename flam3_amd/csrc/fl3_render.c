@@ -766,7 +766,51 @@ struct fl3_many {
    fl3d_multi_job *mj;
    int nmj;
    fl3d_multi_job **rj;    /* per request, before they go into mj */
+   fl3_delivery dv;        /* hand_over NULL: copy the images in place */
 };
+
+struct fl3_images {
+   fl3_many *B;
+   const unsigned char *host;   /* the downloaded arena; NULL if the download failed */
+   int slot, n;
+   fl3d_image_dst *dst;         /* n destinations */
+   int *req;                    /* n request indices */
+};
+
+void fl3_many_set_delivery(fl3_many *B, const fl3_delivery *d) {
+   if (B && d) B->dv = *d;
+}
+
+/* Rows of each image from the downloaded arena into the caller's buffer. */
+static void scatter_images(const unsigned char *host, const fl3d_image_dst *d, int n) {
+   int i, j;
+   for (i = 0; i < n; i++) {
+      const unsigned char *src = host + d[i].img_off;
+      unsigned char *dst = (unsigned char *)d[i].out;
+      if (d[i].out_stride == d[i].row_bytes)
+         memcpy(dst, src, d[i].row_bytes * (size_t)d[i].rows);
+      else
+         for (j = 0; j < d[i].rows; j++)
+            memcpy(dst + (size_t)j * d[i].out_stride, src + (size_t)j * d[i].row_bytes, d[i].row_bytes);
+   }
+}
+
+int fl3_images_deliver(fl3_images *im) {
+   int slot, i;
+   if (!im) return -1;
+   if (im->host) {
+      scatter_images(im->host, im->dst, im->n);
+      for (i = 0; i < im->n; i++) {
+         plan_finish(&im->B->P[im->req[i]]);
+         im->B->reqs[im->req[i]].rc = 0;
+      }
+   }
+   slot = im->slot;
+   free(im->dst);
+   free(im->req);
+   free(im);
+   return slot;
+}
 
 void fl3_many_free(fl3_many *B) {
    int i;
@@ -1043,11 +1087,38 @@ static int many_group(fl3_many *B, int i0, int i1) {
          free((void *)fls);
          free(tps);
       }
-      if (ok) ok = fl3d_images_fetch(dst, nl) == 0;
-      if (!ok) fprintf(stderr, "flam3_amd: device error: %s\n", fl3d_error());
-      for (i = 0; ok && i < nl; i++) {
-         plan_finish(&B->P[B->live[i0 + i]]);
-         B->reqs[B->live[i0 + i]].rc = 0;
+      if (ok && B->dv.hand_over) {
+         /* download into a free slot; the copies into the callers' buffers
+          * happen on the delivering thread */
+         fl3_images *im = (fl3_images *)calloc(1, sizeof(fl3_images));
+         int *req = (int *)malloc(sizeof(int) * (size_t)nl);
+         ok = im && req;
+         if (ok) {
+            im->B = B;
+            im->slot = B->dv.acquire(B->dv.ctx);
+            im->n = nl;
+            im->dst = dst;
+            im->req = req;
+            for (i = 0; i < nl; i++) req[i] = B->live[i0 + i];
+            dst = NULL;
+            req = NULL;
+            if (fl3d_images_download(im->slot, img_total, &im->host)) {
+               ok = 0;
+               im->host = NULL;
+            }
+            B->dv.hand_over(B->dv.ctx, im);   /* also a failed one: it frees the slot */
+            im = NULL;
+         }
+         free(im);
+         free(req);
+         if (!ok) fprintf(stderr, "flam3_amd: device error: %s\n", fl3d_error());
+      } else {
+         if (ok) ok = fl3d_images_fetch(dst, nl) == 0;
+         if (!ok) fprintf(stderr, "flam3_amd: device error: %s\n", fl3d_error());
+         for (i = 0; ok && i < nl; i++) {
+            plan_finish(&B->P[B->live[i0 + i]]);
+            B->reqs[B->live[i0 + i]].rc = 0;
+         }
       }
    }
    free(mj);

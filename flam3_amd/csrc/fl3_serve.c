@@ -2,16 +2,18 @@
  * Render server: continuous batching of independent flam3_render requests
  * (genome-search previews, animation frames) on one GPU.
  *
- * Two threads.  The planner takes the pending requests in arrival order,
+ * Three threads.  The planner takes the pending requests in arrival order,
  * admits as many as fit the histogram budget and the batch limit, and does
  * their host work (fl3_many_prepare: interpolation, tables, jobs, keys, DE
  * filters).  The runner executes the prepared batch on the GPU
  * (fl3_many_execute: one shared walker launch for the small renders, then
  * their density estimation and tone maps; large renders one after another).
- * While the runner has a batch on the GPU the planner prepares the next one
- * from whatever arrived meanwhile, so requests that arrive while others are
- * on the GPU are admitted as soon as the GPU frees.  Every request comes out
- * as flam3_render would make it alone.
+ * The finisher copies each downloaded group of images into the callers'
+ * buffers, completes the batch and frees it, while the runner already has
+ * the next batch on the GPU.  While the runner has a batch on the GPU the
+ * planner prepares the next one from whatever arrived meanwhile, so requests
+ * that arrive while others are on the GPU are admitted as soon as the GPU
+ * frees.  Every request comes out as flam3_render would make it alone.
  */
 #include <math.h>
 #include <pthread.h>
@@ -29,10 +31,21 @@ typedef struct {
    double t_submit, t_done;
 } srv_rec;
 
+/* Finisher queue entry: a group of downloaded images, or the end of a
+ * batch (every group of it queued before). */
+typedef struct fin_item {
+   struct fin_item *next;
+   fl3_images *im;            /* NULL: end of batch */
+   fl3_many *B;
+   fl3h_request *reqs;
+   long long *tickets;
+   int n;
+} fin_item;
+
 struct fl3h_server {
    pthread_mutex_t mu;
-   pthread_cond_t cv_work, cv_ready, cv_done;
-   pthread_t planner, runner;
+   pthread_cond_t cv_work, cv_ready, cv_done, cv_fin, cv_slot;
+   pthread_t planner, runner, finisher;
    int threads;
    srv_rec **recs;        /* by ticket */
    long long nrecs, cap;
@@ -47,6 +60,10 @@ struct fl3h_server {
    int ready_n;
    long long batches, renders;
    double prep_s, exec_s;   /* planner and runner busy time */
+   /* finisher queue and the two download slots */
+   fin_item *fin_head, *fin_tail;
+   int runner_done;
+   int slot_busy[2], next_slot;
 };
 
 static double now_s(void) {
@@ -128,6 +145,82 @@ static void *planner_main(void *arg) {
    return NULL;
 }
 
+static void fin_push(fl3h_server *s, fin_item *it) {
+   pthread_mutex_lock(&s->mu);
+   it->next = NULL;
+   if (s->fin_tail) s->fin_tail->next = it;
+   else s->fin_head = it;
+   s->fin_tail = it;
+   pthread_cond_signal(&s->cv_fin);
+   pthread_mutex_unlock(&s->mu);
+}
+
+/* fl3_delivery hooks (runner thread) */
+static int slot_acquire(void *ctx) {
+   fl3h_server *s = (fl3h_server *)ctx;
+   int k;
+   pthread_mutex_lock(&s->mu);
+   while (s->slot_busy[s->next_slot]) pthread_cond_wait(&s->cv_slot, &s->mu);
+   k = s->next_slot;
+   s->slot_busy[k] = 1;
+   s->next_slot ^= 1;
+   pthread_mutex_unlock(&s->mu);
+   return k;
+}
+
+static void images_hand_over(void *ctx, fl3_images *im) {
+   fin_item *it = (fin_item *)calloc(1, sizeof(fin_item));
+   if (!it) {   /* deliver here rather than lose the group */
+      fl3h_server *s = (fl3h_server *)ctx;
+      const int k = fl3_images_deliver(im);
+      pthread_mutex_lock(&s->mu);
+      if (k >= 0) s->slot_busy[k] = 0;
+      pthread_cond_broadcast(&s->cv_slot);
+      pthread_mutex_unlock(&s->mu);
+      return;
+   }
+   it->im = im;
+   fin_push((fl3h_server *)ctx, it);
+}
+
+static void *finisher_main(void *arg) {
+   fl3h_server *s = (fl3h_server *)arg;
+   pthread_mutex_lock(&s->mu);
+   for (;;) {
+      fin_item *it;
+      int i;
+      while (s->fin_head == NULL && !s->runner_done) pthread_cond_wait(&s->cv_fin, &s->mu);
+      if (s->fin_head == NULL) break;
+      it = s->fin_head;
+      s->fin_head = it->next;
+      if (!s->fin_head) s->fin_tail = NULL;
+      pthread_mutex_unlock(&s->mu);
+      if (it->im) {
+         const int k = fl3_images_deliver(it->im);
+         pthread_mutex_lock(&s->mu);
+         if (k >= 0) s->slot_busy[k] = 0;
+         pthread_cond_broadcast(&s->cv_slot);
+      } else {
+         fl3_many_free(it->B);
+         pthread_mutex_lock(&s->mu);
+         for (i = 0; i < it->n; i++) {
+            srv_rec *r = s->recs[it->tickets[i]];
+            r->rc = it->reqs[i].rc;
+            r->state = 2;
+            r->t_done = now_s();
+         }
+         s->batches++;
+         s->renders += it->n;
+         free(it->reqs);
+         free(it->tickets);
+         pthread_cond_broadcast(&s->cv_done);
+      }
+      free(it);
+   }
+   pthread_mutex_unlock(&s->mu);
+   return NULL;
+}
+
 static void *runner_main(void *arg) {
    fl3h_server *s = (fl3h_server *)arg;
    pthread_mutex_lock(&s->mu);
@@ -147,9 +240,28 @@ static void *runner_main(void *arg) {
       pthread_cond_signal(&s->cv_work);   /* the planner may prepare the next batch */
       pthread_mutex_unlock(&s->mu);
       te = now_s();
-      fl3_many_execute(B);
+      {
+         fl3_delivery dv;
+         fin_item *end = (fin_item *)calloc(1, sizeof(fin_item));
+         dv.ctx = s;
+         dv.acquire = slot_acquire;
+         dv.hand_over = images_hand_over;
+         if (end) fl3_many_set_delivery(B, &dv);   /* else the images are copied in place */
+         fl3_many_execute(B);
+         te = now_s() - te;
+         if (end) {
+            /* the finisher completes the batch after its image groups */
+            end->B = B;
+            end->reqs = reqs;
+            end->tickets = tickets;
+            end->n = n;
+            fin_push(s, end);
+            pthread_mutex_lock(&s->mu);
+            s->exec_s += te;
+            continue;
+         }
+      }
       fl3_many_free(B);
-      te = now_s() - te;
       pthread_mutex_lock(&s->mu);
       s->exec_s += te;
       for (i = 0; i < n; i++) {
@@ -164,8 +276,28 @@ static void *runner_main(void *arg) {
       free(tickets);
       pthread_cond_broadcast(&s->cv_done);
    }
+   s->runner_done = 1;
+   pthread_cond_signal(&s->cv_fin);
    pthread_mutex_unlock(&s->mu);
    return NULL;
+}
+
+/* Stop and join the threads started so far: the planner ends when every
+ * admitted request is prepared, the runner when the planner has ended and
+ * the last batch ran, the finisher when the runner has ended and its queue
+ * is empty.  A thread that was never started is marked ended here. */
+static void stop_threads(fl3h_server *s) {
+   pthread_mutex_lock(&s->mu);
+   s->stop = 1;
+   pthread_cond_broadcast(&s->cv_work);
+   if (s->threads < 2) s->planner_done = 1;
+   if (s->threads < 3) s->runner_done = 1;
+   pthread_cond_broadcast(&s->cv_ready);
+   pthread_cond_broadcast(&s->cv_fin);
+   pthread_mutex_unlock(&s->mu);
+   if (s->threads >= 2) pthread_join(s->planner, NULL);
+   if (s->threads >= 3) pthread_join(s->runner, NULL);
+   if (s->threads >= 1) pthread_join(s->finisher, NULL);
 }
 
 fl3h_server *fl3h_server_create(const fl3h_server_opts *o) {
@@ -179,19 +311,24 @@ fl3h_server *fl3h_server_create(const fl3h_server_opts *o) {
    pthread_cond_init(&s->cv_work, NULL);
    pthread_cond_init(&s->cv_ready, NULL);
    pthread_cond_init(&s->cv_done, NULL);
-   if (pthread_create(&s->planner, NULL, planner_main, s)) goto bad;
+   pthread_cond_init(&s->cv_fin, NULL);
+   pthread_cond_init(&s->cv_slot, NULL);
+   if (pthread_create(&s->finisher, NULL, finisher_main, s)) goto bad;
    s->threads = 1;
-   if (pthread_create(&s->runner, NULL, runner_main, s)) {
-      pthread_mutex_lock(&s->mu);
-      s->stop = 1;
-      pthread_cond_broadcast(&s->cv_work);
-      pthread_mutex_unlock(&s->mu);
-      pthread_join(s->planner, NULL);
+   if (pthread_create(&s->planner, NULL, planner_main, s)) {
+      stop_threads(s);
       goto bad;
    }
    s->threads = 2;
+   if (pthread_create(&s->runner, NULL, runner_main, s)) {
+      stop_threads(s);
+      goto bad;
+   }
+   s->threads = 3;
    return s;
 bad:
+   pthread_cond_destroy(&s->cv_slot);
+   pthread_cond_destroy(&s->cv_fin);
    pthread_cond_destroy(&s->cv_done);
    pthread_cond_destroy(&s->cv_ready);
    pthread_cond_destroy(&s->cv_work);
@@ -278,14 +415,11 @@ void fl3h_server_counts(fl3h_server *s, long long *batches, long long *renders, 
 void fl3h_server_destroy(fl3h_server *s) {
    long long i;
    if (!s) return;
-   pthread_mutex_lock(&s->mu);
-   s->stop = 1;
-   pthread_cond_broadcast(&s->cv_work);
-   pthread_mutex_unlock(&s->mu);
-   if (s->threads >= 1) pthread_join(s->planner, NULL);
-   if (s->threads >= 2) pthread_join(s->runner, NULL);
+   stop_threads(s);
    for (i = 0; i < s->nrecs; i++) free(s->recs[i]);
    free(s->recs);
+   pthread_cond_destroy(&s->cv_slot);
+   pthread_cond_destroy(&s->cv_fin);
    pthread_cond_destroy(&s->cv_done);
    pthread_cond_destroy(&s->cv_ready);
    pthread_cond_destroy(&s->cv_work);

@@ -76,3 +76,18 @@ def test_histogram_budget_limits_a_batch(prod):
     for (g, s), (img, _, _) in zip(small[:3], res[:3]):
         want, _ = prod.render(g, seed=s)
         assert np.array_equal(img, want)
+
+
+def test_batch_of_several_image_groups(prod):
+    """A batch above the shared launch's cell limit (32e6 cells) runs as
+    several groups; each group's images are downloaded into one of two
+    pinned slots and copied out while the next group runs."""
+    reqs = [_req(900 + k, size=512, quality=5) for k in range(240)]   # 240 x 0.29e6 cells: 3 groups
+    with Server(hist_budget_bytes=1e12) as srv:
+        ts = [srv.submit(g, seed=s) for g, s in reqs]
+        res = [srv.wait(t) for t in ts]
+        batches, renders = srv.counts()
+    assert renders == len(reqs)
+    for k in (0, 108, 110, 200, 239):
+        want, _ = prod.render(reqs[k][0], seed=reqs[k][1])
+        assert np.array_equal(res[k][0], want), k
