@@ -399,11 +399,15 @@ static bool same_tables(const fl3_job_hdr *a, const fl3_job_hdr *b) {
 }
 
 // Palette (as integers and floats) and visibilities of every job of a
-// launch, for the accumulation of job records.
-static int upload_job_tables(const fl3_job_hdr *const *h, int n, int nxf_max, BinArgs *b) {
+// launch, for the accumulation of job records; *pal8 (if not null) gets
+// the palettes as bytes R | G << 8 | B << 16 | A << 24 per entry, for
+// records that carry the colour, when every entry is below 256 (else null).
+static int upload_job_tables(const fl3_job_hdr *const *h, int n, int nxf_max, BinArgs *b,
+                             const uint32_t **pal8 = nullptr) {
    const int vs = nxf_max > 0 ? nxf_max : 1;
-   std::vector<uint32_t> du((size_t)n * 1024);
+   std::vector<uint32_t> du((size_t)n * 1024), pb((size_t)n * 256);
    std::vector<float> df((size_t)n * 1024), vis((size_t)n * vs, 1.0f);
+   bool bytes = true;
    for (int j = 0; j < n; j++) {
       const unsigned char *hb = (const unsigned char *)h[j];
       const double *dm = (const double *)(hb + h[j]->off_dmap);
@@ -411,21 +415,37 @@ static int upload_job_tables(const fl3_job_hdr *const *h, int n, int nxf_max, Bi
       for (int e = 0; e < 1024; e++) {
          du[(size_t)j * 1024 + e] = (uint32_t)dm[e];
          df[(size_t)j * 1024 + e] = (float)dm[e];
+         bytes = bytes && dm[e] < 256.0;
+      }
+      for (int e = 0; e < 256; e++) {
+         const uint32_t *c = &du[(size_t)j * 1024 + 4 * e];
+         pb[(size_t)j * 256 + e] = (c[0] & 0xffu) | (c[1] & 0xffu) << 8 | (c[2] & 0xffu) << 16 | (c[3] & 0xffu) << 24;
       }
       for (int e = 0; e < h[j]->nxf; e++) vis[(size_t)j * vs + e] = (float)px[e].vis;
    }
-   const size_t nb = du.size() * 4, fb = df.size() * 4, vb = vis.size() * 4;
-   if (g_ctx.jobtab.ensure(nb + fb + vb)) return 1;
+   const size_t nb = du.size() * 4, fb = df.size() * 4, vb = vis.size() * 4, pbb = pb.size() * 4;
+   if (g_ctx.jobtab.ensure(nb + fb + vb + pbb)) return 1;
    unsigned char *t = (unsigned char *)g_ctx.jobtab.p;
    HIPCHK(hipMemcpyAsync(t, du.data(), nb, hipMemcpyHostToDevice, g_ctx.stream));
    HIPCHK(hipMemcpyAsync(t + nb, df.data(), fb, hipMemcpyHostToDevice, g_ctx.stream));
    HIPCHK(hipMemcpyAsync(t + nb + fb, vis.data(), vb, hipMemcpyHostToDevice, g_ctx.stream));
+   HIPCHK(hipMemcpyAsync(t + nb + fb + vb, pb.data(), pbb, hipMemcpyHostToDevice, g_ctx.stream));
    HIPCHK(hipStreamSynchronize(g_ctx.stream));  // the host vectors go out of scope
    b->dmapu_all = (const uint32_t *)t;
    b->dmapf_all = (const float *)(t + nb);
    b->vis_all = (const float *)(t + nb + fb);
    b->vis_stride = vs;
+   if (pal8) *pal8 = bytes ? (const uint32_t *)(t + nb + fb + vb) : nullptr;
    return 0;
+}
+
+// Job records carry the colour (and take the packed accumulation) when
+// every job that writes them has full visibility and a stepped palette of
+// entries below 256, the tiles hold 2048 or 4096 cells and the walker's LDS
+// has room for the palette after its tile counts.
+static bool colour_records(const fl3_job_hdr *const *h, int n, const uint32_t *pal8, int shift, size_t lds_walker) {
+   return pal8 && (shift == 11 || shift == 12) && job_records_packed(h, n) &&
+          ((lds_walker + 15) & ~(size_t)15) + 1024 <= g_ctx.lds_max;
 }
 
 // Records per LDS round of the packed 4-byte-record accumulation: the
@@ -513,14 +533,20 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    b.width = g_ctx.width;
    b.height = g_ctx.height;
    b.chunk = item_records(js.h, js.n);
-   b.round = narrow || (format == 2 && job_records_packed(js.h, js.n)) ? narrow_round(js.h, js.n) : 0;
-   if (format == 2 && upload_job_tables(js.h, js.n, js.nxf_max, &b)) return 1;
+   const uint32_t *pal8 = nullptr;
+   if (format == 2 && upload_job_tables(js.h, js.n, js.nxf_max, &b, &pal8)) return 1;
    a.rec_mode = narrow ? 2 : (format == 2 ? 3 : 1);
    a.wstate = (uint32_t *)g_ctx.wstate.p;
    a.bin_shift = shift;
    a.nbins = nbins;
    a.hist_off = (unsigned)((js.lds_rec + 15) & ~(size_t)15);
-   const size_t lds_rec = a.hist_off + sizeof(unsigned) * (size_t)nbins;
+   size_t lds_rec = a.hist_off + sizeof(unsigned) * (size_t)nbins;
+   // job records with the colour in them: the packed accumulation
+   const bool colour = format == 2 && colour_records(js.h, js.n, pal8, shift, lds_rec);
+   a.pal8 = colour ? pal8 : nullptr;
+   a.pal_off = (unsigned)((lds_rec + 15) & ~(size_t)15);
+   if (colour) lds_rec = a.pal_off + 1024;
+   b.round = narrow || colour ? narrow_round(js.h, js.n) : 0;
    // chunk k: walkers on the render stream, then its sort and accumulation
    // on the bin stream, which chunk k + 2's walkers wait for before they
    // overwrite the same record buffer
@@ -848,13 +874,8 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
       b.width = (int)(tcells < 0x7fffffffLL ? tcells : 0x7fffffffLL);
       b.height = 1;
       b.chunk = item_records(hs.data(), njobs);
-      {
-         std::vector<const fl3_job_hdr *> rh;   // the jobs that write records
-         for (int j = 0; j < njobs; j++)
-            if (grp[j] >= MG_REC) rh.push_back(hs[j]);
-         b.round = job_records_packed(rh.data(), (int)rh.size()) ? narrow_round(hs.data(), njobs) : 0;
-      }
-      if (upload_job_tables(hs.data(), njobs, nxf_max, &b)) return 1;
+      const uint32_t *pal8 = nullptr;
+      if (upload_job_tables(hs.data(), njobs, nxf_max, &b, &pal8)) return 1;
       IterArgs ar = a;
       ar.rec_mode = 3;   // job records
       ar.wstate = (uint32_t *)g_ctx.wstate.p;
@@ -862,7 +883,19 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
       ar.nbins = nbins;
       ar.hist_off = hist_off;
       ar.wtot = Wrec;
-      const size_t lds_r = hist_off + sizeof(unsigned) * (size_t)nbins;
+      size_t lds_r = hist_off + sizeof(unsigned) * (size_t)nbins;
+      {
+         // records with the colour in them (the packed accumulation) when
+         // every job that writes records allows it
+         std::vector<const fl3_job_hdr *> rh;
+         for (int j = 0; j < njobs; j++)
+            if (grp[j] >= MG_REC) rh.push_back(hs[j]);
+         const bool colour = colour_records(rh.data(), (int)rh.size(), pal8, shift, lds_r);
+         ar.pal8 = colour ? pal8 : nullptr;
+         ar.pal_off = (unsigned)((lds_r + 15) & ~(size_t)15);
+         if (colour) lds_r = ar.pal_off + 1024;
+         b.round = colour ? narrow_round(hs.data(), njobs) : 0;
+      }
       hipStream_t ws = g_ctx.stream, bs = g_ctx.bin_stream;
       HIPCHK(hipEventRecord(g_ctx.ev_join, ws));
       HIPCHK(hipStreamWaitEvent(bs, g_ctx.ev_join, 0));

@@ -51,6 +51,11 @@ struct IterArgs {
    unsigned *bin_total;       // [tile] record counts over all workgroups (added atomically)
    int bin_shift, nbins;
    unsigned hist_off;         // LDS byte offset of the workgroup's tile counts
+   // job records carrying the colour: [job][256] palette entries as bytes
+   // R | G << 8 | B << 16 | A << 24 (null: palette index, xform and job),
+   // held in LDS at pal_off
+   const uint32_t *pal8;
+   unsigned pal_off;
    // walker state kept between launches of one batch (10 words per walker,
    // structure of arrays): resume = continue from it without fuse steps
    uint32_t *wstate;

@@ -306,9 +306,9 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum(BinArgs a) {
 }
 
 // Accumulation of records of visibility 1 with a stepped palette (a record
-// adds its palette entry and 255 hits): 4-byte records, or job records whose
-// palette is their job's.  The tile's cells live in LDS as two
-// packed 64-bit words, R:24 | G:24 | count:16 and B:32 | A:32, so a record
+// adds its palette entry and 255 hits): 4-byte records, or job records that
+// carry the entry's bytes (palettes of entries below 256).  The tile's cells
+// live in LDS as two packed 64-bit words, R:24 | G:24 | count:16 and B:32 | A:32, so a record
 // costs two LDS atomics; a round of at most `round` records keeps every
 // field below its width, after which each thread folds the CPT cells it
 // owns into registers and clears them.  The tile is then added to the
@@ -364,13 +364,20 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_narrow(BinArgs a)
             }
             for (int u = 0; u < FL3_ACC_UNROLL; u++) {
                if ((unsigned)v[u] == FL3_REC_NONE) continue;
-               // 4-byte record: cell | palette index << 24; job record: cell,
-               // palette index at bit 32, job at bit 48 (its own palette)
-               const unsigned local = (JOB ? (unsigned)v[u] : ((unsigned)v[u] & 0xffffffu)) - cell0;
-               const uint32_t *du = JOB ? a.dmapu_all + (size_t)(v[u] >> 48) * 1024 + ((v[u] >> 32) & 0xff) * 4
-                                        : dmapu + ((unsigned)v[u] >> 24) * 4;
-               atomicAdd(&p0[local], (unsigned long long)du[0] | ((unsigned long long)du[1] << 24) | (1ull << 48));
-               atomicAdd(&p1[local], (unsigned long long)du[2] | ((unsigned long long)du[3] << 32));
+               // 4-byte record: cell | palette index << 24; job record: cell
+               // | the palette entry's bytes R, G, B, A << 32
+               if (JOB) {
+                  const unsigned local = (unsigned)v[u] - cell0;
+                  const unsigned c = (unsigned)(v[u] >> 32);
+                  atomicAdd(&p0[local], (unsigned long long)(c & 0xffu) | ((unsigned long long)((c >> 8) & 0xffu) << 24) |
+                                            (1ull << 48));
+                  atomicAdd(&p1[local], (unsigned long long)((c >> 16) & 0xffu) | ((unsigned long long)(c >> 24) << 32));
+               } else {
+                  const unsigned local = ((unsigned)v[u] & 0xffffffu) - cell0;
+                  const uint32_t *du = dmapu + ((unsigned)v[u] >> 24) * 4;
+                  atomicAdd(&p0[local], (unsigned long long)du[0] | ((unsigned long long)du[1] << 24) | (1ull << 48));
+                  atomicAdd(&p1[local], (unsigned long long)du[2] | ((unsigned long long)du[3] << 32));
+               }
             }
          }
          __syncthreads();

@@ -10,6 +10,7 @@
 #define FL3_REC_WIDE 1     // 8-byte records (make_record)
 #define FL3_REC_NARROW 2   // 4-byte records: cell | palette index << 24
 #define FL3_REC_JOB 3      // 8-byte records: cell, palette index, xform, job
+                           // (or cell and the palette entry's four bytes)
 
 // Records: row i of walker (workgroup b, thread t) at (b * rows_out + i) *
 // 256 + t, so a workgroup's records are one contiguous block.
@@ -35,6 +36,11 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
    unsigned *bh = (unsigned *)(lds + a.hist_off);
    if (RECORDS)
       for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) bh[b] = 0;
+   uint32_t *pal = (uint32_t *)(lds + a.pal_off);
+   if (RECORDS == FL3_REC_JOB && a.pal8) {
+      const uint32_t *src = a.pal8 + (size_t)(dsc ? djob : jb) * 256;
+      for (int e = threadIdx.x; e < 256; e += blockDim.x) pal[e] = src[e];
+   }
    __syncthreads();
    // W, wid: the record matrix and walker-state layout over all jobs (with
    // independent renders the launch may cover only the columns from
@@ -113,10 +119,10 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
             // into the launch's cell space, the job is the descriptor's
             const unsigned long long v = make_record<float>(G, q, fn);
             const unsigned cell = (unsigned)v == FL3_REC_NONE ? FL3_REC_NONE : (unsigned)(v + cell0);
-            a.rec[(blk * a.rows_out + i) * 256 + threadIdx.x] = cell == FL3_REC_NONE
-                                     ? (unsigned long long)FL3_REC_NONE
-                                     : ((v & 0xffff00000000ull) | cell |
-                                        ((unsigned long long)(dsc ? djob : jb) << 48));
+            const unsigned long long hi = a.pal8 ? (unsigned long long)pal[(v >> 32) & 0xff] << 32
+                                                 : (v & 0xffff00000000ull) | ((unsigned long long)(dsc ? djob : jb) << 48);
+            a.rec[(blk * a.rows_out + i) * 256 + threadIdx.x] =
+                cell == FL3_REC_NONE ? (unsigned long long)FL3_REC_NONE : (hi | cell);
             if (cell != FL3_REC_NONE) atomicAdd(&bh[cell >> a.bin_shift], 1u);
          } else if (RECORDS == FL3_REC_NARROW) {
             const unsigned long long v = make_record<float>(G, q, 0);
