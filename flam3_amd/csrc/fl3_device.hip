@@ -84,13 +84,13 @@ struct Ctx {
    size_t lds_max = 65536;   // LDS bytes one workgroup may allocate
    hipStream_t stream = nullptr;
    // The tile-binned deposit sorts and accumulates chunk k on bin_stream
-   // while the walkers produce chunk k + 1 on stream: records and tile
-   // counts are double-buffered (rec/blkhist for even chunks, rec2/blkhist2
-   // for odd ones).
+   // while the walkers produce the next chunks on stream: chunk k's records
+   // and tile counts are set k % FL3_CHUNK_BUFS of crec/cblk/ctot.
    hipStream_t bin_stream = nullptr;
-   hipEvent_t ev_walk[2] = {nullptr, nullptr}, ev_bin[2] = {nullptr, nullptr}, ev_join = nullptr;
+   hipEvent_t ev_walk[FL3_CHUNK_BUFS] = {}, ev_bin[FL3_CHUNK_BUFS] = {}, ev_join = nullptr;
    DevBuf buckets, accum, job, table, image, isaac, points, misc, findex;
-   DevBuf rec, rec2, sorted, wstate, bins, blkhist, blkhist2, jobtab, soff, bintot, bintot2;   // tile-binned deposit
+   DevBuf crec[FL3_CHUNK_BUFS], cblk[FL3_CHUNK_BUFS], ctot[FL3_CHUNK_BUFS];   // tile-binned deposit
+   DevBuf sorted, wstate, bins, jobtab, soff;
    DevBuf srcv, srcl;                  // density estimation per-source values
    DevBuf detab, deargs;               // batched density estimation: tables, per-render arguments
    unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
@@ -121,7 +121,7 @@ static int ctx_init() {
    g_ctx.lds_max = prop.sharedMemPerBlock;
    HIPCHK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
    HIPCHK(hipStreamCreateWithFlags(&g_ctx.bin_stream, hipStreamNonBlocking));
-   for (int i = 0; i < 2; i++) {
+   for (int i = 0; i < FL3_CHUNK_BUFS; i++) {
       HIPCHK(hipEventCreateWithFlags(&g_ctx.ev_walk[i], hipEventDisableTiming));
       HIPCHK(hipEventCreateWithFlags(&g_ctx.ev_bin[i], hipEventDisableTiming));
    }
@@ -336,6 +336,16 @@ static int pick_tier(const fl3_job_hdr *h) {
    return FL3_TIER_FULL;
 }
 
+// Record, per-workgroup count and tile-total buffers of the chunks of a
+// binned deposit (as many sets as chunks, at most FL3_CHUNK_BUFS).
+static int ensure_chunk_bufs(long long nchunks, size_t rec_bytes, size_t blk_bytes, size_t tot_bytes) {
+   const int nb = nchunks < FL3_CHUNK_BUFS ? (nchunks > 0 ? (int)nchunks : 1) : FL3_CHUNK_BUFS;
+   for (int i = 0; i < nb; i++)
+      if (g_ctx.crec[i].ensure(rec_bytes) || g_ctx.cblk[i].ensure(blk_bytes) || g_ctx.ctot[i].ensure(tot_bytes))
+         return 1;
+   return 0;
+}
+
 // Tile shift of the binned deposit for the current histogram (cells per
 // tile = 1 << shift), or 0 when the histogram is too large for it.
 static int bin_shift_cells(long long ncells);
@@ -500,13 +510,10 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    const long long nblk = W / 256;
    const size_t rec_bytes = (narrow ? 4 : 8) * (size_t)W * per;
    const long long nchunks = (p->nsamples + Wj * per - 1) / (Wj * per);
-   if (g_ctx.rec.ensure(rec_bytes) || g_ctx.sorted.ensure(rec_bytes)) return 1;
-   if (nchunks > 1 && g_ctx.rec2.ensure(rec_bytes)) return 1;
+   if (g_ctx.sorted.ensure(rec_bytes)) return 1;
    if (g_ctx.wstate.ensure(sizeof(uint32_t) * 10 * (size_t)W)) return 1;
    const size_t nsup = (size_t)((nbins + (1 << FL3_SUPER_SHIFT) - 1) >> FL3_SUPER_SHIFT);
-   if (g_ctx.blkhist.ensure(sizeof(unsigned) * (size_t)nblk * nsup)) return 1;
-   if (nchunks > 1 && g_ctx.blkhist2.ensure(sizeof(unsigned) * (size_t)nblk * nsup)) return 1;
-   if (g_ctx.bintot.ensure(sizeof(unsigned) * (size_t)nbins) || g_ctx.bintot2.ensure(sizeof(unsigned) * (size_t)nbins))
+   if (ensure_chunk_bufs(nchunks, rec_bytes, sizeof(unsigned) * (size_t)nblk * nsup, sizeof(unsigned) * (size_t)nbins))
       return 1;
    const int nsuper = (nbins + (1 << FL3_SUPER_SHIFT) - 1) >> FL3_SUPER_SHIFT;
    if (g_ctx.bins.ensure(sizeof(unsigned) * (4 * (size_t)nbins + 8))) return 1;
@@ -516,9 +523,9 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    memset(&b, 0, sizeof(b));
    b.job = js.n > 1 ? js.dev_all : js.dev_one;  // palette tables of job 0 (identical for all when narrow)
    b.buckets = a.buckets;
-   b.rec = g_ctx.rec.p;
+   b.rec = g_ctx.crec[0].p;
    b.sorted = g_ctx.sorted.p;
-   b.blk_hist = (unsigned *)g_ctx.blkhist.p;
+   b.blk_hist = (unsigned *)g_ctx.cblk[0].p;
    b.nblk = (int)nblk;
    b.walkers = W;
    b.bin_total = bw;
@@ -555,15 +562,15 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    HIPCHK(hipStreamWaitEvent(bs, g_ctx.ev_join, 0));
    long long k = 0;
    for (long long done = 0; done < p->nsamples; k++) {
-      const int buf = (int)(k & 1);
+      const int buf = (int)(k % FL3_CHUNK_BUFS);
       const long long c = p->nsamples - done < Wj * per ? p->nsamples - done : Wj * per;
       a.nsamples = c;
       a.rows_out = (c + Wj - 1) / Wj;
       a.resume = done > 0;
-      a.rec = (unsigned long long *)(buf ? g_ctx.rec2.p : g_ctx.rec.p);
-      a.blk_hist = (unsigned *)(buf ? g_ctx.blkhist2.p : g_ctx.blkhist.p);
-      a.bin_total = (unsigned *)(buf ? g_ctx.bintot2.p : g_ctx.bintot.p);
-      if (k >= 2) HIPCHK(hipStreamWaitEvent(ws, g_ctx.ev_bin[buf], 0));
+      a.rec = (unsigned long long *)g_ctx.crec[buf].p;
+      a.blk_hist = (unsigned *)g_ctx.cblk[buf].p;
+      a.bin_total = (unsigned *)g_ctx.ctot[buf].p;
+      if (k >= FL3_CHUNK_BUFS) HIPCHK(hipStreamWaitEvent(ws, g_ctx.ev_bin[buf], 0));
       HIPCHK(hipMemsetAsync(a.bin_total, 0, sizeof(unsigned) * (size_t)nbins, ws));
       int pk = prof_begin(ST_ITER, ws);
       int lrc = fl3k_launch_iter_fast(&a, (unsigned)nblk, lds_rec, ws, g_ctx.bits, 1, js.tier);
@@ -842,14 +849,11 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
       const long long nblk = Wrec / 256;
       const long long nchunks = (rows_total + per - 1) / per;
       const size_t rec_bytes = 8 * (size_t)Wrec * per;
-      if (g_ctx.rec.ensure(rec_bytes) || g_ctx.sorted.ensure(rec_bytes)) return 1;
-      if (nchunks > 1 && g_ctx.rec2.ensure(rec_bytes)) return 1;
+      if (g_ctx.sorted.ensure(rec_bytes)) return 1;
       if (g_ctx.wstate.ensure(sizeof(uint32_t) * 10 * (size_t)Wrec)) return 1;
       const size_t nsup = (size_t)((nbins + (1 << FL3_SUPER_SHIFT) - 1) >> FL3_SUPER_SHIFT);
-      if (g_ctx.blkhist.ensure(sizeof(unsigned) * (size_t)nblk * nsup)) return 1;
-      if (nchunks > 1 && g_ctx.blkhist2.ensure(sizeof(unsigned) * (size_t)nblk * nsup)) return 1;
-      if (g_ctx.bintot.ensure(sizeof(unsigned) * (size_t)nbins) ||
-          g_ctx.bintot2.ensure(sizeof(unsigned) * (size_t)nbins))
+      if (ensure_chunk_bufs(nchunks, rec_bytes, sizeof(unsigned) * (size_t)nblk * nsup,
+                            sizeof(unsigned) * (size_t)nbins))
          return 1;
       const int nsuper = (nbins + (1 << FL3_SUPER_SHIFT) - 1) >> FL3_SUPER_SHIFT;
       if (g_ctx.bins.ensure(sizeof(unsigned) * (4 * (size_t)nbins + 8))) return 1;
@@ -900,14 +904,14 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
       HIPCHK(hipEventRecord(g_ctx.ev_join, ws));
       HIPCHK(hipStreamWaitEvent(bs, g_ctx.ev_join, 0));
       for (long long k = 0; k < nchunks; k++) {
-         const int buf = (int)(k & 1);
+         const int buf = (int)(k % FL3_CHUNK_BUFS);
          ar.row0 = k * per;
          ar.rows_out = rows_total - ar.row0 < per ? rows_total - ar.row0 : per;
          ar.resume = k > 0;
-         ar.rec = (unsigned long long *)(buf ? g_ctx.rec2.p : g_ctx.rec.p);
-         ar.blk_hist = (unsigned *)(buf ? g_ctx.blkhist2.p : g_ctx.blkhist.p);
-         ar.bin_total = (unsigned *)(buf ? g_ctx.bintot2.p : g_ctx.bintot.p);
-         if (k >= 2) HIPCHK(hipStreamWaitEvent(ws, g_ctx.ev_bin[buf], 0));
+         ar.rec = (unsigned long long *)g_ctx.crec[buf].p;
+         ar.blk_hist = (unsigned *)g_ctx.cblk[buf].p;
+         ar.bin_total = (unsigned *)g_ctx.ctot[buf].p;
+         if (k >= FL3_CHUNK_BUFS) HIPCHK(hipStreamWaitEvent(ws, g_ctx.ev_bin[buf], 0));
          HIPCHK(hipMemsetAsync(ar.bin_total, 0, sizeof(unsigned) * (size_t)nbins, ws));
          for (int g = MG_REC; g < MG_N; g++) {
             if (!gblocks[g]) continue;

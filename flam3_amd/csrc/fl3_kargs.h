@@ -70,6 +70,7 @@ struct IterArgs {
 #define FL3_SUPER_SHIFT 5           // tiles per super-tile = 32
 
 #define FL3_BIN_CHUNK_SAMPLES (1LL << 28)
+#define FL3_CHUNK_BUFS 2            // record buffers in rotation: the walkers run one chunk ahead of the sort
 #define FL3_BIN_ITEM_RECORDS (1 << 20)
 #define FL3_BIN_MIN_SAMPLES (1LL << 20)
 
