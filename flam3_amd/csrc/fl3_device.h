@@ -108,6 +108,10 @@ int fl3d_images_fetch(const fl3d_image_dst *d, int n);
 /* The first `bytes` of the image arena into pinned host buffer `slot` (0 or
  * 1, kept until the next download into the same slot); *host points at it. */
 int fl3d_images_download(int slot, size_t bytes, const unsigned char **host);
+/* 1 if the two jobs may share 4-byte records in one launch: job0 has a
+ * stepped palette and visibility 1 for every xform, and job has the same
+ * palette and visibilities. */
+int fl3d_jobs_share_palette(const void *job0, const void *job);
 /* Tone maps of renders 0..n-1 of fl3d_begin_many in one launch (after
  * fl3d_images_begin; render i's image at img_off[i]). */
 int fl3d_tonemap_many(const fl3d_tm_params *p, const double *const *filters, const size_t *img_off, int n);

@@ -408,6 +408,24 @@ static bool same_tables(const fl3_job_hdr *a, const fl3_job_hdr *b) {
    return true;
 }
 
+// Two jobs whose 4-byte records deposit alike: stepped palettes and
+// visibility 1 for every xform that plots, and palette entries that
+// truncate to the same integers (a 4-byte record adds its entry's integers
+// and 255 hits; interpolated palettes of one animation differ only in bits
+// the truncation drops).
+static bool same_narrow_tables(const fl3_job_hdr *a, const fl3_job_hdr *b) {
+   if (!narrow_records_ok(a, 0) || !narrow_records_ok(b, 0)) return false;
+   const double *da = (const double *)((const unsigned char *)a + a->off_dmap);
+   const double *db = (const double *)((const unsigned char *)b + b->off_dmap);
+   for (int e = 0; e < 1024; e++)
+      if ((uint32_t)da[e] != (uint32_t)db[e]) return false;
+   return true;
+}
+
+extern "C" int fl3d_jobs_share_palette(const void *job0, const void *job) {
+   return same_narrow_tables((const fl3_job_hdr *)job0, (const fl3_job_hdr *)job);
+}
+
 // Palette (as integers and floats) and visibilities of every job of a
 // launch, for the accumulation of job records; *pal8 (if not null) gets
 // the palettes as bytes R | G << 8 | B << 16 | A << 24 per entry, for
@@ -495,7 +513,8 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
                           int shift) {
    const long long ncells = (long long)g_ctx.width * g_ctx.height;
    const int nbins = (int)((ncells + (1LL << shift) - 1) >> shift);
-   const bool narrow = js.same_tables && narrow_records_ok(js.h[0], ncells);
+   bool narrow = narrow_records_ok(js.h[0], ncells);
+   for (int j = 1; narrow && j < js.n; j++) narrow = same_narrow_tables(js.h[0], js.h[j]);
    // one palette for all jobs: narrow or wide records; else the job index
    // rides in the record (step palettes only: it takes the fraction's bits)
    const int format = narrow ? 1 : (js.same_tables ? 0 : 2);
@@ -834,11 +853,19 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
       // job records in chunks of `per` rows, sorted and accumulated on the
       // bin stream while the walkers produce the next chunk
       long long rows_total = 0;
+      int jrec = -1;   // the first job that writes records
       for (int j = 0; j < njobs; j++)
          if (grp[j] >= MG_REC) {
             const long long wj = 256LL * d[j].nblocks, r = (d[j].nsamples + wj - 1) / wj;
             rows_total = r > rows_total ? r : rows_total;
+            if (jrec < 0) jrec = j;
          }
+      // 4-byte records (cell | palette index) when every job that writes
+      // records has the first one's palette and visibilities (all 1) and the
+      // joint cell space fits 24 bits; else job records
+      bool narrow = narrow_records_ok(hs[jrec], tcells);
+      for (int j = jrec + 1; narrow && j < njobs; j++)
+         if (grp[j] >= MG_REC && !same_narrow_tables(hs[jrec], hs[j])) narrow = false;
       long long per = FL3_BIN_CHUNK_SAMPLES / Wrec;
       if (per < 1) per = 1;
       if (per > rows_total) per = rows_total;
@@ -848,7 +875,7 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
       }
       const long long nblk = Wrec / 256;
       const long long nchunks = (rows_total + per - 1) / per;
-      const size_t rec_bytes = 8 * (size_t)Wrec * per;
+      const size_t rec_bytes = (narrow ? 4 : 8) * (size_t)Wrec * per;
       if (g_ctx.sorted.ensure(rec_bytes)) return 1;
       if (g_ctx.wstate.ensure(sizeof(uint32_t) * 10 * (size_t)Wrec)) return 1;
       const size_t nsup = (size_t)((nbins + (1 << FL3_SUPER_SHIFT) - 1) >> FL3_SUPER_SHIFT);
@@ -861,7 +888,7 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
       unsigned *bw = (unsigned *)g_ctx.bins.p;
       BinArgs b;
       memset(&b, 0, sizeof(b));
-      b.job = a.jobs;
+      b.job = a.jobs + d[jrec].off;   // the palette of 4-byte records
       b.buckets = g_ctx.buckets.p;
       b.sorted = g_ctx.sorted.p;
       b.nblk = (int)nblk;
@@ -879,9 +906,9 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
       b.height = 1;
       b.chunk = item_records(hs.data(), njobs);
       const uint32_t *pal8 = nullptr;
-      if (upload_job_tables(hs.data(), njobs, nxf_max, &b, &pal8)) return 1;
+      if (!narrow && upload_job_tables(hs.data(), njobs, nxf_max, &b, &pal8)) return 1;
       IterArgs ar = a;
-      ar.rec_mode = 3;   // job records
+      ar.rec_mode = narrow ? 2 : 3;   // 4-byte or job records
       ar.wstate = (uint32_t *)g_ctx.wstate.p;
       ar.bin_shift = shift;
       ar.nbins = nbins;
@@ -894,11 +921,11 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
          std::vector<const fl3_job_hdr *> rh;
          for (int j = 0; j < njobs; j++)
             if (grp[j] >= MG_REC) rh.push_back(hs[j]);
-         const bool colour = colour_records(rh.data(), (int)rh.size(), pal8, shift, lds_r);
+         const bool colour = !narrow && colour_records(rh.data(), (int)rh.size(), pal8, shift, lds_r);
          ar.pal8 = colour ? pal8 : nullptr;
          ar.pal_off = (unsigned)((lds_r + 15) & ~(size_t)15);
          if (colour) lds_r = ar.pal_off + 1024;
-         b.round = colour ? narrow_round(hs.data(), njobs) : 0;
+         b.round = narrow || colour ? narrow_round(rh.data(), (int)rh.size()) : 0;
       }
       hipStream_t ws = g_ctx.stream, bs = g_ctx.bin_stream;
       HIPCHK(hipEventRecord(g_ctx.ev_join, ws));
@@ -934,7 +961,7 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
          b.bin_total = ar.bin_total;
             b.rows = ar.rows_out;
          const int pk = prof_begin(ST_BIN, bs);
-         const int lrc = fl3k_launch_bin(&b, g_ctx.ncu, 2, bs);
+         const int lrc = fl3k_launch_bin(&b, g_ctx.ncu, narrow ? 1 : 2, bs);
          prof_end(pk, bs);
          if (lrc) {
             set_err("binned deposit launch failed");

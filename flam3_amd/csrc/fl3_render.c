@@ -37,6 +37,9 @@
 /* Histogram cells of one shared launch: its renders' joint cell space in at
  * most 8192 tiles of 4096 cells. */
 #define FL3_SHARED_MAX_CELLS (32LL * 1000 * 1000)
+/* ... and of one whose renders share a palette: 4-byte records hold a
+ * 24-bit cell index. */
+#define FL3_SHARED_NARROW_CELLS ((1LL << 24) - 2)
 
 /* One launch over the *n temporal samples collected so far (nsamples each,
  * keys[2t], keys[2t+1] for the t-th); frees their jobs and empties the set. */
@@ -762,6 +765,7 @@ struct fl3_many {
    int *kind;              /* per request: -1 failed, 0 fl3h_render, 1 small, 2 large */
    int *job0, *njob;       /* per request: its jobs in mj */
    int *live;              /* small renders in launch order */
+   int *one_pal;           /* per request: its jobs share 4-byte records */
    int nlive, bits;
    fl3d_multi_job *mj;
    int nmj;
@@ -830,6 +834,7 @@ void fl3_many_free(fl3_many *B) {
          plan_close(&B->P[i]);
       }
    free(B->mj);
+   free(B->one_pal);
    free(B->live);
    free(B->njob);
    free(B->job0);
@@ -911,6 +916,8 @@ static void prep_one(fl3_many *B, int r, const fl3h_render_opts *opts) {
    }
    B->rj[r] = m;
    B->njob[r] = Q->nts;
+   B->one_pal[r] = 1;
+   for (ts = 0; ts < Q->nts && B->one_pal[r]; ts++) B->one_pal[r] = fl3d_jobs_share_palette(m[0].job, m[ts].job);
 }
 
 typedef struct {
@@ -944,7 +951,8 @@ fl3_many *fl3_many_prepare(fl3h_request *reqs, int n, const fl3h_render_opts *op
    B->njob = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
    B->live = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
    B->rj = (fl3d_multi_job **)calloc((size_t)(n > 0 ? n : 1), sizeof(fl3d_multi_job *));
-   if (!B->P || !B->de || !B->kind || !B->job0 || !B->njob || !B->live || !B->rj) {
+   B->one_pal = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
+   if (!B->P || !B->de || !B->kind || !B->job0 || !B->njob || !B->live || !B->rj || !B->one_pal) {
       fl3_many_free(B);
       return NULL;
    }
@@ -1131,17 +1139,24 @@ static int many_group(fl3_many *B, int i0, int i1) {
 }
 
 /* The shared renders in groups whose joint cell space fits the tile-binned
- * deposit (at most FL3_SHARED_MAX_CELLS cells, 65535 jobs). */
+ * deposit (at most FL3_SHARED_MAX_CELLS cells, 65535 jobs); renders that all
+ * share one palette stay below 2^24 cells together, so that their records
+ * take 4 bytes. */
 static int many_small(fl3_many *B) {
    int i0 = 0, rc = 0;
    while (i0 < B->nlive) {
       long long cells = 0;
-      int jobs = 0, i1 = i0;
+      int jobs = 0, i1 = i0, one = 1;
+      const void *job0 = B->mj[B->job0[B->live[i0]]].job;
       while (i1 < B->nlive) {
-         const render_plan *Q = &B->P[B->live[i1]];
+         const int r = B->live[i1];
+         const render_plan *Q = &B->P[r];
          const long long c = (long long)Q->geo.width * Q->geo.height;
-         const int nj = B->njob[B->live[i1]];
-         if (i1 > i0 && (cells + c > FL3_SHARED_MAX_CELLS || jobs + nj > 65535)) break;
+         const int nj = B->njob[r];
+         const int o = one && B->one_pal[r] && fl3d_jobs_share_palette(job0, B->mj[B->job0[r]].job);
+         const long long cap = o ? FL3_SHARED_NARROW_CELLS : FL3_SHARED_MAX_CELLS;
+         if (i1 > i0 && (cells + c > cap || jobs + nj > 65535)) break;
+         one = o;
          cells += c;
          jobs += nj;
          i1++;

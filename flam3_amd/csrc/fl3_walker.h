@@ -24,8 +24,9 @@ template <int BITS, bool EXACT, int RECORDS, int TIER>
 __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
    unsigned char *lds = fl3_lds;
    GenL<float> G;
-   // independent renders: direct deposits, or job records in one cell space
-   const bool dmode = (RECORDS == FL3_REC_DIRECT || RECORDS == FL3_REC_JOB) && a.descs;
+   // independent renders: direct deposits, or job or 4-byte records in one
+   // cell space
+   const bool dmode = RECORDS != FL3_REC_WIDE && a.descs;
    const int djob = dmode ? a.blk_job[blockIdx.x] : 0;
    const JobDesc *dsc = dmode ? a.descs + djob : nullptr;
    const int nj = !dsc && a.njobs > 1 ? a.njobs : 1;
@@ -125,8 +126,10 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
                 cell == FL3_REC_NONE ? (unsigned long long)FL3_REC_NONE : (hi | cell);
             if (cell != FL3_REC_NONE) atomicAdd(&bh[cell >> a.bin_shift], 1u);
          } else if (RECORDS == FL3_REC_NARROW) {
+            // (independent renders sharing one palette: the render's first
+            // cell moves the cell into the launch's cell space)
             const unsigned long long v = make_record<float>(G, q, 0);
-            const unsigned cell = (unsigned)v;
+            const unsigned cell = (unsigned)v == FL3_REC_NONE ? FL3_REC_NONE : (unsigned)(v + cell0);
             ((unsigned *)a.rec)[(blk * a.rows_out + i) * 256 + threadIdx.x] =
                 cell == FL3_REC_NONE ? FL3_REC_NONE : (cell | ((unsigned)(v >> 32) & 0xffu) << 24);
             if (cell != FL3_REC_NONE) atomicAdd(&bh[cell >> a.bin_shift], 1u);

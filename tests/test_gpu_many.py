@@ -96,3 +96,23 @@ def test_many_small_requests_in_one_call(prod):
     for k in (0, 17, 63):
         want, _ = _alone(prod, items[k])
         assert np.array_equal(got[k][0], want)
+
+
+def test_frames_sharing_a_palette_equal_fl3h_render_alone(prod):
+    """Frames of one animation share a palette: the shared launch writes
+    4-byte records (cell | palette index) into the frames' joint cell
+    space; with a second palette in the group it writes job records.  Each
+    frame equals fl3h_render's image either way."""
+    frames = []
+    for t in range(6):
+        it = _animated(311)
+        it["time"] = 0.15 * t
+        it["seed"] = "frame-%d" % t
+        frames.append(it)
+    got = prod.render_many(frames)
+    mixed = prod.render_many(frames[:3] + [_preview(312)] + frames[3:])
+    for k, it in enumerate(frames):
+        want, ws = _alone(prod, it)
+        assert got[k][1]["num_iters"] == ws["num_iters"]
+        assert np.array_equal(got[k][0], want), it["seed"]
+        assert np.array_equal(mixed[k if k < 3 else k + 1][0], want), it["seed"]
