@@ -226,8 +226,9 @@ __device__ __forceinline__ void de_prep_body(const DEArgs &a, int bx, int by) {
 // ls = coef * k1 log(1 + c3 k2) / c3, c *= ls, accumulator += c (float
 // accumulator: rounded once per contribution).  A workgroup covers 16x16
 // cells (four 8x8 blocks, one per wavefront); it walks its halo in chunks of
-// 256 cells, compacts the sources that reach it into LDS in order, and each
-// wavefront skips the ones that miss its block.  No atomics.
+// 256 cells, compacts the sources that reach it into LDS in order, and lists
+// for each wavefront, in the same order, those that reach its block, so a
+// wavefront loops over its own sources only.  No atomics.
 #define DG_REGION 16
 #define DG_CHUNK 256
 template <typename AT, bool EXACT>
@@ -242,6 +243,8 @@ __device__ __forceinline__ void de_gather_body(const DEArgs &a, int bx, int by) 
    __shared__ int s_fi[DG_CHUNK];
    __shared__ VT s_val[DG_CHUNK][NV];
    __shared__ unsigned wave_cnt[DG_CHUNK / 64];
+   __shared__ unsigned short s_list[4][DG_CHUNK];   // per 8x8 block: indices into the chunk's sources
+   __shared__ unsigned list_cnt[4][DG_CHUNK / 64];  // [block][wavefront that listed them]
    const int W = a.width, H = a.height, R = a.half;
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
    const int rx0 = bx * DG_REGION, ry0 = by * DG_REGION;
@@ -277,10 +280,22 @@ __device__ __forceinline__ void de_gather_body(const DEArgs &a, int bx, int by) 
       const unsigned rank = __popcll(m & ((1ull << lane) - 1ull));
       if (lane == 0) wave_cnt[wave] = (unsigned)__popcll(m);
       __syncthreads();
-      unsigned off = 0, n = 0;
-      for (int w = 0; w < DG_CHUNK / 64; w++) {
-         if (w < wave) off += wave_cnt[w];
-         n += wave_cnt[w];
+      unsigned off = 0;
+      for (int w = 0; w < wave; w++) off += wave_cnt[w];
+      // the blocks (of the four 8x8 ones) this source reaches
+      unsigned reach = 0;
+      if (flag) {
+         const int r = radius[fi];
+         for (int b = 0; b < 4; b++) {
+            const int bx0 = rx0 + 8 * (b & 1), by0 = ry0 + 8 * (b >> 1);
+            reach |= (unsigned)(sx + r >= bx0 && sx - r <= bx0 + 7 && sy + r >= by0 && sy - r <= by0 + 7) << b;
+         }
+      }
+      unsigned brank[4];
+      for (int b = 0; b < 4; b++) {
+         const unsigned long long mb = __ballot((reach >> b) & 1u);
+         brank[b] = (unsigned)__popcll(mb & ((1ull << lane) - 1ull));
+         if (lane == 0) list_cnt[b][wave] = (unsigned)__popcll(mb);
       }
       if (flag) {
          const unsigned idx = off + rank;
@@ -295,12 +310,21 @@ __device__ __forceinline__ void de_gather_body(const DEArgs &a, int bx, int by) 
          }
       }
       __syncthreads();
-      for (unsigned k = 0; k < n; k++) {
+      unsigned nb = 0;
+      for (int b = 0; b < 4; b++) {
+         unsigned boff = 0;
+         for (int w = 0; w < DG_CHUNK / 64; w++)
+            if (w < wave) boff += list_cnt[b][w];
+         if ((reach >> b) & 1u) s_list[b][boff + brank[b]] = (unsigned short)(off + rank);
+      }
+      for (int w = 0; w < DG_CHUNK / 64; w++) nb += list_cnt[wave][w];
+      __syncthreads();
+      for (unsigned i = 0; i < nb; i++) {
+         const unsigned k = s_list[wave][i];
          const unsigned pos = s_pos[k];
          const int csx = (int)(pos & 0xffffu), csy = (int)(pos >> 16);
          const int cfi = s_fi[k];
          const int r = radius[cfi];
-         if (csx + r < mx0 || csx - r > mx0 + 7 || csy + r < my0 || csy - r > my0 + 7) continue;
          const int dx = abs(x - csx), dy = abs(y - csy);
          const int hi = max(dx, dy), lo = min(dx, dy);
          if (!live || hi > r) continue;
