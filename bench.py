@@ -99,6 +99,8 @@ def parse_args():
     p.add_argument("--cpu-quality", type=float, default=60.0)
     p.add_argument("--no-animation", action="store_true", help="skip the 120-frame animation measurement")
     p.add_argument("--no-preview", action="store_true", help="skip the preview-serving measurement")
+    p.add_argument("--no-still", action="store_true",
+                   help="profiling aid: run only the animation and preview legs and print their results")
     p.add_argument("--anim-frames", type=int, default=120)
     p.add_argument("--preview-requests", type=int, default=2000)
     p.add_argument("--copy-probe", action="store_true",
@@ -357,6 +359,14 @@ def main():
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.no_still:
+        legs = {}
+        if not args.no_animation:
+            legs["animation"] = animation_bench(args)
+        if not args.no_preview:
+            legs["preview"] = preview_bench(args)
+        print(json.dumps(legs), flush=True)
+        return
     r = gpu_bench(args)
     if args.copy_probe:
         import flam3_amd as F
