@@ -506,6 +506,22 @@ static int item_records(const fl3_job_hdr *const *h, int n) {
    return c < 1 ? 1 : c;
 }
 
+// Walker LDS with the point-exchange buffers of the one-xform-per-wavefront
+// mode appended (two 3 x 256 floats and two 4-word flag sets at a 16-byte
+// boundary); the mode stays off when they would not fit.
+#define FL3_SHUF_BYTES (2 * 3 * 256 * 4 + 2 * 4 * 4)
+static size_t with_shuffle(IterArgs &a, size_t lds) {
+   const size_t off = (lds + 15) & ~(size_t)15;
+   if (off + FL3_SHUF_BYTES <= g_ctx.lds_max) {
+      a.wave_xf = 1;
+      a.shuf_off = (unsigned)off;
+      return off + FL3_SHUF_BYTES;
+   }
+   a.wave_xf = 0;
+   a.shuf_off = 0;
+   return lds;
+}
+
 // Binned deposit: the walkers write records in chunks of `per` rows (the
 // walker state carries over between chunks, so the trajectories are those of
 // a single launch); each chunk is tile-sorted and accumulated.
@@ -572,6 +588,7 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    a.pal8 = colour ? pal8 : nullptr;
    a.pal_off = (unsigned)((lds_rec + 15) & ~(size_t)15);
    if (colour) lds_rec = a.pal_off + 1024;
+   lds_rec = with_shuffle(a, lds_rec);
    b.round = narrow || colour ? narrow_round(js.h, js.n) : 0;
    // chunk k: walkers on the render stream, then its sort and accumulation
    // on the bin stream, which chunk k + 2's walkers wait for before they
@@ -703,8 +720,9 @@ extern "C" int fl3d_iterate_batch(const void *const *jobs, const size_t *job_byt
          if (iterate_binned(a, W, Wj, js, p, shift)) return 1;
       } else {
          const int pk = prof_begin(ST_ITER);
+         const size_t lw = with_shuffle(a, js.lds);
          const int lrc =
-             fl3k_launch_iter_fast(&a, (unsigned)(W / 256), js.lds, g_ctx.stream, g_ctx.bits, p->exact, js.tier);
+             fl3k_launch_iter_fast(&a, (unsigned)(W / 256), lw, g_ctx.stream, g_ctx.bits, p->exact, js.tier);
          prof_end(pk);
          if (lrc) {
             set_err("k_iter_fast launch failed");
@@ -836,6 +854,7 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
    a.fuse = fuse;
    const int *blk_dev = (const int *)(a.jobs + total + dbytes);
    g_prof.samples += samples;
+   lds = with_shuffle(a, lds);
    for (int g = 0; g < MG_REC; g++) {
       if (!gblocks[g]) continue;
       a.blk_job = blk_dev + gstart[g];
@@ -927,6 +946,7 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
          if (colour) lds_r = ar.pal_off + 1024;
          b.round = narrow || colour ? narrow_round(rh.data(), (int)rh.size()) : 0;
       }
+      lds_r = with_shuffle(ar, lds_r);
       hipStream_t ws = g_ctx.stream, bs = g_ctx.bin_stream;
       HIPCHK(hipEventRecord(g_ctx.ev_join, ws));
       HIPCHK(hipStreamWaitEvent(bs, g_ctx.ev_join, 0));

@@ -56,6 +56,10 @@ struct IterArgs {
    // held in LDS at pal_off
    const uint32_t *pal8;
    unsigned pal_off;
+   // one xform per wavefront and step, points permuted between steps
+   // through 6 KB of LDS at shuf_off (genomes without xaos); 0: per lane
+   int wave_xf;
+   unsigned shuf_off;
    // walker state kept between launches of one batch (10 words per walker,
    // structure of arrays): resume = continue from it without fuse steps
    uint32_t *wstate;

@@ -14,12 +14,13 @@
 
 // Records: row i of walker (workgroup b, thread t) at (b * rows_out + i) *
 // 256 + t, so a workgroup's records are one contiguous block.
-// Production walkers.  Walker w plots nsamples/W (+1 for the first
+// Production walkers.  Walker (lane) w plots nsamples/W (+1 for the first
 // nsamples%W walkers) points after `fuse` unplotted steps from a random
-// start, exactly the per-sub-batch procedure of rect.c:393-400 with one long
-// sub-batch per walker.  In record mode each workgroup also counts its
-// records per tile in LDS and stores them per super-tile at
-// blk_hist[block][super-tile].
+// start, the per-sub-batch procedure of rect.c:393-400 with one long
+// sub-batch per walker; without xaos (a.wave_xf) the xform is drawn per
+// wavefront and the workgroup's points change lanes after every step.  In
+// record mode each workgroup also counts its records per tile in LDS and
+// stores them per super-tile at blk_hist[block][super-tile].
 template <int BITS, bool EXACT, int RECORDS, int TIER>
 __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
    unsigned char *lds = fl3_lds;
@@ -89,54 +90,118 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
    }
    uint32_t bad = 0;
    const bool use_final = G.final_index >= 0;
-   for (long long i = a.resume ? 0 : -a.fuse; i < n;) {
-      const uint32_t r = rng.next();
-      const int fn = select_xform(G, G.chaos_enable ? lastxf : 0, r & 16383u);
-      if (apply_xform<float, WalkerRng, TIER>(G, fn, p, q, rng)) {
-         consec++;
-         bad++;
-         if (consec < 5) {
-            p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
-            continue;
-         }
-         consec = 0;
+   // the record (or deposit) of plotted point v at row i
+   auto plot = [&](long long i, const float *v, int fn) {
+      if (RECORDS == FL3_REC_WIDE) {
+         const unsigned long long r = make_record<float>(G, v, fn);
+         a.rec[(blk * a.rows_out + i) * 256 + threadIdx.x] = r;
+         if ((unsigned)r != FL3_REC_NONE) atomicAdd(&bh[(unsigned)r >> a.bin_shift], 1u);
+      } else if (RECORDS == FL3_REC_JOB) {
+         // independent renders: the render's first cell moves the cell
+         // into the launch's cell space, the job is the descriptor's
+         const unsigned long long r = make_record<float>(G, v, fn);
+         const unsigned cell = (unsigned)r == FL3_REC_NONE ? FL3_REC_NONE : (unsigned)(r + cell0);
+         const unsigned long long hi = a.pal8 ? (unsigned long long)pal[(r >> 32) & 0xff] << 32
+                                              : (r & 0xffff00000000ull) | ((unsigned long long)(dsc ? djob : jb) << 48);
+         a.rec[(blk * a.rows_out + i) * 256 + threadIdx.x] =
+             cell == FL3_REC_NONE ? (unsigned long long)FL3_REC_NONE : (hi | cell);
+         if (cell != FL3_REC_NONE) atomicAdd(&bh[cell >> a.bin_shift], 1u);
+      } else if (RECORDS == FL3_REC_NARROW) {
+         // (independent renders sharing one palette: the render's first
+         // cell moves the cell into the launch's cell space)
+         const unsigned long long r = make_record<float>(G, v, 0);
+         const unsigned cell = (unsigned)r == FL3_REC_NONE ? FL3_REC_NONE : (unsigned)(r + cell0);
+         ((unsigned *)a.rec)[(blk * a.rows_out + i) * 256 + threadIdx.x] =
+             cell == FL3_REC_NONE ? FL3_REC_NONE : (cell | ((unsigned)(r >> 32) & 0xffu) << 24);
+         if (cell != FL3_REC_NONE) atomicAdd(&bh[cell >> a.bin_shift], 1u);
       } else
-         consec = 0;
-      lastxf = fn + 1;
-      p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
-      if (use_final) {
-         if (G.final_opacity == 1.0f || u01<float>(rng) < G.final_opacity) {
-            apply_xform<float, WalkerRng, TIER, true>(G, G.final_index, p, q, rng);
-            q[3] = p[3];
-         }
-      }
-      if (i >= 0) {
-         if (RECORDS == FL3_REC_WIDE) {
-            const unsigned long long v = make_record<float>(G, q, fn);
-            a.rec[(blk * a.rows_out + i) * 256 + threadIdx.x] = v;
-            if ((unsigned)v != FL3_REC_NONE) atomicAdd(&bh[(unsigned)v >> a.bin_shift], 1u);
-         } else if (RECORDS == FL3_REC_JOB) {
-            // independent renders: the render's first cell moves the cell
-            // into the launch's cell space, the job is the descriptor's
-            const unsigned long long v = make_record<float>(G, q, fn);
-            const unsigned cell = (unsigned)v == FL3_REC_NONE ? FL3_REC_NONE : (unsigned)(v + cell0);
-            const unsigned long long hi = a.pal8 ? (unsigned long long)pal[(v >> 32) & 0xff] << 32
-                                                 : (v & 0xffff00000000ull) | ((unsigned long long)(dsc ? djob : jb) << 48);
-            a.rec[(blk * a.rows_out + i) * 256 + threadIdx.x] =
-                cell == FL3_REC_NONE ? (unsigned long long)FL3_REC_NONE : (hi | cell);
-            if (cell != FL3_REC_NONE) atomicAdd(&bh[cell >> a.bin_shift], 1u);
-         } else if (RECORDS == FL3_REC_NARROW) {
-            // (independent renders sharing one palette: the render's first
-            // cell moves the cell into the launch's cell space)
-            const unsigned long long v = make_record<float>(G, q, 0);
-            const unsigned cell = (unsigned)v == FL3_REC_NONE ? FL3_REC_NONE : (unsigned)(v + cell0);
-            ((unsigned *)a.rec)[(blk * a.rows_out + i) * 256 + threadIdx.x] =
-                cell == FL3_REC_NONE ? FL3_REC_NONE : (cell | ((unsigned)(v >> 32) & 0xffu) << 24);
-            if (cell != FL3_REC_NONE) atomicAdd(&bh[cell >> a.bin_shift], 1u);
+         deposit<float, BITS, EXACT>(G, v, buckets, satp);
+   };
+   if (a.wave_xf && !G.chaos_enable) {
+      // One xform per wavefront and step (drawn from the first lane's
+      // stream), so a wavefront evaluates that xform's variations only; after
+      // every step the workgroup's 256 points move to other lanes through a
+      // random permutation (slot = m (t ^ x) + o mod 256, m odd, from a hash
+      // of the job's key, the job-local workgroup and the step), so the
+      // xforms a point meets are independent draws with the xforms'
+      // probabilities, as in the per-lane chaos game.  Lanes keep their own
+      // streams and sample counts; a lane past its count keeps moving points
+      // without plotting until the workgroup is done.
+      float *xb = (float *)(lds + a.shuf_off);
+      const uint64_t kk = dsc ? dsc->key0 : (nj > 1 ? a.keys[2 * jb] : a.key0);
+      const uint32_t bjob = dsc ? (uint32_t)(blockIdx.x - dsc->block0)
+                                : (nj > 1 ? (uint32_t)(blockIdx.x / nj) : (uint32_t)blk);
+      const uint32_t hseed = ((uint32_t)kk ^ (uint32_t)(kk >> 32) * 0x85ebca6bu) ^ bjob * 0xc2b2ae35u;
+      uint32_t st = a.resume ? (uint32_t)lastxf : 0u;
+      long long i = a.resume ? 0 : -a.fuse;
+      for (;;) {
+         const uint32_t r = rng.next();
+         const int fn =
+             __builtin_amdgcn_readfirstlane(select_xform(G, 0, __builtin_amdgcn_readfirstlane(r) & 16383u));
+         bool retry = false;
+         if (apply_xform<float, WalkerRng, TIER, true>(G, fn, p, q, rng)) {
+            if (i < n) bad++;
+            consec++;
+            if (consec < 5) retry = true;
+            else consec = 0;
          } else
-            deposit<float, BITS, EXACT>(G, q, buckets, satp);
+            consec = 0;
+         p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
+         if (!retry) {
+            if (use_final) {
+               if (G.final_opacity == 1.0f || u01<float>(rng) < G.final_opacity) {
+                  apply_xform<float, WalkerRng, TIER, true>(G, G.final_index, p, q, rng);
+                  q[3] = p[3];
+               }
+            }
+            if (i >= 0 && i < n) plot(i, q, fn);
+            i++;
+         }
+         uint32_t h = st * 0x9E3779B9u ^ hseed;
+         h ^= h >> 16; h *= 0x7feb352du; h ^= h >> 15; h *= 0x846ca68bu; h ^= h >> 16;
+         const unsigned slot = ((h | 1u) * (threadIdx.x ^ ((h >> 8) & 255u)) + (h >> 16)) & 255u;
+         float *buf = xb + (st & 1u) * 768;
+         unsigned *more4 = (unsigned *)(xb + 1536) + (st & 1u) * 4;   // per wavefront: lanes left
+         st++;
+         buf[slot] = p[0];
+         buf[256 + slot] = p[1];
+         buf[512 + slot] = p[2];
+         const unsigned long long left = __ballot(i < n);
+         if ((threadIdx.x & 63) == 0) more4[threadIdx.x >> 6] = left != 0ull;
+         // (two buffers: a wavefront writes buffer st & 1 only after every
+         // wavefront has passed the barrier behind its reads of it)
+         __syncthreads();
+         p[0] = buf[threadIdx.x];
+         p[1] = buf[256 + threadIdx.x];
+         p[2] = buf[512 + threadIdx.x];
+         if (!(more4[0] | more4[1] | more4[2] | more4[3])) break;
       }
-      i++;
+      lastxf = (int)st;   // the step count is what resumes here
+   } else {
+      for (long long i = a.resume ? 0 : -a.fuse; i < n;) {
+         const uint32_t r = rng.next();
+         const int fn = select_xform(G, G.chaos_enable ? lastxf : 0, r & 16383u);
+         if (apply_xform<float, WalkerRng, TIER>(G, fn, p, q, rng)) {
+            consec++;
+            bad++;
+            if (consec < 5) {
+               p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
+               continue;
+            }
+            consec = 0;
+         } else
+            consec = 0;
+         lastxf = fn + 1;
+         p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
+         if (use_final) {
+            if (G.final_opacity == 1.0f || u01<float>(rng) < G.final_opacity) {
+               apply_xform<float, WalkerRng, TIER, true>(G, G.final_index, p, q, rng);
+               q[3] = p[3];
+            }
+         }
+         if (i >= 0) plot(i, q, fn);
+         i++;
+      }
    }
    if (RECORDS == FL3_REC_WIDE || RECORDS == FL3_REC_JOB)
       for (long long i = n; i < a.rows_out; i++) a.rec[(blk * a.rows_out + i) * 256 + threadIdx.x] = FL3_REC_NONE;
