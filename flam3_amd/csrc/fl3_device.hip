@@ -506,13 +506,29 @@ static int item_records(const fl3_job_hdr *const *h, int n) {
    return c < 1 ? 1 : c;
 }
 
+// Genomes whose walkers draw the xform per wavefront: no xaos, and room in
+// LDS for the exchange buffers beside the largest genome image, tile counts
+// and palette bytes any launch of them can need (so that the choice does
+// not depend on what else shares a launch).
+static bool wave_ok(const fl3_job_hdr *h) {
+   return !h->chaos_enable && ((lds_bytes_for<float>(h, true) + 15) & ~(size_t)15) + 4 * (size_t)FL3_MAX_BINS + 16 +
+                                      1024 + 16 + (2 * 3 * 256 * 4 + 2 * 4 * 4) <=
+                                  g_ctx.lds_max;
+}
+
+static bool jobs_wave_ok(const fl3_job_hdr *const *h, int n) {
+   for (int j = 0; j < n; j++)
+      if (!wave_ok(h[j])) return false;
+   return true;
+}
+
 // Walker LDS with the point-exchange buffers of the one-xform-per-wavefront
 // mode appended (two 3 x 256 floats and two 4-word flag sets at a 16-byte
 // boundary); the mode stays off when they would not fit.
 #define FL3_SHUF_BYTES (2 * 3 * 256 * 4 + 2 * 4 * 4)
-static size_t with_shuffle(IterArgs &a, size_t lds) {
+static size_t with_shuffle(IterArgs &a, size_t lds, bool want) {
    const size_t off = (lds + 15) & ~(size_t)15;
-   if (off + FL3_SHUF_BYTES <= g_ctx.lds_max) {
+   if (want && off + FL3_SHUF_BYTES <= g_ctx.lds_max) {
       a.wave_xf = 1;
       a.shuf_off = (unsigned)off;
       return off + FL3_SHUF_BYTES;
@@ -588,7 +604,7 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    a.pal8 = colour ? pal8 : nullptr;
    a.pal_off = (unsigned)((lds_rec + 15) & ~(size_t)15);
    if (colour) lds_rec = a.pal_off + 1024;
-   lds_rec = with_shuffle(a, lds_rec);
+   lds_rec = with_shuffle(a, lds_rec, jobs_wave_ok(js.h, js.n));
    b.round = narrow || colour ? narrow_round(js.h, js.n) : 0;
    // chunk k: walkers on the render stream, then its sort and accumulation
    // on the bin stream, which chunk k + 2's walkers wait for before they
@@ -720,7 +736,7 @@ extern "C" int fl3d_iterate_batch(const void *const *jobs, const size_t *job_byt
          if (iterate_binned(a, W, Wj, js, p, shift)) return 1;
       } else {
          const int pk = prof_begin(ST_ITER);
-         const size_t lw = with_shuffle(a, js.lds);
+         const size_t lw = with_shuffle(a, js.lds, jobs_wave_ok(js.h, js.n));
          const int lrc =
              fl3k_launch_iter_fast(&a, (unsigned)(W / 256), lw, g_ctx.stream, g_ctx.bits, p->exact, js.tier);
          prof_end(pk);
@@ -752,7 +768,9 @@ extern "C" long long fl3d_job_walkers(int64_t nsamples, int njobs, int requested
 // jobs with a linear palette, or every job when the renders' cells exceed
 // the tile limits, deposit with per-sample atomics (saturating for
 // jobs[j].exact).  Each tier, and each deposit kind, is a launch of its own.
-enum { MG_DIRECT = 0, MG_EXACT = FL3_TIER_FULL + 1, MG_REC = 2 * (FL3_TIER_FULL + 1), MG_N = 3 * (FL3_TIER_FULL + 1) };
+// Launch groups: deposit kind (direct, direct saturating, records) x walker
+// mode (one xform per wavefront, per lane) x kernel tier.
+enum { MG_T = FL3_TIER_FULL + 1, MG_DIRECT = 0, MG_EXACT = 2 * MG_T, MG_REC = 4 * MG_T, MG_N = 6 * MG_T };
 
 extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fuse, int exact, int nrenders,
                                   double *badvals, int *saturated) {
@@ -781,6 +799,11 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
       for (int k = j; k < e; k++) tier[k] = t;
       j = e;
    }
+   // per render: its walkers draw the xform per wavefront (as fl3h_render
+   // would decide for it alone)
+   std::vector<char> rwave((size_t)nrenders, 1);
+   for (int j = 0; j < njobs; j++)
+      if (jobs[j].render >= 0 && jobs[j].render < nrenders && !wave_ok(hs[j])) rwave[(size_t)jobs[j].render] = 0;
    // the joint cell space of the renders, in tiles
    long long tcells = 0;
    for (size_t r = 0; r < g_ctx.rcell0.size(); r++) tcells += (long long)g_ctx.rwidth[r] * g_ctx.rheight[r];
@@ -795,7 +818,8 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
          return 1;
       }
       const bool rec = use_rec && hs[j]->palette_mode == 0;
-      grp[j] = rec ? MG_REC + tier[j] : (exact || jobs[j].exact ? MG_EXACT : MG_DIRECT) + tier[j];
+      const int sub = tier[j] + (rwave[jobs[j].render] ? 0 : MG_T);
+      grp[j] = rec ? MG_REC + sub : (exact || jobs[j].exact ? MG_EXACT : MG_DIRECT) + sub;
       const size_t l = lds_bytes_for<float>(hs[j], !rec);
       if (rec)
          lds_rec = l > lds_rec ? l : lds_rec;
@@ -819,7 +843,7 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
       for (int j = 0; j < njobs; j++)
          if (grp[j] >= MG_REC) {
             gblocks[grp[j]] -= d[j].nblocks;
-            grp[j] = (exact || jobs[j].exact ? MG_EXACT : MG_DIRECT) + tier[j];
+            grp[j] = (exact || jobs[j].exact ? MG_EXACT : MG_DIRECT) + tier[j] + (rwave[jobs[j].render] ? 0 : MG_T);
             d[j].block0 = (int)gblocks[grp[j]];
             gblocks[grp[j]] += d[j].nblocks;
             const size_t l = lds_bytes_for<float>(hs[j]);
@@ -854,13 +878,16 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
    a.fuse = fuse;
    const int *blk_dev = (const int *)(a.jobs + total + dbytes);
    g_prof.samples += samples;
-   lds = with_shuffle(a, lds);
+   const size_t lds_lane = lds;
+   const size_t lds_wave = with_shuffle(a, lds, true);
+   const bool wave_fits = a.wave_xf != 0;
    for (int g = 0; g < MG_REC; g++) {
       if (!gblocks[g]) continue;
       a.blk_job = blk_dev + gstart[g];
+      a.wave_xf = wave_fits && (g / MG_T) % 2 == 0;
       const int pk = prof_begin(ST_ITER);
-      const int lrc = fl3k_launch_iter_fast(&a, (unsigned)gblocks[g], lds, g_ctx.stream, g_ctx.bits, g >= MG_EXACT,
-                                            g % (FL3_TIER_FULL + 1));
+      const int lrc = fl3k_launch_iter_fast(&a, (unsigned)gblocks[g], a.wave_xf ? lds_wave : lds_lane, g_ctx.stream,
+                                            g_ctx.bits, g >= MG_EXACT, g % MG_T);
       prof_end(pk);
       if (lrc) {
          set_err("k_iter_fast launch failed");
@@ -946,7 +973,9 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
          if (colour) lds_r = ar.pal_off + 1024;
          b.round = narrow || colour ? narrow_round(rh.data(), (int)rh.size()) : 0;
       }
-      lds_r = with_shuffle(ar, lds_r);
+      const size_t lds_r_lane = lds_r;
+      lds_r = with_shuffle(ar, lds_r, true);
+      const bool wave_fits_r = ar.wave_xf != 0;
       hipStream_t ws = g_ctx.stream, bs = g_ctx.bin_stream;
       HIPCHK(hipEventRecord(g_ctx.ev_join, ws));
       HIPCHK(hipStreamWaitEvent(bs, g_ctx.ev_join, 0));
@@ -964,8 +993,10 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
             if (!gblocks[g]) continue;
             ar.blk_job = blk_dev + gstart[g];
             ar.blk0 = (int)(gstart[g] - gstart[MG_REC]);
+            ar.wave_xf = wave_fits_r && ((g - MG_REC) / MG_T) % 2 == 0;
             const int pk = prof_begin(ST_ITER, ws);
-            const int lrc = fl3k_launch_iter_fast(&ar, (unsigned)gblocks[g], lds_r, ws, g_ctx.bits, 1, g - MG_REC);
+            const int lrc = fl3k_launch_iter_fast(&ar, (unsigned)gblocks[g], ar.wave_xf ? lds_r : lds_r_lane, ws,
+                                                  g_ctx.bits, 1, (g - MG_REC) % MG_T);
             prof_end(pk, ws);
             if (lrc) {
                set_err("k_iter_fast launch failed");

@@ -4,23 +4,36 @@
 #include "fl3_launch.h"
 
 
+template <bool WAVE>
+static void launch_full(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int bits, int exact) {
+   if (a->rec_mode == FL3_REC_JOB)
+      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_JOB, FL3_TIER_FULL, WAVE>), dim3(grid), dim3(256), lds, s, *a);
+   else if (a->rec_mode == FL3_REC_NARROW)
+      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_NARROW, FL3_TIER_FULL, WAVE>), dim3(grid), dim3(256), lds, s,
+                         *a);
+   else if (a->rec_mode == FL3_REC_WIDE)
+      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_WIDE, FL3_TIER_FULL, WAVE>), dim3(grid), dim3(256), lds, s, *a);
+   else if (bits == 64)
+      hipLaunchKernelGGL((k_iter_fast<64, true, FL3_REC_DIRECT, FL3_TIER_FULL, WAVE>), dim3(grid), dim3(256), lds, s,
+                         *a);
+   else if (exact)
+      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_DIRECT, FL3_TIER_FULL, WAVE>), dim3(grid), dim3(256), lds, s,
+                         *a);
+   else
+      hipLaunchKernelGGL((k_iter_fast<33, false, FL3_REC_DIRECT, FL3_TIER_FULL, WAVE>), dim3(grid), dim3(256), lds, s,
+                         *a);
+}
+
+// a->wave_xf selects the one-xform-per-wavefront walkers (genomes without
+// xaos, exchange buffers at a->shuf_off).
 extern "C" int fl3k_launch_iter_fast(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int bits,
                                      int exact, int tier) {
    if (tier >= 0 && tier < FL3_TIER_FULL && (a->rec_mode || bits != 64))
       return fl3k_launch_iter_tier(a, grid, lds, s, tier, exact);
-   if (a->rec_mode == FL3_REC_JOB)
-      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_JOB, FL3_TIER_FULL>), dim3(grid), dim3(256), lds, s, *a);
-   else if (a->rec_mode == FL3_REC_NARROW)
-      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_NARROW, FL3_TIER_FULL>), dim3(grid), dim3(256), lds, s, *a);
-   else if (a->rec_mode == FL3_REC_WIDE)
-      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_WIDE, FL3_TIER_FULL>), dim3(grid), dim3(256), lds, s, *a);
-   else if (bits == 64)
-      hipLaunchKernelGGL((k_iter_fast<64, true, FL3_REC_DIRECT, FL3_TIER_FULL>), dim3(grid), dim3(256), lds, s, *a);
-   else if (exact)
-      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_DIRECT, FL3_TIER_FULL>), dim3(grid), dim3(256), lds, s, *a);
+   if (a->wave_xf)
+      launch_full<true>(a, grid, lds, s, bits, exact);
    else
-      hipLaunchKernelGGL((k_iter_fast<33, false, FL3_REC_DIRECT, FL3_TIER_FULL>), dim3(grid), dim3(256), lds, s,
-                         *a);
+      launch_full<false>(a, grid, lds, s, bits, exact);
    return hipGetLastError() != hipSuccess;
 }
 

@@ -5,18 +5,26 @@
 #include "fl3_walker.h"
 #include "fl3_launch.h"
 
+template <int TIER, bool WAVE>
+static void launch_tier_mode(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int exact) {
+   if (a->rec_mode == FL3_REC_JOB)
+      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_JOB, TIER, WAVE>), dim3(grid), dim3(256), lds, s, *a);
+   else if (a->rec_mode == FL3_REC_NARROW)
+      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_NARROW, TIER, WAVE>), dim3(grid), dim3(256), lds, s, *a);
+   else if (a->rec_mode == FL3_REC_WIDE)
+      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_WIDE, TIER, WAVE>), dim3(grid), dim3(256), lds, s, *a);
+   else if (exact)
+      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_DIRECT, TIER, WAVE>), dim3(grid), dim3(256), lds, s, *a);
+   else
+      hipLaunchKernelGGL((k_iter_fast<33, false, FL3_REC_DIRECT, TIER, WAVE>), dim3(grid), dim3(256), lds, s, *a);
+}
+
 template <int TIER>
 static void launch_tier(const IterArgs *a, unsigned grid, size_t lds, hipStream_t s, int exact) {
-   if (a->rec_mode == FL3_REC_JOB)
-      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_JOB, TIER>), dim3(grid), dim3(256), lds, s, *a);
-   else if (a->rec_mode == FL3_REC_NARROW)
-      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_NARROW, TIER>), dim3(grid), dim3(256), lds, s, *a);
-   else if (a->rec_mode == FL3_REC_WIDE)
-      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_WIDE, TIER>), dim3(grid), dim3(256), lds, s, *a);
-   else if (exact)
-      hipLaunchKernelGGL((k_iter_fast<33, true, FL3_REC_DIRECT, TIER>), dim3(grid), dim3(256), lds, s, *a);
+   if (a->wave_xf)
+      launch_tier_mode<TIER, true>(a, grid, lds, s, exact);
    else
-      hipLaunchKernelGGL((k_iter_fast<33, false, FL3_REC_DIRECT, TIER>), dim3(grid), dim3(256), lds, s, *a);
+      launch_tier_mode<TIER, false>(a, grid, lds, s, exact);
 }
 
 // Records (binned deposit, either format) or direct bits-33 deposits (fast,

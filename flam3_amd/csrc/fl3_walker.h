@@ -17,11 +17,11 @@
 // Production walkers.  Walker (lane) w plots nsamples/W (+1 for the first
 // nsamples%W walkers) points after `fuse` unplotted steps from a random
 // start, the per-sub-batch procedure of rect.c:393-400 with one long
-// sub-batch per walker; without xaos (a.wave_xf) the xform is drawn per
-// wavefront and the workgroup's points change lanes after every step.  In
+// sub-batch per walker; with WAVE (genomes without xaos) the xform is drawn
+// per wavefront and the workgroup's points change lanes after every step.  In
 // record mode each workgroup also counts its records per tile in LDS and
 // stores them per super-tile at blk_hist[block][super-tile].
-template <int BITS, bool EXACT, int RECORDS, int TIER>
+template <int BITS, bool EXACT, int RECORDS, int TIER, bool WAVE>
 __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
    unsigned char *lds = fl3_lds;
    GenL<float> G;
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
       } else
          deposit<float, BITS, EXACT>(G, v, buckets, satp);
    };
-   if (a.wave_xf && !G.chaos_enable) {
+   if constexpr (WAVE) {
       // One xform per wavefront and step (drawn from the first lane's
       // stream), so a wavefront evaluates that xform's variations only; after
       // every step the workgroup's 256 points move to other lanes through a
