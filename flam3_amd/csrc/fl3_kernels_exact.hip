@@ -239,8 +239,7 @@ __device__ __forceinline__ void de_gather_body(const DEArgs &a, int bx, int by) 
    typedef typename std::conditional<EXACT, double, AT>::type VT;
    constexpr int NV = EXACT ? 5 : 4;
    __shared__ int radius[1024];
-   __shared__ unsigned s_pos[DG_CHUNK];
-   __shared__ int s_fi[DG_CHUNK];
+   __shared__ int4 s_src[DG_CHUNK];   // per source: x, y, first coefficient of its kernel, radius
    __shared__ VT s_val[DG_CHUNK][NV];
    __shared__ unsigned wave_cnt[DG_CHUNK / 64];
    __shared__ unsigned short s_list[4][DG_CHUNK];   // per 8x8 block: indices into the chunk's sources
@@ -300,8 +299,7 @@ __device__ __forceinline__ void de_gather_body(const DEArgs &a, int bx, int by) 
       if (flag) {
          const unsigned idx = off + rank;
          const long cell = sx + (long)sy * W;
-         s_pos[idx] = (unsigned)sx | ((unsigned)sy << 16);
-         s_fi[idx] = fi;
+         s_src[idx] = make_int4(sx, sy, fi * a.kernel_size, radius[fi]);
          if constexpr (EXACT) {
             for (int c = 0; c < 4; c++) s_val[idx][c] = bucket_value(a.buckets, cell * 5 + c, (AT)0);
             s_val[idx][4] = a.srcl[cell];
@@ -319,26 +317,28 @@ __device__ __forceinline__ void de_gather_body(const DEArgs &a, int bx, int by) 
       }
       for (int w = 0; w < DG_CHUNK / 64; w++) nb += list_cnt[wave][w];
       __syncthreads();
+      // (the coefficient is loaded for every listed source, from index 0
+      // when the tap misses this cell, so that the loads of consecutive
+      // sources overlap; a missed tap adds nothing, as before)
+#pragma unroll 4
       for (unsigned i = 0; i < nb; i++) {
          const unsigned k = s_list[wave][i];
-         const unsigned pos = s_pos[k];
-         const int csx = (int)(pos & 0xffffu), csy = (int)(pos >> 16);
-         const int cfi = s_fi[k];
-         const int r = radius[cfi];
-         const int dx = abs(x - csx), dy = abs(y - csy);
+         const int4 src = s_src[k];
+         const int dx = abs(x - src.x), dy = abs(y - src.y);
          const int hi = max(dx, dy), lo = min(dx, dy);
-         if (!live || hi > r) continue;
-         const long ci = (long)cfi * a.kernel_size + hi * (hi + 1) / 2 + lo;
+         const bool tap = live && hi <= src.w;
+         const long ci = tap ? (long)src.z + hi * (hi + 1) / 2 + lo : 0;
          if constexpr (EXACT) {
             const double coef = a.coefs[ci];
-            if (coef == 0.0) continue;
-            const double c3 = s_val[k][3];
-            const double ls = coef * s_val[k][4] / c3;
-            for (int c = 0; c < 4; c++) acc[c] = (AT)((double)acc[c] + s_val[k][c] * ls);
+            if (tap && coef != 0.0) {
+               const double c3 = s_val[k][3];
+               const double ls = coef * s_val[k][4] / c3;
+               for (int c = 0; c < 4; c++) acc[c] = (AT)((double)acc[c] + s_val[k][c] * ls);
+            }
          } else {
             const AT coef = ((const AT *)a.coefs_t)[ci];
-            if (coef == (AT)0) continue;
-            for (int c = 0; c < 4; c++) acc[c] += coef * s_val[k][c];
+            if (tap && coef != (AT)0)
+               for (int c = 0; c < 4; c++) acc[c] += coef * s_val[k][c];
          }
       }
       __syncthreads();
