@@ -63,9 +63,6 @@ extern "C" int fl3k_launch_apply_fp32(const unsigned char *job, int xi, const do
 #ifndef FL3_ACC_UNROLL
 #define FL3_ACC_UNROLL 16
 #endif
-#ifndef FL3_SCATTER_UNROLL
-#define FL3_SCATTER_UNROLL 16
-#endif
 
 // One workgroup: exclusive scans of the tile counts and of the work items
 // (ceil(count / chunk) per tile).
@@ -137,36 +134,66 @@ __global__ __launch_bounds__(256) void k_super_off(BinArgs a) {
    }
 }
 
+// Exclusive prefix sum over a 256-thread workgroup (v from each thread);
+// *total gets the sum.  wsum: 4 words of LDS.  Ends with a barrier.
+__device__ __forceinline__ unsigned wg_excl_scan256(unsigned v, unsigned *wsum, unsigned *total) {
+   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+   unsigned x = v;
+   for (int o = 1; o < 64; o <<= 1) {
+      const unsigned y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+   }
+   if (lane == 63) wsum[w] = x;
+   __syncthreads();
+   unsigned pre = 0;
+   for (int k = 0; k < w; k++) pre += wsum[k];
+   *total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+   __syncthreads();
+   return pre + x - v;
+}
+
 // Pass 1: workgroup k moves the records of walkers [256k, 256k + 256) (one
-// column block of the record matrix) to its range of each super-tile.  With
-// at most 256 super-tiles per workgroup the partly written lines stay in L2
-// until they fill.
+// column block of the record matrix) to its range of each super-tile.  It
+// takes the block in batches of rows, orders each batch by super-tile in LDS
+// (counting sort) and writes it out from there, so consecutive lanes store
+// consecutive slots of one super-tile's range (a few cache lines per store
+// instead of one per lane).
 template <typename RT>
 __global__ __launch_bounds__(256) void k_bin_part1(BinArgs a) {
-   __shared__ unsigned base[256], cnt[256];
+   constexpr int P = sizeof(RT) == 4 ? 16 : 8;   // rows per batch
+   __shared__ unsigned gnext[256];   // this workgroup's next slot in each super-tile
+   __shared__ unsigned lcnt[256], loff[256], wsum[4];
+   __shared__ RT stage[256 * P];
    const int ss = a.shift + FL3_SUPER_SHIFT;
-   for (int s = threadIdx.x; s < a.nsuper; s += blockDim.x) {
-      base[s] = a.soff[(size_t)blockIdx.x * a.nsuper + s];
-      cnt[s] = 0;
-   }
-   __syncthreads();
+   const int t = threadIdx.x;
+   gnext[t] = t < a.nsuper ? a.soff[(size_t)blockIdx.x * a.nsuper + t] : 0u;
    const RT *rec = (const RT *)a.rec;
    RT *out = (RT *)a.sorted;
-   long long i = 0;
-   for (; i + FL3_SCATTER_UNROLL <= a.rows; i += FL3_SCATTER_UNROLL) {
-      RT v[FL3_SCATTER_UNROLL];
-      for (int k = 0; k < FL3_SCATTER_UNROLL; k++) v[k] = rec[(blockIdx.x * a.rows + i + k) * 256 + threadIdx.x];
-      for (int k = 0; k < FL3_SCATTER_UNROLL; k++) {
-         if (rec_none(v[k])) continue;
-         const unsigned s = rec_cell(v[k]) >> ss;
-         out[base[s] + atomicAdd(&cnt[s], 1u)] = v[k];
+   for (long long i0 = 0; i0 < a.rows; i0 += P) {
+      lcnt[t] = 0;
+      __syncthreads();
+      RT v[P];
+      unsigned rk[P];
+      for (int k = 0; k < P; k++) {
+         const long long row = i0 + k;
+         v[k] = row < a.rows ? rec[(blockIdx.x * a.rows + row) * 256 + t] : (RT)FL3_REC_NONE;
       }
-   }
-   for (; i < a.rows; i++) {
-      const RT v = rec[(blockIdx.x * a.rows + i) * 256 + threadIdx.x];
-      if (rec_none(v)) continue;
-      const unsigned s = rec_cell(v) >> ss;
-      out[base[s] + atomicAdd(&cnt[s], 1u)] = v;
+      for (int k = 0; k < P; k++)
+         if (!rec_none(v[k])) rk[k] = atomicAdd(&lcnt[rec_cell(v[k]) >> ss], 1u);
+      __syncthreads();
+      unsigned tot;
+      loff[t] = wg_excl_scan256(lcnt[t], wsum, &tot);
+      __syncthreads();
+      for (int k = 0; k < P; k++)
+         if (!rec_none(v[k])) stage[loff[rec_cell(v[k]) >> ss] + rk[k]] = v[k];
+      __syncthreads();
+      for (unsigned j = t; j < tot; j += 256) {
+         const RT w = stage[j];
+         const unsigned sp = rec_cell(w) >> ss;
+         out[gnext[sp] + (j - loff[sp])] = w;
+      }
+      __syncthreads();
+      gnext[t] += lcnt[t];
    }
 }
 
