@@ -200,12 +200,17 @@ __global__ __launch_bounds__(256) void k_bin_part1(BinArgs a) {
 // Pass 2: workgroup c moves the super-tile-grouped records
 // [c * FL3_PART2_RECORDS, ...) to their tiles: ranks within the workgroup
 // from LDS counters, each tile's slots reserved with one global atomic.  The
-// order within a tile is arbitrary (the accumulation adds integers).
+// order within a tile is arbitrary (the accumulation adds integers).  When
+// the records span at most 256 tiles (8 super-tiles, the usual case) they
+// are ordered by tile in LDS first and written out from there, consecutive
+// lanes to consecutive slots.
 #define FL3_PART2_PER 32
 #define FL3_PART2_RECORDS (256 * FL3_PART2_PER)
 template <typename RT>
 __global__ __launch_bounds__(256) void k_bin_part2(BinArgs a) {
    unsigned *cnt = (unsigned *)fl3_lds;   // [nbins]
+   RT *stage = (RT *)(fl3_lds + ((sizeof(unsigned) * (size_t)a.nbins + 15) & ~(size_t)15));   // [FL3_PART2_RECORDS]
+   __shared__ unsigned lofs[256], gofs[256], wsum[4];
    const long long total = a.bin_start[a.nbins];
    const long long r0 = (long long)blockIdx.x * FL3_PART2_RECORDS;
    if (r0 >= total) return;
@@ -222,6 +227,27 @@ __global__ __launch_bounds__(256) void k_bin_part2(BinArgs a) {
    for (int k = 0; k < FL3_PART2_PER; k++)
       if (!rec_none(v[k])) rank[k] = atomicAdd(&cnt[rec_cell(v[k]) >> a.shift], 1u);
    __syncthreads();
+   const long long rl = (total < r0 + FL3_PART2_RECORDS ? total : r0 + FL3_PART2_RECORDS) - 1;
+   const int t_lo = (int)((rec_cell(in[r0]) >> a.shift) & ~((1u << FL3_SUPER_SHIFT) - 1u));
+   const int t_hi = (int)((rec_cell(in[rl]) >> a.shift) | ((1u << FL3_SUPER_SHIFT) - 1u));
+   if (t_hi - t_lo < 256) {
+      const int b = t_lo + (int)threadIdx.x;
+      const unsigned c = b <= t_hi && b < a.nbins ? cnt[b] : 0u;
+      unsigned tot;
+      const unsigned l = wg_excl_scan256(c, wsum, &tot);
+      lofs[threadIdx.x] = l;
+      gofs[threadIdx.x] = c ? atomicAdd(&a.cursor[b], c) : 0u;
+      __syncthreads();
+      for (int k = 0; k < FL3_PART2_PER; k++)
+         if (!rec_none(v[k])) stage[lofs[(rec_cell(v[k]) >> a.shift) - t_lo] + rank[k]] = v[k];
+      __syncthreads();
+      for (unsigned j = threadIdx.x; j < tot; j += 256) {
+         const RT w = stage[j];
+         const unsigned bb = (rec_cell(w) >> a.shift) - (unsigned)t_lo;
+         out[gofs[bb] + (j - lofs[bb])] = w;
+      }
+      return;
+   }
    for (int b = threadIdx.x; b < a.nbins; b += blockDim.x)
       if (cnt[b]) cnt[b] = atomicAdd(&a.cursor[b], cnt[b]);
    __syncthreads();
@@ -465,7 +491,15 @@ extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_
    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, *a);
    hipLaunchKernelGGL(k_super_off, dim3(a->nsuper), dim3(256), 0, s, *a);
    const unsigned g2 = (unsigned)(((long long)a->walkers * a->rows + FL3_PART2_RECORDS - 1) / FL3_PART2_RECORDS);
-   const size_t lds2 = sizeof(unsigned) * a->nbins;
+   // tile counters, then the staging area of the 4- or 8-byte records
+   const size_t lds2c = (sizeof(unsigned) * (size_t)a->nbins + 15) & ~(size_t)15;
+   const size_t lds2_4 = lds2c + 4 * (size_t)FL3_PART2_RECORDS, lds2_8 = lds2c + 8 * (size_t)FL3_PART2_RECORDS;
+   if (lds2_4 > 65536)
+      (void)hipFuncSetAttribute((const void *)k_bin_part2<unsigned>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds2_4);
+   if (lds2_8 > 65536)
+      (void)hipFuncSetAttribute((const void *)k_bin_part2<unsigned long long>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2_8);
    const size_t lds = fl3k_bin_lds_bytes(a->shift);
    const unsigned gacc = (unsigned)ncu * (lds > 65536 ? 1 : 3);
    const int big = lds > 65536;
@@ -474,10 +508,10 @@ extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_
       const bool job = format == 2;
       if (job) {
          hipLaunchKernelGGL(k_bin_part1<unsigned long long>, dim3(a->nblk), dim3(256), 0, s, *a);
-         if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned long long>, dim3(g2), dim3(256), lds2, s, *a);
+         if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned long long>, dim3(g2), dim3(256), lds2_8, s, *a);
       } else {
          hipLaunchKernelGGL(k_bin_part1<unsigned>, dim3(a->nblk), dim3(256), 0, s, *a);
-         if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned>, dim3(g2), dim3(256), lds2, s, *a);
+         if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned>, dim3(g2), dim3(256), lds2_4, s, *a);
       }
       const size_t ln = fl3k_bin_narrow_lds_bytes(a->shift);
       const unsigned gn = (unsigned)ncu * (ln > 65536 ? 2 : 3);
@@ -494,14 +528,14 @@ extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_
          hipLaunchKernelGGL((k_bin_accum_narrow<2, false>), dim3(gn), dim3(FL3_ACC_THREADS), ln, s, *a);
    } else if (format == 1) {
       hipLaunchKernelGGL(k_bin_part1<unsigned>, dim3(a->nblk), dim3(256), 0, s, *a);
-      if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned>, dim3(g2), dim3(256), lds2, s, *a);
+      if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned>, dim3(g2), dim3(256), lds2_4, s, *a);
       if (big)
          (void)hipFuncSetAttribute((const void *)k_bin_accum<unsigned, false>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       hipLaunchKernelGGL((k_bin_accum<unsigned, false>), dim3(gacc), dim3(FL3_ACC_THREADS), lds, s, *a);
    } else {
       hipLaunchKernelGGL(k_bin_part1<unsigned long long>, dim3(a->nblk), dim3(256), 0, s, *a);
-      if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned long long>, dim3(g2), dim3(256), lds2, s, *a);
+      if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned long long>, dim3(g2), dim3(256), lds2_8, s, *a);
       if (format == 2) {
          if (big)
             (void)hipFuncSetAttribute((const void *)k_bin_accum<unsigned long long, true>,
