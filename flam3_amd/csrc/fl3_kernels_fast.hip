@@ -153,13 +153,48 @@ __device__ __forceinline__ unsigned wg_excl_scan256(unsigned v, unsigned *wsum, 
 }
 
 // Pass 1: workgroup k moves the records of walkers [256k, 256k + 256) (one
-// column block of the record matrix) to its range of each super-tile.  It
-// takes the block in batches of rows, orders each batch by super-tile in LDS
-// (counting sort) and writes it out from there, so consecutive lanes store
-// consecutive slots of one super-tile's range (a few cache lines per store
-// instead of one per lane).
+// column block of the record matrix) to its range of each super-tile.  With
+// at most 256 super-tiles per workgroup the partly written lines stay in L2
+// until they fill.
+#define FL3_SCATTER_UNROLL 16
 template <typename RT>
 __global__ __launch_bounds__(256) void k_bin_part1(BinArgs a) {
+   __shared__ unsigned base[256], cnt[256];
+   const int ss = a.shift + FL3_SUPER_SHIFT;
+   for (int s = threadIdx.x; s < a.nsuper; s += blockDim.x) {
+      base[s] = a.soff[(size_t)blockIdx.x * a.nsuper + s];
+      cnt[s] = 0;
+   }
+   __syncthreads();
+   const RT *rec = (const RT *)a.rec;
+   RT *out = (RT *)a.sorted;
+   long long i = 0;
+   for (; i + FL3_SCATTER_UNROLL <= a.rows; i += FL3_SCATTER_UNROLL) {
+      RT v[FL3_SCATTER_UNROLL];
+      for (int k = 0; k < FL3_SCATTER_UNROLL; k++) v[k] = rec[(blockIdx.x * a.rows + i + k) * 256 + threadIdx.x];
+      for (int k = 0; k < FL3_SCATTER_UNROLL; k++) {
+         if (rec_none(v[k])) continue;
+         const unsigned s = rec_cell(v[k]) >> ss;
+         out[base[s] + atomicAdd(&cnt[s], 1u)] = v[k];
+      }
+   }
+   for (; i < a.rows; i++) {
+      const RT v = rec[(blockIdx.x * a.rows + i) * 256 + threadIdx.x];
+      if (rec_none(v)) continue;
+      const unsigned s = rec_cell(v) >> ss;
+      out[base[s] + atomicAdd(&cnt[s], 1u)] = v;
+   }
+}
+
+// The same for 4-byte records of tall blocks (many rows per chunk): the
+// block is taken in batches of 16 rows, each batch ordered by super-tile in
+// LDS (counting sort) and written out from there, so consecutive lanes store
+// consecutive slots of one super-tile's range (a few cache lines per store
+// instead of one per lane).  Faster from a few dozen rows on; for short
+// blocks the batches' scans and barriers cost more than they save.
+#define FL3_PART1_STAGED_ROWS 64
+template <typename RT>
+__global__ __launch_bounds__(256) void k_bin_part1_staged(BinArgs a) {
    constexpr int P = sizeof(RT) == 4 ? 16 : 8;   // rows per batch
    __shared__ unsigned gnext[256];   // this workgroup's next slot in each super-tile
    __shared__ unsigned lcnt[256], loff[256], wsum[4];
@@ -201,9 +236,10 @@ __global__ __launch_bounds__(256) void k_bin_part1(BinArgs a) {
 // [c * FL3_PART2_RECORDS, ...) to their tiles: ranks within the workgroup
 // from LDS counters, each tile's slots reserved with one global atomic.  The
 // order within a tile is arbitrary (the accumulation adds integers).  When
-// the records span at most 256 tiles (8 super-tiles, the usual case) they
+// 4-byte records span at most 256 tiles (8 super-tiles, the usual case) they
 // are ordered by tile in LDS first and written out from there, consecutive
-// lanes to consecutive slots.
+// lanes to consecutive slots (8-byte records would need twice the LDS and
+// halve the workgroups a CU holds, which costs more than it saves).
 #define FL3_PART2_PER 32
 #define FL3_PART2_RECORDS (256 * FL3_PART2_PER)
 template <typename RT>
@@ -230,7 +266,7 @@ __global__ __launch_bounds__(256) void k_bin_part2(BinArgs a) {
    const long long rl = (total < r0 + FL3_PART2_RECORDS ? total : r0 + FL3_PART2_RECORDS) - 1;
    const int t_lo = (int)((rec_cell(in[r0]) >> a.shift) & ~((1u << FL3_SUPER_SHIFT) - 1u));
    const int t_hi = (int)((rec_cell(in[rl]) >> a.shift) | ((1u << FL3_SUPER_SHIFT) - 1u));
-   if (t_hi - t_lo < 256) {
+   if (sizeof(RT) == 4 && t_hi - t_lo < 256) {
       const int b = t_lo + (int)threadIdx.x;
       const unsigned c = b <= t_hi && b < a.nbins ? cnt[b] : 0u;
       unsigned tot;
@@ -493,13 +529,10 @@ extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_
    const unsigned g2 = (unsigned)(((long long)a->walkers * a->rows + FL3_PART2_RECORDS - 1) / FL3_PART2_RECORDS);
    // tile counters, then the staging area of the 4- or 8-byte records
    const size_t lds2c = (sizeof(unsigned) * (size_t)a->nbins + 15) & ~(size_t)15;
-   const size_t lds2_4 = lds2c + 4 * (size_t)FL3_PART2_RECORDS, lds2_8 = lds2c + 8 * (size_t)FL3_PART2_RECORDS;
+   const size_t lds2_4 = lds2c + 4 * (size_t)FL3_PART2_RECORDS, lds2_8 = lds2c;
    if (lds2_4 > 65536)
       (void)hipFuncSetAttribute((const void *)k_bin_part2<unsigned>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds2_4);
-   if (lds2_8 > 65536)
-      (void)hipFuncSetAttribute((const void *)k_bin_part2<unsigned long long>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2_8);
    const size_t lds = fl3k_bin_lds_bytes(a->shift);
    const unsigned gacc = (unsigned)ncu * (lds > 65536 ? 1 : 3);
    const int big = lds > 65536;
@@ -510,7 +543,10 @@ extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_
          hipLaunchKernelGGL(k_bin_part1<unsigned long long>, dim3(a->nblk), dim3(256), 0, s, *a);
          if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned long long>, dim3(g2), dim3(256), lds2_8, s, *a);
       } else {
-         hipLaunchKernelGGL(k_bin_part1<unsigned>, dim3(a->nblk), dim3(256), 0, s, *a);
+         if (a->rows >= FL3_PART1_STAGED_ROWS)
+            hipLaunchKernelGGL(k_bin_part1_staged<unsigned>, dim3(a->nblk), dim3(256), 0, s, *a);
+         else
+            hipLaunchKernelGGL(k_bin_part1<unsigned>, dim3(a->nblk), dim3(256), 0, s, *a);
          if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned>, dim3(g2), dim3(256), lds2_4, s, *a);
       }
       const size_t ln = fl3k_bin_narrow_lds_bytes(a->shift);
@@ -527,7 +563,10 @@ extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_
       else
          hipLaunchKernelGGL((k_bin_accum_narrow<2, false>), dim3(gn), dim3(FL3_ACC_THREADS), ln, s, *a);
    } else if (format == 1) {
-      hipLaunchKernelGGL(k_bin_part1<unsigned>, dim3(a->nblk), dim3(256), 0, s, *a);
+      if (a->rows >= FL3_PART1_STAGED_ROWS)
+         hipLaunchKernelGGL(k_bin_part1_staged<unsigned>, dim3(a->nblk), dim3(256), 0, s, *a);
+      else
+         hipLaunchKernelGGL(k_bin_part1<unsigned>, dim3(a->nblk), dim3(256), 0, s, *a);
       if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned>, dim3(g2), dim3(256), lds2_4, s, *a);
       if (big)
          (void)hipFuncSetAttribute((const void *)k_bin_accum<unsigned, false>,
