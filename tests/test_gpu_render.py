@@ -216,6 +216,9 @@ BINNED_CASES = [
      {}, 512, 512 * 101),
     ("earlyclip_rgba", lambda: G.synthetic_small(seed=84, width=96, height=80, quality=30),
      dict(earlyclip=1, nchan=4, transp=1), 768, 768 * 19),
+    # chunks of 80 rows: the first sort pass orders its batches in LDS
+    ("tall_chunks", lambda: G.synthetic_1080p(seed=89, width=256, height=144, quality=40, supersample=2),
+     {}, 1024, 1024 * 80 + 3),
     # an xform with opacity < 1: 8-byte records carrying the xform index
     ("opacity_wide", lambda: _with_opacity(G.synthetic_small(seed=88, width=128, height=96, quality=30)), {},
      1024, 1024 * 23),
