@@ -138,25 +138,29 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
          const uint32_t r = rng.next();
          const int fn =
              __builtin_amdgcn_readfirstlane(select_xform(G, 0, __builtin_amdgcn_readfirstlane(r) & 16383u));
-         bool retry = false;
-         if (apply_xform<float, WalkerRng, TIER, true>(G, fn, p, q, rng)) {
-            if (i < n) bad++;
-            consec++;
-            if (consec < 5) retry = true;
-            else consec = 0;
-         } else
-            consec = 0;
+         // A bad value restarts the point from a random one and plots
+         // nothing at this step (no retry): the workgroup steps together,
+         // and a retry would hold every lane for a step, at a place that
+         // depends on the chunk boundaries.  So a lane plots at most one
+         // point per step and the schedule is that of a single launch.
+         const bool good = !apply_xform<float, WalkerRng, TIER, true>(G, fn, p, q, rng);
+         if (!good && i < n) bad++;
          p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
-         if (!retry) {
-            if (use_final) {
-               if (G.final_opacity == 1.0f || u01<float>(rng) < G.final_opacity) {
-                  apply_xform<float, WalkerRng, TIER, true>(G, G.final_index, p, q, rng);
-                  q[3] = p[3];
-               }
+         if (good && use_final) {
+            if (G.final_opacity == 1.0f || u01<float>(rng) < G.final_opacity) {
+               apply_xform<float, WalkerRng, TIER, true>(G, G.final_index, p, q, rng);
+               q[3] = p[3];
             }
-            if (i >= 0 && i < n) plot(i, q, fn);
-            i++;
          }
+         if (i >= 0 && i < n) {
+            if (good)
+               plot(i, q, fn);
+            else if (RECORDS == FL3_REC_NARROW)
+               ((unsigned *)a.rec)[(blk * a.rows_out + i) * 256 + threadIdx.x] = FL3_REC_NONE;
+            else if (RECORDS)
+               a.rec[(blk * a.rows_out + i) * 256 + threadIdx.x] = FL3_REC_NONE;
+         }
+         i++;
          uint32_t h = st * 0x9E3779B9u ^ hseed;
          h ^= h >> 16; h *= 0x7feb352du; h ^= h >> 15; h *= 0x846ca68bu; h ^= h >> 16;
          const unsigned slot = ((h | 1u) * (threadIdx.x ^ ((h >> 8) & 255u)) + (h >> 16)) & 255u;

@@ -219,10 +219,21 @@ BINNED_CASES = [
     # chunks of 80 rows: the first sort pass orders its batches in LDS
     ("tall_chunks", lambda: G.synthetic_1080p(seed=89, width=256, height=144, quality=40, supersample=2),
      {}, 1024, 1024 * 80 + 3),
+    # an xform that overflows FP32 when applied twice running: bad values
+    # restart points mid-chunk, and the chunked render must still follow the
+    # single launch's schedule
+    ("bad_values_chunks", lambda: _with_blowup(G.synthetic_small(seed=90, width=128, height=96, quality=30)), {},
+     1024, 1024 * 29 + 7),
     # an xform with opacity < 1: 8-byte records carrying the xform index
     ("opacity_wide", lambda: _with_opacity(G.synthetic_small(seed=88, width=128, height=96, quality=30)), {},
      1024, 1024 * 23),
 ]
+
+
+def _with_blowup(g):
+    x = g.xforms[-1]
+    x.coefs = (1e20, 0.0, 0.0, 1e20, 0.0, 0.0)
+    return g
 
 
 def _with_opacity(g):
@@ -242,6 +253,8 @@ def test_binned_deposit_equals_atomic_deposit(prod, name, mk, kw, walkers, chunk
     b, sb = prod.render(g, seed="bin", nwalkers=walkers, deposit=2, chunk_samples=chunk, **kw)
     assert sa["num_iters"] == sb["num_iters"]
     assert sa["badvals"] == sb["badvals"]
+    if name == "bad_values_chunks":
+        assert sa["badvals"] > 100, sa["badvals"]
     assert np.array_equal(a, b), np.abs(a.astype(int) - b.astype(int)).max()
     assert a.mean() > 1.0
 
