@@ -132,12 +132,25 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
       const uint32_t bjob = dsc ? (uint32_t)(blockIdx.x - dsc->block0)
                                 : (nj > 1 ? (uint32_t)(blockIdx.x / nj) : (uint32_t)blk);
       const uint32_t hseed = ((uint32_t)kk ^ (uint32_t)(kk >> 32) * 0x85ebca6bu) ^ bjob * 0xc2b2ae35u;
+      // the xform at the first slot of each 64-slot bucket of the 14-bit
+      // selection slot: a draw starts there and steps over the few
+      // breakpoints inside its bucket (select_xform's count, in fewer steps)
+      unsigned short *sel = (unsigned short *)(xb + 1544);
+      {
+         const uint32_t b0 = threadIdx.x * 64u;
+         int f = 0;
+         for (int j = 1; j < G.nstd; j++) f += b0 >= (uint32_t)G.starts[j] ? 1 : 0;
+         sel[threadIdx.x] = (unsigned short)f;
+      }
+      __syncthreads();
       uint32_t st = a.resume ? (uint32_t)lastxf : 0u;
       long long i = a.resume ? 0 : -a.fuse;
       for (;;) {
          const uint32_t r = rng.next();
-         const int fn =
-             __builtin_amdgcn_readfirstlane(select_xform(G, 0, __builtin_amdgcn_readfirstlane(r) & 16383u));
+         const uint32_t xslot = __builtin_amdgcn_readfirstlane(r) & 16383u;
+         int fs = __builtin_amdgcn_readfirstlane(sel[xslot >> 6]);
+         while (fs + 1 < G.nstd && xslot >= (uint32_t)__builtin_amdgcn_readfirstlane(G.starts[fs + 1])) fs++;
+         const int fn = fs;
          // A bad value restarts the point from a random one and plots
          // nothing at this step (no retry): the workgroup steps together,
          // and a retry would hold every lane for a step, at a place that
