@@ -512,7 +512,7 @@ static int item_records(const fl3_job_hdr *const *h, int n) {
 // not depend on what else shares a launch).
 static bool wave_ok(const fl3_job_hdr *h) {
    return !h->chaos_enable && ((lds_bytes_for<float>(h, true) + 15) & ~(size_t)15) + 4 * (size_t)FL3_MAX_BINS + 16 +
-                                      1024 + 16 + (2 * 3 * 256 * 4 + 2 * 4 * 4 + 256 * 2) <=
+                                      1024 + 16 + (2 * 3 * 256 * 4 + 256 * 2) <=
                                   g_ctx.lds_max;
 }
 
@@ -523,10 +523,10 @@ static bool jobs_wave_ok(const fl3_job_hdr *const *h, int n) {
 }
 
 // Walker LDS with the point-exchange buffers of the one-xform-per-wavefront
-// mode appended (two 3 x 256 floats, two 4-word flag sets and the 256-entry
-// xform-selection table, at a 16-byte boundary); the mode stays off when
-// they would not fit.
-#define FL3_SHUF_BYTES (2 * 3 * 256 * 4 + 2 * 4 * 4 + 256 * 2)
+// mode appended (two 3 x 256 floats and the 256-entry xform-selection
+// table, at a 16-byte boundary); the mode stays off when they would not
+// fit.
+#define FL3_SHUF_BYTES (2 * 3 * 256 * 4 + 256 * 2)
 static size_t with_shuffle(IterArgs &a, size_t lds, bool want) {
    const size_t off = (lds + 15) & ~(size_t)15;
    if (want && off + FL3_SHUF_BYTES <= g_ctx.lds_max) {

@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
       // the xform at the first slot of each 64-slot bucket of the 14-bit
       // selection slot: a draw starts there and steps over the few
       // breakpoints inside its bucket (select_xform's count, in fewer steps)
-      unsigned short *sel = (unsigned short *)(xb + 1544);
+      unsigned short *sel = (unsigned short *)(xb + 1536);
       {
          const uint32_t b0 = threadIdx.x * 64u;
          int f = 0;
@@ -144,6 +144,11 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
       }
       __syncthreads();
       uint32_t st = a.resume ? (uint32_t)lastxf : 0u;
+      // every lane takes one step at a time, so i is the same in all lanes
+      // and the workgroup runs until its first walker (the one with the
+      // most samples) is done
+      long long nwg = nsj / Wj + ((wj - (long long)threadIdx.x) < nsj % Wj ? 1 : 0);
+      if (dsc && RECORDS) nwg = nwg - a.row0 < 0 ? 0 : (nwg - a.row0 < a.rows_out ? nwg - a.row0 : a.rows_out);
       long long i = a.resume ? 0 : -a.fuse;
       for (;;) {
          const uint32_t r = rng.next();
@@ -178,20 +183,17 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
          h ^= h >> 16; h *= 0x7feb352du; h ^= h >> 15; h *= 0x846ca68bu; h ^= h >> 16;
          const unsigned slot = ((h | 1u) * (threadIdx.x ^ ((h >> 8) & 255u)) + (h >> 16)) & 255u;
          float *buf = xb + (st & 1u) * 768;
-         unsigned *more4 = (unsigned *)(xb + 1536) + (st & 1u) * 4;   // per wavefront: lanes left
          st++;
          buf[slot] = p[0];
          buf[256 + slot] = p[1];
          buf[512 + slot] = p[2];
-         const unsigned long long left = __ballot(i < n);
-         if ((threadIdx.x & 63) == 0) more4[threadIdx.x >> 6] = left != 0ull;
          // (two buffers: a wavefront writes buffer st & 1 only after every
          // wavefront has passed the barrier behind its reads of it)
          __syncthreads();
          p[0] = buf[threadIdx.x];
          p[1] = buf[256 + threadIdx.x];
          p[2] = buf[512 + threadIdx.x];
-         if (!(more4[0] | more4[1] | more4[2] | more4[3])) break;
+         if (i >= nwg) break;
       }
       lastxf = (int)st;   // the step count is what resumes here
    } else {
