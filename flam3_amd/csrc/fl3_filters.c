@@ -8,8 +8,10 @@
  * lanczos3 there, evaluates the Mitchell kernel with support 2.0).
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 #include "fl3_host.h"
 
 #define DE_THRESH 100
@@ -170,7 +172,7 @@ int fl3_create_spatial_filter(const flam3_genome *g0, double aspect, int field, 
 /* Density-estimation kernel bank (filters.c:272-407).  Kernel k has radius
  * h_k that shrinks with the hit count; only the octant j >= k >= 0 of each
  * (2r-1)^2 Gaussian is stored. */
-int fl3_create_de_filters(double max_rad, double min_rad, double curve, int ss, fl3_de_filters *de) {
+static int build_de_filters(double max_rad, double min_rad, double curve, int ss, fl3_de_filters *de) {
    double cmax, cmin, nf_d;
    int nf, max_ind, row, half, loop;
    const int keep = 100;
@@ -236,6 +238,61 @@ int fl3_create_de_filters(double max_rad, double min_rad, double curve, int ss, 
       if (de->max_filter_index > 0) break;
    }
    if (de->max_filter_index == 0) de->max_filter_index = max_ind - 1;
+   return 0;
+}
+
+/* The bank is a pure function of (max_rad, min_rad, curve, ss) and takes
+ * most of a millisecond to build at radius 9 and oversample 2, once per
+ * frame: the last few banks are kept and handed out as copies. */
+#define DE_CACHE 8
+static struct {
+   double key[3];
+   int ss, used;
+   fl3_de_filters de;
+} de_cache[DE_CACHE];
+static int de_cache_next;
+static pthread_mutex_t de_cache_mu = PTHREAD_MUTEX_INITIALIZER;
+
+static int copy_de_filters(const fl3_de_filters *src, fl3_de_filters *dst) {
+   const size_t nc = (size_t)src->nfilters * (size_t)src->kernel_size;
+   *dst = *src;
+   dst->filter_coefs = (double *)malloc(sizeof(double) * (nc > 0 ? nc : 1));
+   dst->filter_widths = (double *)malloc(sizeof(double) * (size_t)(src->nfilters > 0 ? src->nfilters : 1));
+   if (!dst->filter_coefs || !dst->filter_widths) {
+      free(dst->filter_coefs);
+      free(dst->filter_widths);
+      dst->filter_coefs = dst->filter_widths = NULL;
+      return 1;
+   }
+   memcpy(dst->filter_coefs, src->filter_coefs, sizeof(double) * nc);
+   memcpy(dst->filter_widths, src->filter_widths, sizeof(double) * (size_t)src->nfilters);
+   return 0;
+}
+
+int fl3_create_de_filters(double max_rad, double min_rad, double curve, int ss, fl3_de_filters *de) {
+   const double key[3] = {max_rad, min_rad, curve};
+   int i, rc;
+   pthread_mutex_lock(&de_cache_mu);
+   for (i = 0; i < DE_CACHE; i++)
+      if (de_cache[i].used && de_cache[i].ss == ss && !memcmp(de_cache[i].key, key, sizeof(key))) {
+         rc = copy_de_filters(&de_cache[i].de, de);
+         pthread_mutex_unlock(&de_cache_mu);
+         return rc;
+      }
+   pthread_mutex_unlock(&de_cache_mu);
+   rc = build_de_filters(max_rad, min_rad, curve, ss, de);
+   if (rc) return rc;
+   pthread_mutex_lock(&de_cache_mu);
+   i = de_cache_next;
+   de_cache_next = (de_cache_next + 1) % DE_CACHE;
+   if (de_cache[i].used) fl3_free_de_filters(&de_cache[i].de);
+   de_cache[i].used = 0;
+   if (copy_de_filters(de, &de_cache[i].de) == 0) {
+      memcpy(de_cache[i].key, key, sizeof(key));
+      de_cache[i].ss = ss;
+      de_cache[i].used = 1;
+   }
+   pthread_mutex_unlock(&de_cache_mu);
    return 0;
 }
 
