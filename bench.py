@@ -102,7 +102,7 @@ def parse_args():
     p.add_argument("--no-still", action="store_true",
                    help="profiling aid: run only the animation and preview legs and print their results")
     p.add_argument("--anim-frames", type=int, default=120)
-    p.add_argument("--preview-requests", type=int, default=2000)
+    p.add_argument("--preview-requests", type=int, default=10000)
     p.add_argument("--copy-probe", action="store_true",
                    help="after the timed steps, run k_copy_probe over 1 GiB (a plain copy kernel of known traffic: "
                         "1 GiB read, 1 GiB written) to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -179,6 +179,8 @@ def animation_bench(args):
     from flam3_amd import abi
     from flam3_amd import genome as G
 
+    from flam3_amd import opcount
+
     L = F.lib()
     gs = G.synthetic_animation(seed=3, frames=args.anim_frames)
     arr, owned = G.genome_array(gs)
@@ -233,6 +235,8 @@ def animation_bench(args):
                 "plain_frames_per_s": n / plain, "plain_ms_per_frame": 1e3 * plain / n,
                 "served_frames_per_s": n / served, "served_ms_per_frame": 1e3 * served / n,
                 "served_samples_per_s": samples / served,
+                "served_vs_plain": plain / served,
+                "roofline": served_roofline(samples / served, opcount.ops_per_sample(gs[0])),
                 "note": "per_sample_launch: one walker launch per temporal sample (the reference's loop, first "
                         "frames only); plain: one flam3_render call per frame, its 100 temporal samples in one "
                         "walker launch; served: the render server, ~8 frames per shared launch with the host work "
@@ -242,40 +246,64 @@ def animation_bench(args):
             G.free_struct(gp)
 
 
+ACCUM_BYTES_PER_SAMPLE = 40.0   # 5 channels x (4 B read + 4 B written) per deposited sample (bits 33)
+
+
+def served_roofline(samples_per_s, ops_per_sample):
+    """A served leg's throughput as fractions of the two bounds of its
+    dominant stages: the deposit's algorithmic bytes (40 B per sample at bits
+    33) against HBM, and the iteration's counted FP32 ops against the FP32
+    vector peak."""
+    return {"accumulation_hbm_frac": samples_per_s * ACCUM_BYTES_PER_SAMPLE / (HBM_PEAK_GBS * 1e9),
+            "iteration_fp32_frac": samples_per_s * ops_per_sample / (FP32_PEAK_TFLOPS * 1e12),
+            "ops_per_sample": ops_per_sample}
+
+
+def preview_pool(n):
+    """The genome-search preview workload: n seeded random genomes (2-12
+    xforms of the 12 workload variations, 64^2..512^2, quality U[10,100],
+    ss1), one per request."""
+    from flam3_amd import genome as G
+    return [G.synthetic_preview(1000 + k) for k in range(n)]
+
+
 def preview_bench(args):
-    """Genome-search preview serving: requests drawn from 64 seeded preview
-    genomes (2-12 xforms of the 12 workload variations, 64^2..512^2, quality
-    U[10,100], ss1), Poisson arrivals at several rates into the render
-    server; latency from submit to image ready."""
+    """Genome-search preview serving: a stream of distinct seeded preview
+    genomes with Poisson arrivals at several rates into the render server;
+    latency from submit to image ready."""
     import numpy as np
     import flam3_amd as F
     from flam3_amd import abi
     from flam3_amd import genome as G
+    from flam3_amd import opcount
 
     L = F.lib()
-    pool = [G.synthetic_preview(1000 + k) for k in range(64)]
-    arrs = [G.genome_array([g]) for g in pool]
     n = args.preview_requests
+    t0 = time.perf_counter()
+    pool = preview_pool(n)
+    arrs = [G.genome_array([g]) for g in pool]
+    log("preview: %d genomes built in %.1f s" % (n, time.perf_counter() - t0))
+    samples = np.array([g.width * g.height * g.quality for g in pool])
+    ops = float(np.sum([opcount.ops_per_sample(g) * s for g, s in zip(pool[:500], samples[:500])])
+                / samples[:500].sum())
     rng = np.random.default_rng(4)
-    picks = rng.integers(0, len(pool), n)
 
     def build(k):
-        g = pool[picks[k]]
-        f = F.make_frame(ctypes.addressof(arrs[picks[k]][0]), 1, seed="req-%d" % k)
+        g = pool[k]
+        f = F.make_frame(ctypes.addressof(arrs[k][0]), 1, seed="req-%d" % k)
         return f, np.zeros((g.height, g.width, 3), np.uint8), abi.stat_struct()
 
     try:
         f, img, st = build(0)
         L.flam3_render(ctypes.byref(f), img.ctypes.data_as(ctypes.c_void_p), 0, 3, 0, ctypes.byref(st))
-        nseq = min(n, 200)
+        nseq = min(n, 300)
         t0 = time.perf_counter()
         for k in range(nseq):
             f, img, st = build(k)
             L.flam3_render(ctypes.byref(f), img.ctypes.data_as(ctypes.c_void_p), 0, 3, 0, ctypes.byref(st))
         alone = (time.perf_counter() - t0) / nseq
-        mean_samples = float(np.mean([pool[p].width * pool[p].height * pool[p].quality for p in picks]))
         sweeps = []
-        for rate in (200.0, 1000.0, 2000.0, None):
+        for rate in (1000.0, 2000.0, 4000.0, None):
             arrive = np.cumsum(rng.exponential(1.0 / rate, n)) if rate else np.zeros(n)
             inputs = [build(k) for k in range(n)]
             srv = F.Server()
@@ -298,23 +326,37 @@ def preview_bench(args):
             batches, _ = srv.counts()
             srv.close()
             lat = np.array(lat)
+            sps = float(samples.sum()) / wall
             sweeps.append({"offered_per_s": rate if rate else "all at once", "requests": n,
                            "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
-                           "requests_per_s": n / wall, "samples_per_s": n * mean_samples / wall, "batches": batches})
+                           "requests_per_s": n / wall, "samples_per_s": sps, "batches": batches,
+                           "roofline": served_roofline(sps, ops)})
             log("preview rate %s: p50 %.2f ms p99 %.2f ms, %.0f req/s" % (rate, sweeps[-1]["p50_ms"],
                                                                          sweeps[-1]["p99_ms"], n / wall))
-        return {"workload": "preview requests: 64 seeded genomes (2-12 xforms, 64^2..512^2, q U[10,100], ss1), "
-                            "Poisson arrivals", "mean_samples_per_request": mean_samples,
-                "alone_ms_per_request": 1e3 * alone, "alone_requests_per_s": 1.0 / alone, "sweep": sweeps}
+        return {"workload": "preview requests: %d distinct seeded genomes (2-12 xforms, 64^2..512^2, q U[10,100], "
+                            "ss1), Poisson arrivals" % n, "mean_samples_per_request": float(samples.mean()),
+                "alone_ms_per_request": 1e3 * alone, "alone_requests_per_s": 1.0 / alone,
+                "alone_sample": "first %d requests, one flam3_render call each" % nseq, "sweep": sweeps}
     finally:
         for arr, owned in arrs:
             for gp in owned:
                 G.free_struct(gp)
 
 
+def host_threads():
+    """Threads for the CPU reference: the host's CPUs available to this
+    process, at most 16 (the CPU share of one GPU on the GPU hosts, which
+    report more CPUs than the share)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
 def cpu_preview_baseline(args):
-    """Reference flam3_render (oracle/_ref fast build, all host threads) on
-    the first 20 preview requests, one after another."""
+    """Reference flam3_render (oracle/_ref fast build, host_threads()) on
+    the first preview requests, one after another."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import pyref
@@ -322,36 +364,44 @@ def cpu_preview_baseline(args):
 
     if not pyref.available("fast"):
         return None
-    threads = max(1, min(16, os.cpu_count() or 1))
-    pool = [G.synthetic_preview(1000 + k) for k in range(64)]
-    picks = np.random.default_rng(4).integers(0, len(pool), args.preview_requests)[:20]
-    docs = [G.flame_document([pool[p]]) for p in picks]
+    threads = host_threads()
+    pool = preview_pool(60)
+    docs = [G.flame_document([g]) for g in pool]
     t0 = time.perf_counter()
     for k, xml in enumerate(docs):
         pyref.render(xml, seed="req-%d" % k, nthreads=threads, kind="fast")
     dt = time.perf_counter() - t0
     return {"requests_per_s": len(docs) / dt, "ms_per_request": 1e3 * dt / len(docs), "cores": threads,
-            "kind": "reference", "sample": "first %d preview requests, serial flam3_render calls" % len(docs)}
+            "kind": "reference", "sample": "first %d preview requests, serial flam3_render calls (%.1f s)"
+                                          % (len(docs), dt)}
 
 
 def cpu_baseline(args):
-    """Reference flam3_render (oracle/_ref fast build) with every host thread."""
+    """Reference flam3_render (oracle/_ref fast build) with host_threads()
+    threads on the same genome at a reduced quality, once with density
+    estimation and once without, so that the DE share of the CPU time is
+    known."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyref
     from flam3_amd import genome as G
 
     if not pyref.available("fast"):
         return None
-    # the box's CPU share for one GPU is 16 threads, whatever os.cpu_count() reports
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
     g = G.synthetic_1080p(seed=2, quality=args.cpu_quality)
     xml = G.flame_document([g])
     t0 = time.perf_counter()
     _, st = pyref.render(xml, seed="cpu", nthreads=threads, kind="fast")
     dt = time.perf_counter() - t0
+    g.estimator_radius = 0.0
+    t0 = time.perf_counter()
+    pyref.render(G.flame_document([g]), seed="cpu", nthreads=threads, kind="fast")
+    dt0 = time.perf_counter() - t0
     return {"value": st["num_iters"] / dt, "unit": "samples/s", "cores": threads, "kind": "reference",
+            "de_share": max(0.0, (dt - dt0) / dt),
             "sample": "reference flam3_render of the same 1080p genome at quality %g (%d samples, %d threads, "
-                      "%.2f s wall incl. DE and tone map)" % (args.cpu_quality, st["num_iters"], threads, dt)}
+                      "%.2f s wall incl. DE and tone map; %.2f s with DE off)"
+                      % (args.cpu_quality, st["num_iters"], threads, dt, dt0)}
 
 
 def main():
@@ -393,17 +443,19 @@ def main():
                                "1920x1080 supersample 2 quality %g, DE radius 9, bits 33, 8-bit RGB out"
                                % args.quality, "samples_per_frame": samples_per_step,
                    "frame_time_ms": ms_per_step},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": measured_traffic() if args.quality == 500.0 else None,
                      "kernel": "k_iter_fast", "kernel_ms": it_ms, "samples_per_launch": per_launch,
                      "ops_per_sample": ops,
                      "algorithmic_bytes_per_launch": 4.0 * per_launch,
                      "valu_issue_frac": valu_issue_frac(per_launch, it_ms) if args.quality == 500.0 else None,
-                     "note": "FP32 vector-ALU bound (no matrix work on this path); peak is MI355X FP32, "
-                             "157.3 TFLOP/s for vector and dense matrix alike. achieved counts only the formula "
-                             "ops of each sample's chosen xform; the kernel issues ~517 VALU instructions per "
-                             "sample (every xform's variations present in a wavefront, RNG, selection, trig "
-                             "reduction, record), valu_issue_frac is the share of VALU issue slots it uses"},
+                     "valu_instructions_per_sample": profile_value(VALU_FILE, "valu_instructions_per_sample"),
+                     "note": "bound by FP32 vector ALU work (no matrix work on this path); peak is MI355X's "
+                             "FP32 vector peak, 157.3 TFLOP/s. achieved counts only the formula ops of each "
+                             "sample's chosen xform and the final xform; the kernel also issues RNG, selection, "
+                             "trig range reduction, the point exchange and the record "
+                             "(valu_instructions_per_sample, rocprofv3 SQ_INSTS_VALU); valu_issue_frac is the "
+                             "share of VALU issue slots it uses"},
         "stages": {
             "iterate_ms_per_launch": it_ms,
             "bin_ms_per_chunk": bin_ms,

@@ -100,6 +100,8 @@ def lib():
     L.fl3h_deposit_points.restype = ci
     L.fl3h_density_estimate.argtypes = [vp, ci, ci, ci, ci, cd, cd, cd, ci, ci, ci, ci, vp, vp, vp]
     L.fl3h_density_estimate.restype = ci
+    L.fl3h_density_estimate_production.argtypes = [vp, ci, ci, ci, ci, cd, cd, cd, ci, ci, ci, ci, vp, vp, vp]
+    L.fl3h_density_estimate_production.restype = ci
     L.fl3h_tonemap.argtypes = [vp, ci, ci, ci, P(fl3h_tonemap_params), vp, vp]
     L.fl3h_tonemap.restype = ci
     L.fl3h_init_frame.argtypes = [P(abi.flam3_frame), ctypes.c_char_p]

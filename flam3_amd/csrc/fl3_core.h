@@ -358,6 +358,10 @@ __device__ __forceinline__ void opaque_pre(Pre<R> &f) {
                    "+v"(f.cosa), "+v"(f.atanyx));
 }
 
+#ifndef FL3_UNI_STRAIGHT
+#define FL3_UNI_STRAIGHT 1
+#endif
+
 // UNI: fn is the same in every lane of the wavefront (the final xform), so
 // the loop runs over that xform's own variations only.
 template <typename R, class RNG, int TIER = FL3_TIER_FULL, bool UNI = false>
@@ -395,9 +399,21 @@ __device__ bool apply_xform(const GenL<R> &G, int fn, const R *p, R *q, RNG &rng
    // compiler hoisting every case's loads (and registers) out of the loop.
    const uint32_t xoff = (uint32_t)((const unsigned char *)&X - fl3_lds);
    if (UNI) {
+      // The first FL3_UNI_STRAIGHT variations of the xform are dispatched in
+      // straight-line code (no loop, so nothing of the other cases can be
+      // hoisted and the inputs need no opaque copies); any further ones in
+      // a loop as below.
       const int fu = __builtin_amdgcn_readfirstlane(fn);
-      const int nv = G.vcount[fu];
-      for (int k = 0; k < nv; k++) {
+      const int nv = __builtin_amdgcn_readfirstlane(G.vcount[fu]);
+#pragma unroll
+      for (int k = 0; k < FL3_UNI_STRAIGHT; k++) {
+         if (k < nv) {
+            const int u = __builtin_amdgcn_readfirstlane((int)G.vlist[fu * G.nunion + k]);
+            const int v = __builtin_amdgcn_readfirstlane(G.uid[u]);
+            fl3_var_tier<R, RNG, TIER>(v, wrow[u], X, f, rng);
+         }
+      }
+      for (int k = FL3_UNI_STRAIGHT; k < nv; k++) {
          const int u = __builtin_amdgcn_readfirstlane((int)G.vlist[fu * G.nunion + k]);
          const int v = __builtin_amdgcn_readfirstlane(G.uid[u]);
          const R w = wrow[u];
@@ -540,7 +556,7 @@ template <typename R>
 __device__ __forceinline__ unsigned long long make_record(const GenL<R> &G, const R *q, int fn) {
 #pragma clang fp contract(off)
    R x = q[0], y = q[1];
-   if (G.rotate) {
+   if (__builtin_amdgcn_readfirstlane(G.rotate)) {   // (the same in every lane: a scalar branch)
       R x0 = x - G.rc0, y0 = y - G.rc1;
       x = x0 * G.rot00 + y0 * G.rot01 + G.rc0;
       y = x0 * G.rot10 + y0 * G.rot11 + G.rc1;
@@ -549,10 +565,12 @@ __device__ __forceinline__ unsigned long long make_record(const GenL<R> &G, cons
    // the reference's linear index ix + width * iy (rect.c:440-441) in 32-bit
    // unsigned arithmetic: histograms here stay below 2^31 cells, so a
    // negative index wraps above the cell count and is dropped like one past
-   // the end
+   // the end.  Inside the bounds iy is in [0, height] (a truncation towards
+   // zero of at least -1 ulp), below 2^24, so a 24-bit multiply forms the
+   // same product.
    const int ix = (int)(G.ws0 * x - G.wb0s0);
    const int iy = (int)(G.hs1 * y - G.hb1s1);
-   const unsigned cell = (unsigned)ix + (unsigned)G.width * (unsigned)iy;
+   const unsigned cell = (unsigned)ix + __umul24((unsigned)G.width, (unsigned)iy);
    if (cell >= (unsigned)G.width * (unsigned)G.height) return FL3_REC_NONE;
    int ci;
    R frac;

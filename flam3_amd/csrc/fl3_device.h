@@ -19,6 +19,9 @@ const char *fl3d_error(void);
 
 /* Histogram + accumulator for one render, cleared. */
 int fl3d_begin(int width, int height, int bits);
+/* Device bytes a render may use (the device's free memory plus what the
+ * library's buffers already hold) and the device's total memory. */
+int fl3d_device_memory(double *avail, double *total);
 /* Zero the histogram (start of each batch, rect.c:721). */
 int fl3d_clear_buckets(void);
 /* Histograms of n renders in one allocation (cleared); fl3d_select(r) makes

@@ -17,6 +17,7 @@
 //  * k_earlyclip + k_tonemap: spatial filter, log/gamma/vibrancy/highlight
 //    tone map and quantisation fused into one pass (rect.c:1067-1247).
 #include <hip/hip_runtime.h>
+#include <dlfcn.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -93,6 +94,7 @@ struct Ctx {
    DevBuf sorted, wstate, bins, jobtab, soff;
    DevBuf srcv, srcl;                  // density estimation per-source values
    DevBuf detab, deargs;               // batched density estimation: tables, per-render arguments
+   DevBuf deblk;                       // split DE gather: widest source per 16x16 block
    unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
    void *staging = nullptr;                   // pinned host buffer for the image download
    size_t staging_cap = 0;
@@ -129,6 +131,26 @@ static int ctx_init() {
    HIPCHK(hipMalloc((void **)&g_ctx.counters, 64));
    g_ctx.ready = 1;
    g_ready.store(1, std::memory_order_release);
+   return 0;
+}
+
+// Device memory a render may use: the device's free bytes plus what this
+// context already holds (its buffers are reused for the next renders).
+extern "C" int fl3d_device_memory(double *avail, double *total) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   if (ctx_init()) return 1;
+   size_t fr = 0, tot = 0;
+   HIPCHK(hipMemGetInfo(&fr, &tot));
+   const DevBuf *bufs[] = {&g_ctx.buckets, &g_ctx.accum, &g_ctx.job, &g_ctx.table, &g_ctx.image, &g_ctx.isaac,
+                           &g_ctx.points, &g_ctx.misc, &g_ctx.findex, &g_ctx.sorted, &g_ctx.wstate, &g_ctx.bins,
+                           &g_ctx.jobtab, &g_ctx.soff, &g_ctx.srcv, &g_ctx.srcl, &g_ctx.detab, &g_ctx.deargs,
+                           &g_ctx.rcount, &g_ctx.descs, &g_ctx.deblk};
+   double held = 0.0;
+   for (const DevBuf *b : bufs) held += (double)b->cap;
+   for (int i = 0; i < FL3_CHUNK_BUFS; i++)
+      held += (double)(g_ctx.crec[i].cap + g_ctx.cblk[i].cap + g_ctx.ctot[i].cap);
+   if (avail) *avail = (double)fr + held;
+   if (total) *total = (double)tot;
    return 0;
 }
 
@@ -233,6 +255,7 @@ extern "C" int fl3h_copy_probe(long long bytes) {
 
 static size_t bucket_bytes(int bits) { return bits == 64 ? 40 : 20; }
 #define DG_REGION_HOST 16   // cells per side of a density-estimation workgroup (DG_REGION)
+#define DE_SPLIT_LDS_MAX 24576   // largest small-kernel table the split DE gather keeps in LDS
 static size_t accum_bytes(int bits);
 // the selected render's histogram and accumulator
 static void *cur_buckets() { return (unsigned char *)g_ctx.buckets.p + g_ctx.cell0 * bucket_bytes(g_ctx.bits); }
@@ -302,10 +325,78 @@ static int read_counters(double *badvals, int *saturated) {
 
 // Walkers per launch: 8 workgroups of 256 per CU, never fewer than ~256
 // samples per walker (each walker pays its fuse steps once).
+// ------------------------------------------------------------------ tuning
+//
+// Launch parameters chosen by the autotuner (flam3_amd/autotune.py), read at
+// set-up from the file FL3H_TUNING names, else from tuning.conf beside the
+// library ("key value" lines), else the defaults below.  They shape the
+// walkers (and so the sample streams) the same way for every render of the
+// process, alone or shared.
+struct Tuning {
+   int wg_per_cu = 8;               // walker workgroups of 256 per CU
+   long long min_samples = 256;     // samples per walker at least
+   long long chunk_samples = FL3_BIN_CHUNK_SAMPLES;   // binned deposit: samples per chunk
+   int de_split = 1;                // production DE: split gather (1) or one pass (0)
+};
+static Tuning g_tune;
+static std::once_flag g_tune_once;
+
+static int tune_set(const char *key, double v) {
+   if (!strcmp(key, "wg_per_cu") && v >= 1 && v <= 16) g_tune.wg_per_cu = (int)v;
+   else if (!strcmp(key, "min_samples") && v >= 1 && v <= 65536) g_tune.min_samples = (long long)v;
+   else if (!strcmp(key, "chunk_samples") && v >= (1 << 20) && v <= (double)(1LL << 30))
+      g_tune.chunk_samples = (long long)v;
+   else if (!strcmp(key, "de_split") && (v == 0 || v == 1)) g_tune.de_split = (int)v;
+   else return 1;
+   return 0;
+}
+
+static void tune_load() {
+   const char *path = getenv("FL3H_TUNING");
+   char buf[1024];
+   if (!path) {
+      Dl_info di;
+      if (!dladdr((void *)&tune_set, &di) || !di.dli_fname) return;
+      snprintf(buf, sizeof(buf), "%s", di.dli_fname);
+      char *slash = strrchr(buf, '/');
+      if (!slash) return;
+      snprintf(slash + 1, sizeof(buf) - (size_t)(slash + 1 - buf), "tuning.conf");
+      path = buf;
+   }
+   FILE *fp = fopen(path, "r");
+   if (!fp) return;
+   char line[256], key[64];
+   double v;
+   while (fgets(line, sizeof(line), fp))
+      if (line[0] != '#' && sscanf(line, "%63s %lf", key, &v) == 2 && tune_set(key, v))
+         fprintf(stderr, "flam3_amd: %s: ignoring tuning entry %s %g\n", path, key, v);
+   fclose(fp);
+}
+
+static void tune_init() { std::call_once(g_tune_once, tune_load); }
+
+// Set (or, with value NULL, read) one tuning parameter for the rest of the
+// process; returns 0, or 1 for an unknown key or a value out of range.
+extern "C" int fl3h_tuning(const char *key, const double *value, double *out) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   tune_init();
+   if (!key) return 1;
+   if (value && tune_set(key, *value)) return 1;
+   double v;
+   if (!strcmp(key, "wg_per_cu")) v = g_tune.wg_per_cu;
+   else if (!strcmp(key, "min_samples")) v = (double)g_tune.min_samples;
+   else if (!strcmp(key, "chunk_samples")) v = (double)g_tune.chunk_samples;
+   else if (!strcmp(key, "de_split")) v = g_tune.de_split;
+   else return 1;
+   if (out) *out = v;
+   return 0;
+}
+
 static long long choose_walkers(long long nsamples, int requested) {
    if (requested > 0) return requested;
-   long long maxw = (long long)g_ctx.ncu * 8 * 256;
-   long long w = nsamples / 256;
+   tune_init();
+   long long maxw = (long long)g_ctx.ncu * g_tune.wg_per_cu * 256;
+   long long w = nsamples / g_tune.min_samples;
    if (w > maxw) w = maxw;
    if (w < 256) w = 256;
    return (w + 255) / 256 * 256;
@@ -551,7 +642,7 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    // one palette for all jobs: narrow or wide records; else the job index
    // rides in the record (step palettes only: it takes the fraction's bits)
    const int format = narrow ? 1 : (js.same_tables ? 0 : 2);
-   long long per = p->chunk > 0 ? (p->chunk + W - 1) / W : (FL3_BIN_CHUNK_SAMPLES / W);
+   long long per = p->chunk > 0 ? (p->chunk + W - 1) / W : (g_tune.chunk_samples / W);
    if (per < 1) per = 1;
    const long long rows_total = (p->nsamples + Wj - 1) / Wj;
    if (per > rows_total) per = rows_total;
@@ -913,7 +1004,7 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
       bool narrow = narrow_records_ok(hs[jrec], tcells);
       for (int j = jrec + 1; narrow && j < njobs; j++)
          if (grp[j] >= MG_REC && !same_narrow_tables(hs[jrec], hs[j])) narrow = false;
-      long long per = FL3_BIN_CHUNK_SAMPLES / Wrec;
+      long long per = g_tune.chunk_samples / Wrec;
       if (per < 1) per = 1;
       if (per > rows_total) per = rows_total;
       if ((double)Wrec * per >= 4.0e9) {
@@ -1129,7 +1220,15 @@ extern "C" int fl3d_density(const fl3d_de_params *p, const fl3_de_filters *de) {
    if (g_ctx.srcl.ensure(sizeof(double) * (size_t)g_ctx.width * g_ctx.height)) return 1;
    a.srcv = g_ctx.srcv.p;
    a.srcl = (double *)g_ctx.srcl.p;
-   a.exact = p->exact || g_ctx.bits == 64;
+   a.exact = p->exact;
+   tune_init();
+   if (!a.exact && g_tune.de_split && fl3k_de_split_lds(a.max_filter_index, g_ctx.bits) <= DE_SPLIT_LDS_MAX) {
+      a.nbx = (g_ctx.width + DG_REGION_HOST - 1) / DG_REGION_HOST;
+      const size_t nblk = (size_t)a.nbx * ((g_ctx.height + DG_REGION_HOST - 1) / DG_REGION_HOST);
+      if (g_ctx.deblk.ensure(sizeof(int) * nblk)) return 1;
+      a.blkmax = (int *)g_ctx.deblk.p;
+   } else if (g_ctx.bits == 64)
+      a.exact = 1;   // (the one-pass FP64 gather is the exact one)
    const int pk = prof_begin(ST_DE);
    const int lrc = fl3k_launch_de_tiled(&a, g_ctx.stream, g_ctx.bits);
    prof_end(pk);
@@ -1196,6 +1295,19 @@ static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *d
       return 1;
    std::vector<DEArgs> as((size_t)nd);
    std::vector<unsigned> first(2 * (size_t)(nd + 1));
+   // split gather when every render's small-kernel table fits the LDS limit
+   int mfi_max = 0;
+   long long nblk = 0;
+   for (int k = 0; k < nd; k++) {
+      mfi_max = std::max(mfi_max, de[dej[k]]->max_filter_index);
+      nblk += (long long)((g_ctx.rwidth[dej[k]] + DG_REGION_HOST - 1) / DG_REGION_HOST) *
+              ((g_ctx.rheight[dej[k]] + DG_REGION_HOST - 1) / DG_REGION_HOST);
+   }
+   const size_t split_lds = fl3k_de_split_lds(mfi_max, 33);
+   tune_init();
+   const bool split = g_tune.de_split && split_lds <= DE_SPLIT_LDS_MAX;
+   if (split && g_ctx.deblk.ensure(sizeof(int) * (size_t)nblk)) return 1;
+   long long blk0 = 0;
    unsigned *pf = first.data(), *gf = first.data() + nd + 1;
    unsigned long long pb = 0, gb = 0;
    std::vector<unsigned char> hostt(tab);
@@ -1253,6 +1365,11 @@ static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *d
       int half = 0;
       while ((half + 1) * (half + 2) / 2 < d->kernel_size) half++;
       a.half = half;
+      if (split) {
+         a.nbx = (a.width + DG_REGION_HOST - 1) / DG_REGION_HOST;
+         a.blkmax = (int *)g_ctx.deblk.p + blk0;
+         blk0 += (long long)a.nbx * ((a.height + DG_REGION_HOST - 1) / DG_REGION_HOST);
+      }
       pf[k] = (unsigned)pb;
       gf[k] = (unsigned)gb;
       pb += (unsigned long long)((a.width + 255) / 256) * a.height;
@@ -1273,7 +1390,8 @@ static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *d
    const int pk = prof_begin(ST_DE);
    const unsigned *dfirst = (const unsigned *)((unsigned char *)g_ctx.deargs.p + ab);
    const int lrc = fl3k_launch_de_many((const DEArgs *)g_ctx.deargs.p, dfirst, (unsigned)pb, dfirst + nd + 1,
-                                       (unsigned)gb, nd, g_ctx.stream);
+                                       (unsigned)gb, nd, g_ctx.stream, split ? (int *)g_ctx.deblk.p : nullptr,
+                                       (long)nblk, split_lds);
    prof_end(pk);
    if (lrc) {
       set_err("density estimation launch failed");
@@ -1736,10 +1854,32 @@ extern "C" int fl3h_deposit_points(const fl3h_camera *cam, const double *points,
    return rcode;
 }
 
+static int density_estimate_host(const void *buckets, int width, int height, int bits, int oversample, double k1,
+                                 double k2, double curve, int max_filtered_counts, int max_filter_index,
+                                 int kernel_size, int nfilters, const double *filter_widths,
+                                 const double *filter_coefs, void *accum, int exact);
+
 extern "C" int fl3h_density_estimate(const void *buckets, int width, int height, int bits, int oversample,
                                      double k1, double k2, double curve, int max_filtered_counts,
                                      int max_filter_index, int kernel_size, int nfilters,
                                      const double *filter_widths, const double *filter_coefs, void *accum) {
+   return density_estimate_host(buckets, width, height, bits, oversample, k1, k2, curve, max_filtered_counts,
+                                max_filter_index, kernel_size, nfilters, filter_widths, filter_coefs, accum, 1);
+}
+
+extern "C" int fl3h_density_estimate_production(const void *buckets, int width, int height, int bits,
+                                                int oversample, double k1, double k2, double curve,
+                                                int max_filtered_counts, int max_filter_index, int kernel_size,
+                                                int nfilters, const double *filter_widths,
+                                                const double *filter_coefs, void *accum) {
+   return density_estimate_host(buckets, width, height, bits, oversample, k1, k2, curve, max_filtered_counts,
+                                max_filter_index, kernel_size, nfilters, filter_widths, filter_coefs, accum, 0);
+}
+
+static int density_estimate_host(const void *buckets, int width, int height, int bits, int oversample, double k1,
+                                 double k2, double curve, int max_filtered_counts, int max_filter_index,
+                                 int kernel_size, int nfilters, const double *filter_widths,
+                                 const double *filter_coefs, void *accum, int exact) {
    std::lock_guard<std::recursive_mutex> lk(g_mu);
    if (ctx_init()) return 1;
    if (bits != 33 && bits != 64) { set_err("fl3h_density_estimate: bits must be 33 or 64"); return 1; }
@@ -1763,7 +1903,7 @@ extern "C" int fl3h_density_estimate(const void *buckets, int width, int height,
    p.k1 = k1;
    p.k2 = k2;
    p.curve = curve;
-   p.exact = 1;
+   p.exact = exact;
    if (max_filter_index > 0) {
       if (fl3d_density(&p, &de)) return 1;
    } else {

@@ -131,6 +131,11 @@ struct DEArgs {
    int max_filtered_counts, max_filter_index, kernel_size;
    int half;                // largest kernel radius (cells)
    double k1, k2, curve;
+   // split gather (sums without the reference's order): per 16x16 block of
+   // sources, the largest radius of its sources wider than DE_RS (0: none);
+   // nullptr: the one-pass gather
+   int *blkmax;
+   int nbx;                 // blocks per row
 };
 
 struct TMArgs {

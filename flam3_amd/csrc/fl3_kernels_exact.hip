@@ -167,16 +167,26 @@ __global__ __launch_bounds__(256) void k_deposit_points(const unsigned char *job
 __device__ __forceinline__ double bucket_value(const void *b, long i, float) { return (double)((const uint32_t *)b)[i]; }
 __device__ __forceinline__ double bucket_value(const void *b, long i, double) { return ((const double *)b)[i]; }
 
+// Sources of radius at most DE_RS are gathered from an LDS window around
+// each workgroup's region; wider ones (sparse cells) from per-region lists.
+#define DE_RS 3
+#define DE_RS_TAPS 10   // coefficients of a kernel with hi <= DE_RS: hi (hi + 1) / 2 + lo
+
+// Per 16x16 block of cells (one DE workgroup region): the largest radius of
+// its sources wider than DE_RS.  Lanes 16k..16k+15 of a wavefront hold
+// cells of one block (rows of 256 cells, blocks 16 wide).
+__device__ __forceinline__ void de_mark_block(const DEArgs &a, int i, int j, int r_large) {
+   int m = r_large;
+   for (int o = 1; o < 16; o <<= 1) m = max(m, __shfl_xor(m, o));
+   if ((threadIdx.x & 15) == 0 && m > 0) atomicMax(&a.blkmax[(j >> 4) * a.nbx + (i >> 4)], m);
+}
+
+// One cell of the DE preparation: its kernel index (DE_SKIP if it is no
+// source), per-source values and log term; returns the index.
 template <typename BT, typename AT>
-__device__ __forceinline__ void de_prep_body(const DEArgs &a, int bx, int by) {
-   const int os = a.oversample;
-   const int ss = os / 2;
-   const bool scf = !(os & 1);
-   const double scfact = pow(os / (os + 1.0), 2.0);
+__device__ __forceinline__ unsigned de_prep_cell(const DEArgs &a, int i, int j, int os, int ss, bool scf,
+                                                 double scfact) {
    const int W = a.width, H = a.height;
-   const int i = bx * blockDim.x + threadIdx.x;
-   const int j = by;
-   if (i >= W) return;
    const long cell = i + (long)j * W;
    uint16_t out = DE_SKIP;
    const BT *b = (const BT *)a.buckets;
@@ -206,7 +216,7 @@ __device__ __forceinline__ void de_prep_body(const DEArgs &a, int bx, int by) {
          // applied to the colour once; the kernel coefficient multiplies
          // it per tap
          const double lnum = a.k1 * log(1.0 + c3 * a.k2);
-         a.srcl[cell] = lnum;
+         if (a.exact) a.srcl[cell] = lnum;
          const double sc = lnum / c3;
          AT *v = (AT *)a.srcv + cell * 4;
          v[0] = (AT)((double)b[cell * 5 + 0] * sc);
@@ -216,6 +226,33 @@ __device__ __forceinline__ void de_prep_body(const DEArgs &a, int bx, int by) {
       }
    }
    a.findex[cell] = out;
+   return out;
+}
+
+template <typename BT, typename AT>
+__device__ __forceinline__ void de_prep_body(const DEArgs &a, int bx, int by) {
+   const int os = a.oversample;
+   const int ss = os / 2;
+   const bool scf = !(os & 1);
+   const double scfact = pow(os / (os + 1.0), 2.0);
+   const int W = a.width, H = a.height;
+   const int i = bx * blockDim.x + threadIdx.x;
+   const int j = by;
+   if (a.blkmax) {
+      // (every lane takes part in the block reduction)
+      int rl = 0;
+      if (i < W) {
+         const unsigned f = de_prep_cell<BT, AT>(a, i, j, os, ss, scf, scfact);
+         if (f != DE_SKIP) {
+            const int r = (int)ceil(a.widths[f]) - 1;
+            rl = r > DE_RS ? r : 0;
+         }
+      }
+      de_mark_block(a, i < W ? i : W - 1, j, rl);
+      return;
+   }
+   if (i >= W) return;
+   de_prep_cell<BT, AT>(a, i, j, os, ss, scf, scfact);
 }
 
 // Gather form: each thread owns one output cell and adds, in registers,
@@ -347,6 +384,172 @@ __device__ __forceinline__ void de_gather_body(const DEArgs &a, int bx, int by) 
       for (int c = 0; c < 4; c++) accp[c] = acc[c];
 }
 
+// Split gather (production sums, not in the reference's order): each
+// thread owns one output cell of a 16x16 region.  Part A: sources of radius
+// at most DE_RS -- nearly every cell of a dense image -- are read once per
+// region from a (16 + 2 DE_RS)^2 window staged in LDS, with their kernels'
+// first DE_RS_TAPS coefficients in LDS, so the taps need no lists and no
+// global loads.  Part B: wider sources are gathered from the 16x16 blocks
+// whose widest source (blkmax, from the preparation pass) reaches the
+// region, one block of 256 cells at a time, compacted into LDS and listed
+// per 8x8 quarter as in the one-pass gather.  Deterministic order: window
+// row-major, then blocks row-major.  The coefficient table of small kernels
+// takes (max_filter_index + 1) x DE_RS_TAPS entries of dynamic LDS.
+#define DS_W (DG_REGION + 2 * DE_RS)
+template <typename AT>
+__device__ __forceinline__ void de_gather_split_body(const DEArgs &a, int bx, int by) {
+   extern __shared__ unsigned char de_dyn[];
+   AT *cs = (AT *)de_dyn;                  // [(max_filter_index + 1) * DE_RS_TAPS]
+   typedef AT V4[4];
+   __shared__ V4 s_win[DS_W * DS_W];
+   __shared__ int s_wfr[DS_W * DS_W];      // small source: kernel index * 4 + radius; else -1
+   __shared__ int radius[1024];
+   __shared__ int4 s_src[DG_CHUNK];
+   __shared__ V4 s_val[DG_CHUNK];
+   __shared__ unsigned wave_cnt[DG_CHUNK / 64];
+   __shared__ unsigned short s_list[4][DG_CHUNK];
+   __shared__ unsigned list_cnt[4][DG_CHUNK / 64];
+   const int W = a.width, H = a.height, R = a.half;
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   const int rx0 = bx * DG_REGION, ry0 = by * DG_REGION;
+   const int rx1 = min(rx0 + DG_REGION, W) - 1, ry1 = min(ry0 + DG_REGION, H) - 1;
+   const int mx0 = rx0 + 8 * (wave & 1), my0 = ry0 + 8 * (wave >> 1);
+   const int lx = 8 * (wave & 1) + (lane & 7), ly = 8 * (wave >> 1) + (lane >> 3);
+   const int x = mx0 + (lane & 7), y = my0 + (lane >> 3);
+   const bool live = x < W && y < H;
+   AT *accp = (AT *)a.accum + (live ? (x + (long)y * W) * 4 : 0);
+   AT acc[4] = {(AT)0, (AT)0, (AT)0, (AT)0};
+   if (live)
+      for (int c = 0; c < 4; c++) acc[c] = accp[c];
+   const int nf = a.max_filter_index + 1;
+   for (int e = tid; e < nf && e < 1024; e += DG_CHUNK) radius[e] = (int)ceil(a.widths[e]) - 1;
+   const AT *coefs_t = (const AT *)a.coefs_t;
+   for (int e = tid; e < nf * DE_RS_TAPS; e += DG_CHUNK) {
+      const int f = e / DE_RS_TAPS, k = e - f * DE_RS_TAPS;
+      cs[e] = k < a.kernel_size ? coefs_t[(long)f * a.kernel_size + k] : (AT)0;
+   }
+   __syncthreads();
+   // ---- part A: the window of small sources
+   const AT *srcv = (const AT *)a.srcv;
+   for (int w = tid; w < DS_W * DS_W; w += DG_CHUNK) {
+      const int gx = rx0 - DE_RS + w % DS_W, gy = ry0 - DE_RS + w / DS_W;
+      int fr = -1;
+      if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+         const long cell = gx + (long)gy * W;
+         const unsigned f = a.findex[cell];
+         if (f != DE_SKIP) {
+            const int r = radius[f];
+            if (r <= DE_RS) {
+               fr = (int)f * 4 + r;
+               for (int c = 0; c < 4; c++) s_win[w][c] = srcv[cell * 4 + c];
+            }
+         }
+      }
+      s_wfr[w] = fr;
+   }
+   __syncthreads();
+#pragma unroll
+   for (int wy = -DE_RS; wy <= DE_RS; wy++) {
+#pragma unroll
+      for (int wx = -DE_RS; wx <= DE_RS; wx++) {
+         const int hi = max(wx < 0 ? -wx : wx, wy < 0 ? -wy : wy), lo = min(wx < 0 ? -wx : wx, wy < 0 ? -wy : wy);
+         const int w = (ly + DE_RS + wy) * DS_W + (lx + DE_RS + wx);
+         const int fr = s_wfr[w];
+         if (fr >= 0 && (fr & 3) >= hi) {
+            const AT c = cs[(fr >> 2) * DE_RS_TAPS + hi * (hi + 1) / 2 + lo];
+            if (c != (AT)0)
+               for (int k = 0; k < 4; k++) acc[k] += c * s_win[w][k];
+         }
+      }
+   }
+   // ---- part B: wider sources, from the blocks that can reach the region
+   const int nbx = a.nbx, nby = (H + DG_REGION - 1) / DG_REGION;
+   const int b0x = max(0, (rx0 - R) / DG_REGION), b1x = min(nbx - 1, (rx1 + R) / DG_REGION);
+   const int b0y = max(0, (ry0 - R) / DG_REGION), b1y = min(nby - 1, (ry1 + R) / DG_REGION);
+   for (int qy = b0y; qy <= b1y; qy++) {
+      for (int qx = b0x; qx <= b1x; qx++) {
+         const int bm = a.blkmax[qy * nbx + qx];
+         const int qx0 = qx * DG_REGION, qy0 = qy * DG_REGION;
+         const int gapx = max(max(rx0 - (qx0 + DG_REGION - 1), qx0 - rx1), 0);
+         const int gapy = max(max(ry0 - (qy0 + DG_REGION - 1), qy0 - ry1), 0);
+         if (bm == 0 || gapx > bm || gapy > bm) continue;   // (uniform over the workgroup)
+         const int sx = qx0 + (tid & 15), sy = qy0 + (tid >> 4);
+         int flag = 0, fi = 0;
+         if (sx < W && sy < H) {
+            const unsigned f = a.findex[sx + (long)sy * W];
+            if (f != DE_SKIP) {
+               fi = (int)f;
+               const int r = radius[fi];
+               const int gx = max(max(rx0 - sx, sx - rx1), 0), gy = max(max(ry0 - sy, sy - ry1), 0);
+               flag = r > DE_RS && gx <= r && gy <= r;
+            }
+         }
+         const unsigned long long m = __ballot(flag);
+         const unsigned rank = __popcll(m & ((1ull << lane) - 1ull));
+         if (lane == 0) wave_cnt[wave] = (unsigned)__popcll(m);
+         __syncthreads();
+         unsigned off = 0;
+         for (int w = 0; w < wave; w++) off += wave_cnt[w];
+         unsigned reach = 0;
+         if (flag) {
+            const int r = radius[fi];
+            for (int b = 0; b < 4; b++) {
+               const int bx0 = rx0 + 8 * (b & 1), by0 = ry0 + 8 * (b >> 1);
+               reach |= (unsigned)(sx + r >= bx0 && sx - r <= bx0 + 7 && sy + r >= by0 && sy - r <= by0 + 7) << b;
+            }
+         }
+         unsigned brank[4];
+         for (int b = 0; b < 4; b++) {
+            const unsigned long long mb = __ballot((reach >> b) & 1u);
+            brank[b] = (unsigned)__popcll(mb & ((1ull << lane) - 1ull));
+            if (lane == 0) list_cnt[b][wave] = (unsigned)__popcll(mb);
+         }
+         if (flag) {
+            const unsigned idx = off + rank;
+            const long cell = sx + (long)sy * W;
+            s_src[idx] = make_int4(sx, sy, fi * a.kernel_size, radius[fi]);
+            for (int c = 0; c < 4; c++) s_val[idx][c] = srcv[cell * 4 + c];
+         }
+         __syncthreads();
+         unsigned nb = 0;
+         for (int b = 0; b < 4; b++) {
+            unsigned boff = 0;
+            for (int w = 0; w < DG_CHUNK / 64; w++)
+               if (w < wave) boff += list_cnt[b][w];
+            if ((reach >> b) & 1u) s_list[b][boff + brank[b]] = (unsigned short)(off + rank);
+         }
+         for (int w = 0; w < DG_CHUNK / 64; w++) nb += list_cnt[wave][w];
+         __syncthreads();
+#pragma unroll 4
+         for (unsigned i = 0; i < nb; i++) {
+            const unsigned k = s_list[wave][i];
+            const int4 src = s_src[k];
+            const int dx = abs(x - src.x), dy = abs(y - src.y);
+            const int hi = max(dx, dy), lo = min(dx, dy);
+            const bool tap = live && hi <= src.w;
+            const long ci = tap ? (long)src.z + hi * (hi + 1) / 2 + lo : 0;
+            const AT coef = coefs_t[ci];
+            if (tap && coef != (AT)0)
+               for (int c = 0; c < 4; c++) acc[c] += coef * s_val[k][c];
+         }
+         __syncthreads();
+      }
+   }
+   if (live)
+      for (int c = 0; c < 4; c++) accp[c] = acc[c];
+}
+
+template <typename AT>
+__global__ __launch_bounds__(DG_CHUNK) void k_de_gather_split(DEArgs a) {
+   de_gather_split_body<AT>(a, blockIdx.x, blockIdx.y);
+}
+
+
+__global__ __launch_bounds__(256) void k_fill_i32(int *p, long n, int v) {
+   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) p[i] = v;
+}
+
 template <typename BT, typename AT>
 __global__ __launch_bounds__(256) void k_de_prep(DEArgs a) {
    de_prep_body<BT, AT>(a, blockIdx.x, blockIdx.y);
@@ -386,17 +589,40 @@ __global__ __launch_bounds__(DG_CHUNK) void k_de_gather_many(const DEArgs *as, c
    de_gather_body<AT, EXACT>(a, (int)(local % cols), (int)(local / cols));
 }
 
+template <typename AT>
+__global__ __launch_bounds__(DG_CHUNK) void k_de_gather_split_many(const DEArgs *as, const unsigned *first, int n) {
+   const int j = de_job(first, n, blockIdx.x);
+   const DEArgs a = as[j];
+   const unsigned local = blockIdx.x - first[j], cols = (unsigned)(a.width + DG_REGION - 1) / DG_REGION;
+   de_gather_split_body<AT>(a, (int)(local % cols), (int)(local / cols));
+}
+
 // Density estimation of n renders: per-render arguments in device memory,
 // workgroup prefixes prep_first[n + 1] (rows of 256 cells) and
 // gather_first[n + 1] (16x16 regions).  bits 32/33 with the FP32
 // accumulator (exact = 0) only.
+// With blkmax (a device array of nblk ints the renders' blkmax pointers
+// point into) the split gather runs, with split_lds bytes of dynamic LDS.
 extern "C" int fl3k_launch_de_many(const DEArgs *as, const unsigned *prep_first, unsigned prep_blocks,
-                                   const unsigned *gather_first, unsigned gather_blocks, int n, hipStream_t s) {
+                                   const unsigned *gather_first, unsigned gather_blocks, int n, hipStream_t s,
+                                   int *blkmax, long nblk, size_t split_lds) {
    if (n <= 0) return 0;
+   if (blkmax)
+      hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, s, blkmax, nblk, 0);
    hipLaunchKernelGGL((k_de_prep_many<uint32_t, float>), dim3(prep_blocks), dim3(256), 0, s, as, prep_first, n);
-   hipLaunchKernelGGL((k_de_gather_many<float, false>), dim3(gather_blocks), dim3(DG_CHUNK), 0, s, as, gather_first,
-                      n);
+   if (blkmax)
+      hipLaunchKernelGGL((k_de_gather_split_many<float>), dim3(gather_blocks), dim3(DG_CHUNK), split_lds, s, as,
+                         gather_first, n);
+   else
+      hipLaunchKernelGGL((k_de_gather_many<float, false>), dim3(gather_blocks), dim3(DG_CHUNK), 0, s, as,
+                         gather_first, n);
    return hipGetLastError() != hipSuccess;
+}
+
+// Dynamic LDS of the split gather: the first DE_RS_TAPS coefficients of
+// every kernel.
+extern "C" size_t fl3k_de_split_lds(int max_filter_index, int bits) {
+   return (size_t)(max_filter_index + 1) * DE_RS_TAPS * (bits == 64 ? sizeof(double) : sizeof(float));
 }
 
 // Log scaling of every cell without DE (rect.c:947-974).
@@ -540,6 +766,20 @@ extern "C" int fl3k_launch_deposit(const unsigned char *job, const double *pts, 
 extern "C" int fl3k_launch_de_tiled(const DEArgs *a, hipStream_t s, int bits) {
    const dim3 g1((a->width + 255) / 256, a->height), b1(256);
    const dim3 g3((a->width + DG_REGION - 1) / DG_REGION, (a->height + DG_REGION - 1) / DG_REGION);
+   if (a->blkmax && !a->exact) {
+      // split gather: blkmax cleared, marked by the preparation pass
+      const long nblk = (long)g3.x * g3.y;
+      const size_t dyn = fl3k_de_split_lds(a->max_filter_index, bits);
+      hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, s, a->blkmax, nblk, 0);
+      if (bits == 64) {
+         hipLaunchKernelGGL((k_de_prep<double, double>), g1, b1, 0, s, *a);
+         hipLaunchKernelGGL((k_de_gather_split<double>), g3, dim3(DG_CHUNK), dyn, s, *a);
+      } else {
+         hipLaunchKernelGGL((k_de_prep<uint32_t, float>), g1, b1, 0, s, *a);
+         hipLaunchKernelGGL((k_de_gather_split<float>), g3, dim3(DG_CHUNK), dyn, s, *a);
+      }
+      return hipGetLastError() != hipSuccess;
+   }
    if (bits == 64) {
       hipLaunchKernelGGL((k_de_prep<double, double>), g1, b1, 0, s, *a);
       hipLaunchKernelGGL((k_de_gather<double, true>), g3, dim3(DG_CHUNK), 0, s, *a);

@@ -391,7 +391,7 @@ static void plan_de_params(const render_plan *P, fl3d_de_params *dp) {
    plan_k1k2(P, &dp->k1, &dp->k2);
    dp->oversample = P->oversample;
    dp->curve = P->cp.estimator_curve;
-   dp->exact = P->replay || P->spec->bits == 64;
+   dp->exact = P->replay;
 }
 
 /* Tone-map parameters (rect.c:1067-1081). */
@@ -530,6 +530,9 @@ int fl3h_render(flam3_frame *spec, void *out, int field, int nchan, int transp, 
       int saturated = 0, exact = 0, pass;
 
       if (plan_batch_de(&P, batch_num, &de)) goto fail;
+      /* every batch starts from empty buckets; its density estimate adds to
+       * the accumulator (rect.c:709-721) */
+      if (batch_num > 0 && fl3d_clear_buckets()) goto devfail_de;
 
       /* Pass 0 uses fast deposits; if a hit counter saturated, pass 1
        * repeats the batch with saturating adds on every channel, reusing

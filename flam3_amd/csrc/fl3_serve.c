@@ -22,6 +22,7 @@
 #include <string.h>
 #include <time.h>
 #include "fl3_host.h"
+#include "fl3_device.h"
 
 typedef struct {
    fl3h_request req;
@@ -64,7 +65,12 @@ struct fl3h_server {
    fin_item *fin_head, *fin_tail;
    int runner_done;
    int slot_busy[2], next_slot;
+   double dev_avail;      /* device bytes renders may use (0: unknown) */
 };
+
+/* Fraction of the device memory a batch's histograms may take: the rest is
+ * for the record buffers of the binned deposit, the DE tables and images. */
+#define SRV_MEM_FRACTION 0.85
 
 static double now_s(void) {
    struct timespec t;
@@ -84,13 +90,11 @@ static double render_bytes(const flam3_frame *f) {
 
 static void *planner_main(void *arg) {
    fl3h_server *s = (fl3h_server *)arg;
-   /* default: the histograms one shared launch bins together (32e6 cells) */
-   const double budget = s->o.hist_budget_bytes > 0 ? s->o.hist_budget_bytes : 32e6 * 36.0;
    const int maxb = s->o.max_batch > 0 ? s->o.max_batch : 512;
    pthread_mutex_lock(&s->mu);
    for (;;) {
       long long t0, t1;
-      double bytes = 0.0, tp;
+      double bytes = 0.0, tp, budget, mem;
       int n, i;
       fl3h_request *reqs;
       long long *tickets;
@@ -98,11 +102,31 @@ static void *planner_main(void *arg) {
       while (!(s->next < s->nrecs && s->ready == NULL) && !(s->stop && s->next == s->nrecs))
          pthread_cond_wait(&s->cv_work, &s->mu);
       if (s->next == s->nrecs && s->stop) break;
-      /* admission in arrival order: the batch limit and the histogram budget
-       * (the first request always goes) */
+      /* Admission by histogram memory: a batch's histograms and accumulators
+       * fit the device memory the renders may use (refreshed after every
+       * batch), and by default one shared launch's cell space (32e6 cells):
+       * the binned deposit bins a shared group of at most that many cells,
+       * and groups of a batch run one after another on the same buffers.  A
+       * request that alone exceeds the device memory fails without running. */
+      mem = s->dev_avail > 0 ? SRV_MEM_FRACTION * s->dev_avail : 0.0;
+      budget = s->o.hist_budget_bytes > 0 ? s->o.hist_budget_bytes : 32e6 * 36.0;
+      if (mem > 0 && budget > mem) budget = mem;
+      while (s->next < s->nrecs && mem > 0 && s->recs[s->next]->bytes > mem) {
+         srv_rec *r = s->recs[s->next++];
+         fprintf(stderr, "flam3_amd: render server: a render needing %.3g GB of histograms exceeds the %.3g GB "
+                         "of device memory available\n", r->bytes / 1e9, mem / 1e9);
+         r->rc = 1;
+         r->state = 2;
+         r->t_done = now_s();
+         pthread_cond_broadcast(&s->cv_done);
+      }
+      if (s->next == s->nrecs) continue;
+      /* admission in arrival order: the batch limit and the budget (the
+       * first request always goes) */
       t0 = s->next;
       for (t1 = t0; t1 < s->nrecs && t1 - t0 < maxb; t1++) {
          if (t1 > t0 && bytes + s->recs[t1]->bytes > budget) break;
+         if (mem > 0 && s->recs[t1]->bytes > mem) break;
          bytes += s->recs[t1]->bytes;
          s->recs[t1]->state = 1;
       }
@@ -249,6 +273,14 @@ static void *runner_main(void *arg) {
          if (end) fl3_many_set_delivery(B, &dv);   /* else the images are copied in place */
          fl3_many_execute(B);
          te = now_s() - te;
+         {
+            double avail = 0.0;
+            if (fl3d_device_memory(&avail, NULL) == 0) {
+               pthread_mutex_lock(&s->mu);
+               s->dev_avail = avail;
+               pthread_mutex_unlock(&s->mu);
+            }
+         }
          if (end) {
             /* the finisher completes the batch after its image groups */
             end->B = B;
@@ -313,6 +345,7 @@ fl3h_server *fl3h_server_create(const fl3h_server_opts *o) {
    pthread_cond_init(&s->cv_done, NULL);
    pthread_cond_init(&s->cv_fin, NULL);
    pthread_cond_init(&s->cv_slot, NULL);
+   if (fl3d_device_memory(&s->dev_avail, NULL)) s->dev_avail = 0.0;
    if (pthread_create(&s->finisher, NULL, finisher_main, s)) goto bad;
    s->threads = 1;
    if (pthread_create(&s->planner, NULL, planner_main, s)) {

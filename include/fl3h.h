@@ -162,6 +162,16 @@ int fl3h_density_estimate(const void *buckets, int width, int height, int bits, 
                           int max_filter_index, int kernel_size, int nfilters,
                           const double *filter_widths, const double *filter_coefs, void *accum);
 
+/* The same stage with the production renders' arithmetic: the split gather
+ * (sources of radius <= 3 from an LDS window, wider ones from per-region
+ * lists), which adds each cell's contributions in another order than the
+ * reference, in the accumulator type.  No counterpart in the reference: it
+ * exposes the production path's density estimation to tests. */
+int fl3h_density_estimate_production(const void *buckets, int width, int height, int bits, int oversample,
+                                     double k1, double k2, double curve, int max_filtered_counts,
+                                     int max_filter_index, int kernel_size, int nfilters,
+                                     const double *filter_widths, const double *filter_coefs, void *accum);
+
 typedef struct {
    int image_width, image_height, oversample, de_offset, filter_width;
    int nchan, transp, bytes_per_channel, earlyclip;
@@ -176,6 +186,42 @@ typedef struct {
  * double for 64, uint32 for 32) into out (image_width*image_height*nchan*bpc). */
 int fl3h_tonemap(const void *accum, int width, int height, int bits, const fl3h_tonemap_params *tp,
                  const double *filter, void *out);
+
+/* Replaces flam3_estimate_bounding_box (flam3.h:588, flam3.c:3574-3639):
+ * one FP64 chaos-game trajectory of nsamples points (fl3h_iterate, start
+ * drawn from rc, 20 fused steps), then the eps / 1 - eps quantiles of x and
+ * y (eps raised to 3x the bad-value rate, at most 0.3; the extremes when
+ * eps * nsamples < 1).  rc advances as the reference advances it.  Returns
+ * the number of bad values, or -1.  The genome-search preview loop
+ * (flam3-genome.c:1033) calls it once per candidate genome. */
+int fl3h_estimate_bounding_box(flam3_genome *cp, double eps, int nsamples, double *bmin, double *bmax,
+                               randctx *rc);
+
+/* Replaces flam3_improve_colors (flam3.h:586, palettes.c:476-504) for
+ * change_palette = 0: the genome's colouring and ntries random recolourings
+ * (the reference's sequence of random() calls) are scored by try_colors'
+ * ~10000-pixel trial images (palettes.c:362-456, including its integer
+ * score hits / res^3), all rendered in one fl3h_render_many call; g keeps
+ * the first best colouring.  Palette changes (change_palette != 0) need the
+ * palette library and return 1.  Returns 0, or 1 on failure. */
+int fl3h_improve_colors(flam3_genome *g, int ntries, int change_palette, int color_resolution);
+
+/* Genome ingestion on a host thread pool (no counterpart in the reference;
+ * for a render server's or preview loop's input): XML documents parsed by
+ * the caller's parser, with flam3_parse_xml2's signature (flam3.h:581,
+ * parser.c:220-1256), on nthreads threads (0: 8).  The first document is
+ * parsed alone (the parser initialises process state lazily), the rest in
+ * parallel.  Each genome is checked for what rendering relies on (final
+ * xform last, parser.c:802-821; positive sizes, oversampling, batches,
+ * temporal samples and quality; xaos rows).  fl3h_ingest_wait hands over
+ * the parser's genome array (the caller frees it as after
+ * flam3_parse_xml2) and returns 0, or 1 when parsing or the checks failed. */
+typedef flam3_genome *(*fl3h_parse_fn)(char *xml, char *filename, int default_flag, int *ncps);
+typedef struct fl3h_ingest fl3h_ingest;
+fl3h_ingest *fl3h_ingest_create(fl3h_parse_fn parse, int nthreads, int default_flag);
+long long fl3h_ingest_submit(fl3h_ingest *q, const char *xml, const char *filename);
+int fl3h_ingest_wait(fl3h_ingest *q, long long ticket, flam3_genome **genomes, int *ncps);
+void fl3h_ingest_destroy(fl3h_ingest *q);
 
 /* Seed f->rc from an explicit string the way flam3_init_frame seeds it from
  * the isaac_seed environment variable (flam3.h:639, flam3.c:2494-2516);
@@ -199,6 +245,20 @@ int fl3h_interpolate(const flam3_genome *genomes, int ngenomes, double time, dou
 /* Release the xforms and xaos rows of a genome filled by fl3h_interpolate
  * (the struct itself stays the caller's). */
 void fl3h_genome_clear(flam3_genome *g);
+
+/* Launch tuning (no counterpart in the reference): the parameters the
+ * autotuner (flam3_amd/autotune.py) chooses, loaded once per process from
+ * the file named by the FL3H_TUNING environment variable, else from
+ * tuning.conf beside the library ("key value" lines), else defaults.
+ * Keys: wg_per_cu (walker workgroups of 256 per CU, default 8), min_samples
+ * (samples per walker at least, 256), chunk_samples (binned deposit chunk,
+ * 2^28), de_split (production density estimation: split gather 1, one pass
+ * 0).  With value non-NULL the key is set for the rest of the process; out
+ * (may be NULL) receives the current value.  Returns 0, or 1 for an unknown
+ * key or a value out of range.  The walker parameters shape the sample
+ * streams: renders made under different settings are different samples of
+ * the same image. */
+int fl3h_tuning(const char *key, const double *value, double *out);
 
 /* Stage timing (no counterpart in the reference): with profiling enabled
  * every kernel launch of the render path is bracketed by HIP events on the

@@ -43,7 +43,26 @@ REPLAY_CASES = [
     ("bits64_rgba16", lambda: G.synthetic_small(seed=65, width=48, height=36, quality=10),
      dict(nthreads=2, bits=64, bpc=2, nchan=4, transp=1)),
     ("earlyclip", lambda: G.synthetic_small(seed=66, width=48, height=36, quality=10), dict(nthreads=2, earlyclip=1)),
+    # interlaced fields (flam3_field_even = 1, flam3_field_odd = 2): half the
+    # histogram rows, the camera shifted by half a row, every other output
+    # row written (rect.c:616-624, rect.c:800-807)
+    ("field_even", lambda: G.synthetic_small(seed=67, width=64, height=48, quality=10), dict(nthreads=1, field=1)),
+    ("field_odd", lambda: G.synthetic_small(seed=68, width=64, height=48, quality=10), dict(nthreads=1, field=2)),
+    # a rotated camera: points turn about rot_center before the bounds test
+    # (rect.c:423-431, rect.c:813-827)
+    ("rotated", lambda: _with(G.synthetic_small(seed=69, width=64, height=48, quality=10), rotate=33.0),
+     dict(nthreads=1)),
+    # two batches (passes): k1 and k2 carry 1/nbatches and nbatches
+    # (rect.c:933-937), DE runs per batch into one accumulator
+    ("nbatches2", lambda: _with(G.synthetic_small(seed=70, width=64, height=48, quality=10), passes=2),
+     dict(nthreads=1)),
 ]
+
+
+def _with(g, **kw):
+    for k, v in kw.items():
+        setattr(g, k, v)
+    return g
 
 
 @pytest.mark.parametrize("name,mk,kw", REPLAY_CASES, ids=[c[0] for c in REPLAY_CASES])
@@ -61,6 +80,10 @@ def test_replay_render_reproduces_reference(prod, ref, name, mk, kw):
     got, gs = prod.render(g, seed="replay", replay=True, **kw)
     lsb = 256 if kw.get("bpc", 1) == 2 else 1
     rows = g.height - int(np.ceil(g.estimator_radius)) - 1 if g.supersample == 1 else g.height
+    if kw.get("field"):
+        # a field's histogram has half the rows: the unread row past its end
+        # reaches twice as many image rows
+        rows = g.height - 2 * (int(np.ceil(g.estimator_radius)) + 1) if g.supersample == 1 else g.height
     # With several threads the reference's own images differ from run to run
     # (up to 9% of 16-bit channels for bits64_rgba16 over 12 runs), so the
     # GPU image is compared with the nearest of up to four reference runs.
@@ -100,9 +123,9 @@ def test_replay_render_without_de_matches(prod, ref, name, mk, kw):
 
 
 @pytest.mark.parametrize("mk,kw", [
-    (lambda: G.synthetic_small(seed=71, width=128, height=96, quality=40), {}),
-    (lambda: G.synthetic_1080p(seed=72, width=128, height=72, quality=40, supersample=2), {}),
-    (lambda: G.synthetic_8k(seed=73, width=96, height=54, quality=30, supersample=3), dict(bits=64)),
+    (lambda: G.synthetic_small(seed=71, width=256, height=192, quality=40), {}),
+    (lambda: G.synthetic_1080p(seed=72, width=256, height=144, quality=40, supersample=2), {}),
+    (lambda: G.synthetic_8k(seed=73, width=192, height=108, quality=30, supersample=3), dict(bits=64)),
 ], ids=["small", "ss2_final", "xaos_bits64"])
 def test_production_render_within_noise_floor(prod, ref, mk, kw):
     """FP32 production render vs two reference renders of the same genome
@@ -113,17 +136,23 @@ def test_production_render_within_noise_floor(prod, ref, mk, kw):
     target)."""
     g = mk()
     xml = G.flame_document([g])
-    a, _ = ref.render(xml, seed="seed-a", nthreads=8, kind="fast", **kw)
-    b, _ = ref.render(xml, seed="seed-b", nthreads=8, kind="fast", **kw)
+    # three reference seeds: with density estimation the pixels of an image
+    # this small are few independent samples, and the RMS of a single pair
+    # of seeds varies by ~10% (measured: 1.72 to 1.90 for xaos_bits64)
+    refs = [ref.render(xml, seed="seed-%s" % k, nthreads=8, kind="fast", **kw)[0] for k in "abc"]
     got, st = prod.render(g, seed="gpu", **kw)
-    floor = rms(a, b)
-    err = 0.5 * (rms(got, a) + rms(got, b))
-    assert err <= 1.15 * floor, (err, floor)
-    # brightness: two reference seeds can agree closely by chance; the
-    # seed-to-seed spread of the mean is ~1.5% for these small images (measured
-    # over 8 GPU seeds of xaos_bits64: sd 0.2 at a mean of 13.3)
-    m = 0.5 * (a.mean() + b.mean())
-    assert abs(got.mean() - m) <= 3 * abs(a.mean() - b.mean()) + 0.03 * m + 0.05, (got.mean(), a.mean(), b.mean())
+    floor = np.mean([rms(refs[i], refs[j]) for i in range(3) for j in range(i + 1, 3)])
+    err = np.mean([rms(got, r) for r in refs])
+    means = [r.mean() for r in refs]
+    print("noise floor: gpu %.4f reference %.4f ratio %.4f; mean %.3f vs %s"
+          % (err, floor, err / floor, got.mean(), np.round(means, 3)), flush=True)
+    assert err <= 1.10 * floor, (err, floor)
+    # brightness within three times the references' spread, plus 2% (the
+    # seed-to-seed spread of the mean measured over 8 GPU seeds of
+    # xaos_bits64 at 96x54: sd 0.2 at a mean of 13.3)
+    m = float(np.mean(means))
+    assert abs(got.mean() - m) <= 3 * (max(means) - min(means)) + 0.02 * m + 0.05, (got.mean(), means)
+    a, b = refs[0], refs[1]
     g.estimator_radius = 0.0
     hq = mk()
     hq.estimator_radius = 0.0
@@ -132,11 +161,12 @@ def test_production_render_within_noise_floor(prod, ref, mk, kw):
     a0, _ = ref.render(G.flame_document([g]), seed="seed-a", nthreads=8, kind="fast", **kw)
     b0, _ = ref.render(G.flame_document([g]), seed="seed-b", nthreads=8, kind="fast", **kw)
     got0, _ = prod.render(g, seed="gpu", **kw)
+    print("16x quality: gpu %.4f reference %.4f" % (rms(got0, ref_hq), rms(a0, ref_hq)), flush=True)
     assert rms(got0, ref_hq) <= 1.1 * rms(a0, ref_hq), (rms(got0, ref_hq), rms(a0, ref_hq))
     # the log density is concave, so a low-quality image is darker than a
     # high-quality one on average: compare brightness at the same quality
     m0 = 0.5 * (a0.mean() + b0.mean())
-    assert abs(got0.mean() - m0) <= 3 * abs(a0.mean() - b0.mean()) + 0.03 * m0 + 0.05, (got0.mean(), a0.mean(),
+    assert abs(got0.mean() - m0) <= 3 * abs(a0.mean() - b0.mean()) + 0.02 * m0 + 0.05, (got0.mean(), a0.mean(),
                                                                                          b0.mean())
 
 
@@ -259,6 +289,31 @@ def test_binned_deposit_equals_atomic_deposit(prod, name, mk, kw, walkers, chunk
     assert a.mean() > 1.0
 
 
+def test_bad_values_follow_reference(prod, ref):
+    """A genome whose third xform overflows every time it is applied: a
+    third of all steps give a bad value, which the reference retries from a
+    random point (flam3.c:257-270).  The production walkers retry the same
+    way inside the step, so the plotted count, the image and its brightness
+    follow the reference's (noise floor and mean brightness against two
+    reference seeds)."""
+    g = _with_blowup(G.synthetic_small(seed=93, width=256, height=192, quality=30))
+    xml = G.flame_document([g])
+    a, sa = ref.render(xml, seed="seed-a", nthreads=8, kind="fast")
+    b, _ = ref.render(xml, seed="seed-b", nthreads=8, kind="fast")
+    got, st = prod.render(g, seed="gpu-bad", deposit=2)
+    floor = rms(a, b)
+    err = 0.5 * (rms(got, a) + rms(got, b))
+    m = 0.5 * (a.mean() + b.mean())
+    print("bad values: gpu %d, reference %d of %d; rms ratio %.4f; mean %.3f vs %.3f/%.3f"
+          % (st["badvals"], sa["badvals"], st["num_iters"], err / floor, got.mean(), a.mean(), b.mean()), flush=True)
+    # both count the bad values of their unplotted fuse steps too, and the
+    # walkers fuse 100 steps per walker against the reference's 15 per
+    # 10000-sample sub-batch
+    assert 0.1 * st["num_iters"] < sa["badvals"] <= st["badvals"] <= 1.5 * sa["badvals"], (st, sa)
+    assert err <= 1.10 * floor, (err, floor)
+    assert abs(got.mean() - m) <= 3 * abs(a.mean() - b.mean()) + 0.02 * m + 0.05, (got.mean(), a.mean(), b.mean())
+
+
 def test_binned_deposit_with_density_estimation(prod):
     g = G.synthetic_1080p(seed=85, width=192, height=108, quality=25, supersample=2)
     a, _ = prod.render(g, seed="bin-de", nwalkers=1024, deposit=1)
@@ -298,13 +353,20 @@ def _animated_pair(seed, de):
     return [a, b]
 
 
-@pytest.mark.parametrize("t", [0.0, 0.37, 1.0])
-def test_replay_animation_frame_without_de(prod, ref, t):
+@pytest.mark.parametrize("t,tfilter", [(0.0, "box"), (0.37, "box"), (1.0, "box"), (0.37, "gaussian"),
+                                       (0.61, "exp")])
+def test_replay_animation_frame_without_de(prod, ref, t, tfilter):
     """A frame between two control points with motion blur (4 temporal
     samples): every temporal sample interpolates the genome
     (rect.c:754-770, flam3_interpolate), aligned xform counts differ.  Same
-    criterion as the still replay without DE."""
+    criterion as the still replay without DE.  Gaussian and exponential
+    temporal filters weight each sample's palette differently and change
+    the filter sum in k2 (filters.c:409-489, rect.c:778-790)."""
     gs = _animated_pair(90, 0.0)
+    for g in gs:
+        g.temporal_filter_type = tfilter
+        g.temporal_filter_width = 1.4
+        g.temporal_filter_exp = 0.8 if tfilter == "exp" else 0.0
     want, ws = ref.render(G.flame_document(gs), seed="anim", time=t, nthreads=2)
     got, st = prod.render(gs, seed="anim", time=t, nthreads=2, replay=True)
     assert st["num_iters"] == ws["num_iters"]
@@ -364,12 +426,15 @@ def test_production_animation_frame_within_noise_floor(prod, ref):
     gs = _animated_pair(92, 9.0)
     for g in gs:
         g.quality = 40
+        g.width, g.height, g.scale = 192, 144, g.scale * 3
     xml = G.flame_document(gs)
     a, _ = ref.render(xml, seed="seed-a", time=0.6, nthreads=8, kind="fast")
     b, _ = ref.render(xml, seed="seed-b", time=0.6, nthreads=8, kind="fast")
     got, _ = prod.render(gs, seed="gpu", time=0.6)
     floor = rms(a, b)
-    assert 0.5 * (rms(got, a) + rms(got, b)) <= 1.15 * floor
+    err = 0.5 * (rms(got, a) + rms(got, b))
+    print("animation noise floor: gpu %.4f reference %.4f ratio %.4f" % (err, floor, err / floor), flush=True)
+    assert err <= 1.10 * floor
 
 
 def test_8k_bits64_geometry(prod):

@@ -6,6 +6,8 @@ over 16 steps (the first refill of the walkers' ISAAC streams falls inside
 that window); the FP64 tone map within 1 LSB at 8 bits per channel.
 """
 import ctypes
+import json
+import os
 
 import numpy as np
 import pytest
@@ -13,6 +15,8 @@ import pytest
 from flam3_amd import abi
 from flam3_amd import genome as G
 from flam3_amd._lib import fl3h_camera, fl3h_tonemap_params
+
+import fp32_variation_error as E
 
 pytestmark = pytest.mark.gpu
 
@@ -27,56 +31,59 @@ def _prepared(ref, g):
 
 # ---------------------------------------------------------------- iteration
 
-@pytest.mark.parametrize("mk,steps,tol", [
-    (lambda: G.synthetic_small(seed=51, width=64, height=64), 16, 1e-9),
-    (lambda: G.synthetic_8k(seed=53, width=64, height=64), 16, 1e-9),
-    (lambda: G.synthetic_1080p(seed=52, width=64, height=64), 8, 1e-9),
+@pytest.mark.parametrize("mk", [
+    lambda: G.synthetic_small(seed=51, width=64, height=64),
+    lambda: G.synthetic_8k(seed=53, width=64, height=64),
+    lambda: G.synthetic_1080p(seed=52, width=64, height=64),
 ], ids=["contractive", "xaos_final", "expanding"])
-def test_iterate_fp64_replay(prod, oracle, ref, mk, steps, tol):
+def test_iterate_fp64_replay(prod, oracle, ref, mk):
     """fl3h_iterate (FP64, device ISAAC) vs or_iterate with the same ISAAC
-    contexts and start points.  The expanding genome amplifies the ulp-level
-    libm differences by ~10x per step, so its window is 8 steps; every
-    walker's RNG state must match exactly at the end either way."""
+    contexts and start points: within 1e-9 relative at each of 16 steps,
+    and every walker's RNG state exactly equal at the end.  The GPU's FP64
+    libm differs from glibc by an ulp now and then, and a chaotic map grows
+    such a difference by its per-step gain; the printout gives, beside the
+    error, the divergence of the oracle itself from start points moved by
+    one ulp.  Measured: GPU error at most 3.4e-11 over 16 steps, while one
+    ulp moved at the start grows to 3e-9 for the xaos genome -- the window
+    holds because the GPU's libm agrees with glibc on nearly every call."""
     p, dist = _prepared(ref, mk())
     try:
-        W, n = 64, steps
+        W, n = 64, 16
         rng = np.random.default_rng(0)
         starts = rng.uniform(-1, 1, (W, 4))
         starts[:, 2:] = np.abs(starts[:, 2:])
         want = np.zeros((W, 4 * n))
+        nudged = np.zeros((W, 4 * n))
         rc_o = (abi.randctx * W)()
+        rc_n = (abi.randctx * W)()
         rc_g = (abi.randctx * W)()
         for w in range(W):
             rc_o[w] = oracle.seeded_rc("walker%d" % w)
+            rc_n[w] = oracle.seeded_rc("walker%d" % w)
             rc_g[w] = oracle.seeded_rc("walker%d" % w)
             want[w, :4] = starts[w]
+            nudged[w, :4] = starts[w]
+            nudged[w, :2] = np.nextafter(starts[w, :2], np.inf)
             oracle.lib().or_iterate(p.ptr, n, 0, want[w].ctypes.data_as(ctypes.c_void_p), dist,
                                     ctypes.byref(rc_o[w]))
+            oracle.lib().or_iterate(p.ptr, n, 0, nudged[w].ctypes.data_as(ctypes.c_void_p), dist,
+                                    ctypes.byref(rc_n[w]))
         got = np.zeros((W, 4 * n))
         got[:, :4] = starts
         bad = np.zeros(W, dtype=np.int32)
         assert prod.lib().fl3h_iterate(p.ptr, n, 0, got.ctypes.data_as(ctypes.c_void_p), dist, rc_g, W,
                                        bad.ctypes.data_as(ctypes.c_void_p)) == 0, prod.last_error()
-        rel = np.abs(got - want) / np.maximum(1.0, np.abs(want))
-        assert rel.max() <= tol, rel.reshape(W, n, 4).max(axis=(0, 2))
+        scale = np.maximum(1.0, np.abs(want))
+        rel = (np.abs(got - want) / scale).reshape(W, n, 4).max(axis=(0, 2))
+        ulp = (np.abs(nudged - want) / scale).reshape(W, n, 4).max(axis=(0, 2))
+        print("per-step max relative error: %s\none-ulp divergence of the oracle: %s"
+              % (np.array2string(rel, precision=1), np.array2string(ulp, precision=1)), flush=True)
+        assert (rel <= 1e-9).all(), (rel, ulp)
         for w in range(W):
             assert list(rc_o[w].randrsl) == list(rc_g[w].randrsl)
             assert rc_o[w].randcnt == rc_g[w].randcnt
     finally:
         p.close()
-
-
-def _variation_genome(vi):
-    rng = np.random.default_rng(1000 + vi)
-    name = abi.VARIATION_NAMES[vi]
-    x = G.Xform(weight=1.0, color=0.5, coefs=tuple(rng.uniform(-1, 1, 6)),
-                variations={name: float(rng.uniform(0.3, 1.2))})
-    if name == "pre_blur":
-        x.variations["linear"] = 1.0
-    for f, _ in abi.flam3_xform._fields_:
-        if f.startswith(name.split("_")[0] + "_"):
-            x.params[f] = float(rng.uniform(0.2, 2.0))
-    return G.Genome(width=32, height=32, xforms=[x])
 
 
 # Variations with a discontinuity (floor/fmod/rint/int cast or a branch on
@@ -93,7 +100,7 @@ def test_variation_fp64(prod, oracle, vi):
     """fl3h_xform_preview (one xform, 3 applications on a 13x13 grid) vs the
     oracle: within 1e-9 relative; for variations with discontinuities at
     most 3 of 169 points may land on the other side of an edge."""
-    g = _variation_genome(vi)
+    g = E.variation_genome(vi)
     name = abi.VARIATION_NAMES[vi]
     nv, depth = 6, 3
     gp = g.to_struct()
@@ -129,56 +136,26 @@ def test_variation_fp64(prod, oracle, vi):
     assert list(rc.randrsl) == list(rco.randrsl) and rc.randcnt == rco.randcnt
 
 
-# Variations that draw random numbers (variations.c: the ones that call
-# flam3_random_isaac_*): FP32 walkers use their own streams, so these are
-# compared by the moments of many draws instead of point by point.
-RANDOM_VARS = {"julia", "noise", "julian", "juliascope", "blur", "gaussian_blur", "radial_blur", "pie", "arch",
-               "square", "rays", "blade", "twintrian", "super_shape", "flower", "conic", "parabola", "boarders",
-               "cpow", "pre_blur", "wedge_julia"}
-
-
-def _apply_oracle(oracle, g, pts, seed):
-    import pyref
-    out = np.zeros_like(pts)
-    with pyref.Parsed(G.flame_document([g])) as p:
-        pyref.ref().prepare_precalc_flags(p.ptr)
-        pyref.ref().xform_precalc(p.ptr, 0)
-        rco = oracle.seeded_rc(seed)
-        q = np.zeros(4)
-        for i in range(len(pts)):
-            pt = np.ascontiguousarray(pts[i])
-            oracle.lib().or_apply_xform(p.ptr, 0, pt.ctypes.data_as(ctypes.c_void_p),
-                                        q.ctypes.data_as(ctypes.c_void_p), ctypes.byref(rco))
-            out[i] = q
-    return out
-
-
-def _apply_fp32(prod, g, pts, tier=-1):
-    out = np.zeros_like(pts)
-    gp = g.to_struct()
-    try:
-        assert prod.lib().fl3h_apply_xform_fp32(gp, 0, pts.ctypes.data_as(ctypes.c_void_p), len(pts),
-                                                out.ctypes.data_as(ctypes.c_void_p), tier) == 0, prod.last_error()
-    finally:
-        G.free_struct(gp)
-    return out
+FP32_BOUNDS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fp32_variation_bounds.json")
 
 
 @pytest.mark.parametrize("vi", range(99))
 def test_variation_fp32_walker(prod, oracle, vi):
     """The production walkers' FP32 xform step, in the kernel tier the
     genome selects, against the FP64 oracle.  Deterministic variations: on a
-    13x13 grid at least 95% of the points within 2e-3 relative (FP32 with
-    the hardware sin/cos/exp2/log2 and approximate division; points next to
-    a pole or a discontinuity may move by more).  Random variations:
-    the same distribution of 4000 draws from one start point (two-sample
-    Kolmogorov-Smirnov, p > 1e-4)."""
-    g = _variation_genome(vi)
+    41x41 grid at depth 1, leaving out only points where the FP64 map is
+    ill-conditioned (local gain above 1e3, fp32_variation_error.py), the
+    p99 and the maximum of the relative error and the fraction of finite
+    results stay within the per-variation bounds of
+    golden/fp32_variation_bounds.json (measured, with margin).  Random
+    variations: the same distribution of 4000 draws from one start point
+    (two-sample Kolmogorov-Smirnov, p > 1e-4)."""
+    g = E.variation_genome(vi)
     name = abi.VARIATION_NAMES[vi]
-    if name in RANDOM_VARS:
+    if name in E.RANDOM_VARS:
         pts = np.tile(np.array([[0.3, -0.4, 0.5, 0.0]]), (4000, 1))
-        got = _apply_fp32(prod, g, pts)
-        want = _apply_oracle(oracle, g, pts, "fp32-%d" % vi)
+        got = E.apply_fp32(prod, g, pts)
+        want = E.apply_oracle(oracle, g, pts, "fp32-%d" % vi)
         ok = (got[:, 3] == 0) & np.isfinite(want[:, :2]).all(axis=1)
         assert ok.mean() > 0.9, name
         from scipy.stats import ks_2samp
@@ -191,16 +168,14 @@ def test_variation_fp32_walker(prod, oracle, vi):
             assert ks_2samp(a, b).pvalue > 1e-4, (name, k, np.percentile(a, [10, 50, 90]),
                                                   np.percentile(b, [10, 50, 90]))
         return
-    nv = 6
-    xs = np.linspace(-2.0, 2.0, 2 * nv + 1)
-    pts = np.array([[x, y, 0.5, 0.0] for x in xs for y in xs])
-    got = _apply_fp32(prod, g, pts)
-    want = _apply_oracle(oracle, g, pts, "fp32-%d" % vi)
-    ok = (got[:, 3] == 0) & np.isfinite(want[:, :2]).all(axis=1) & (np.abs(want[:, :2]) < 1e9).all(axis=1)
-    rel = (np.abs(got[:, :2] - want[:, :2]) / np.maximum(1.0, np.abs(want[:, :2]))).max(axis=1)
-    assert ok.mean() >= 0.9, (name, ok.mean())
-    assert (rel[ok] <= 2e-3).mean() >= 0.95, (name, np.sort(rel[ok])[-10:])
-    assert np.allclose(got[ok, 2], want[ok, 2], rtol=1e-6, atol=1e-6)
+    with open(FP32_BOUNDS) as fh:
+        bound = json.load(fh)["variations"][name]
+    m = E.measure(prod, oracle, vi)
+    assert m["used"] >= 0.5 * m["points"], m
+    assert m["p99"] <= bound["bound_p99"], (name, m, bound["bound_p99"])
+    assert m["max"] <= bound["bound_max"], (name, m, bound["bound_max"])
+    assert m["finite"] >= bound["bound_finite"], (name, m, bound["bound_finite"])
+    assert m["colour_max"] <= 1e-6, (name, m)
 
 
 def test_variation_tiers_agree_with_full_kernel(prod):
@@ -208,9 +183,9 @@ def test_variation_tiers_agree_with_full_kernel(prod):
     rng = np.random.default_rng(5)
     pts = np.column_stack([rng.uniform(-2, 2, 500), rng.uniform(-2, 2, 500), np.full(500, 0.5), np.zeros(500)])
     for vi in (3, 16, 42, 55, 70, 92):
-        g = _variation_genome(vi)
-        a = _apply_fp32(prod, g, pts, tier=-1)
-        b = _apply_fp32(prod, g, pts, tier=6)
+        g = E.variation_genome(vi)
+        a = E.apply_fp32(prod, g, pts, tier=-1)
+        b = E.apply_fp32(prod, g, pts, tier=6)
         ok = (a[:, 3] == 0) & (b[:, 3] == 0)
         rel = np.abs(a[ok, :2] - b[ok, :2]) / np.maximum(1.0, np.abs(b[ok, :2]))
         assert (rel <= 1e-5).mean() >= 0.99, (vi, rel.max())
@@ -367,6 +342,42 @@ def test_density_estimation(prod, oracle, ref, bits, ss, rad):
                                             got.ctypes.data_as(ctypes.c_void_p)) == 0, prod.last_error()
     tol = 1e-12 if bits == 64 else 2e-6
     np.testing.assert_allclose(got, want, rtol=tol, atol=tol * float(np.abs(want).max()))
+
+
+@pytest.mark.parametrize("bits,ss,rad,density", [(33, 1, 9.0, "mixed"), (33, 2, 9.0, "mixed"), (33, 3, 5.0, "mixed"),
+                                                  (64, 2, 7.0, "mixed"), (33, 2, 9.0, "dense"), (33, 1, 9.0, "sparse"),
+                                                  (64, 3, 15.0, "sparse")])
+def test_density_estimation_production(prod, oracle, ref, bits, ss, rad, density):
+    """The production renders' density estimation (split gather: small
+    kernels from an LDS window, wide ones from per-region lists, summed in
+    another order than the reference) against the oracle.  Sums of positive
+    terms in another order: within 2e-5 relative in FP32, 1e-12 in FP64."""
+    rng = np.random.default_rng(19 + ss)
+    W, H = 170, 110
+    if density == "mixed":
+        b = _buckets(rng, W, H, bits)
+    else:
+        lam = 300.0 if density == "dense" else 0.3
+        hits = rng.poisson(lam, (H, W)).astype(np.float64)
+        b = np.zeros((H, W, 5))
+        b[:, :, 4] = b[:, :, 3] = hits * 255
+        for k in range(3):
+            b[:, :, k] = np.floor(hits * rng.uniform(0, 255, (H, W)))
+        b = b if bits == 64 else b.astype(np.uint32)
+    mfc, mfi, ks, wp, cp_ = _de_tables(ref, rad, 0.0, 0.4, ss)
+    adt = np.float64 if bits == 64 else np.float32
+    start = rng.uniform(0.0, 2.0, (H, W, 4)).astype(adt)
+    want = start.copy()
+    k1, k2 = 4.0 * 255 * 268 / 256, 1.0 / 2000.0
+    oracle.lib().or_density(b.ctypes.data_as(ctypes.c_void_p), W, H, bits, ss, k1, k2, 0.4, mfc, mfi, ks, wp, cp_,
+                            want.ctypes.data_as(ctypes.c_void_p), 0)
+    got = start.copy()
+    assert prod.lib().fl3h_density_estimate_production(
+        b.ctypes.data_as(ctypes.c_void_p), W, H, bits, ss, k1, k2, 0.4, mfc, mfi, ks, mfi + 1, wp, cp_,
+        got.ctypes.data_as(ctypes.c_void_p)) == 0, prod.last_error()
+    tol = 1e-12 if bits == 64 else 2e-5
+    np.testing.assert_allclose(got, want, rtol=tol, atol=tol * float(np.abs(want).max()))
+    assert np.abs(want - start).max() > 1.0
 
 
 def _tm_params(W, H, ss, fw, **kw):
