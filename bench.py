@@ -120,7 +120,7 @@ class fl3h_stage_times(ctypes.Structure):
     _fields_ = [("iterate_ms", ctypes.c_double), ("density_ms", ctypes.c_double), ("tonemap_ms", ctypes.c_double),
                 ("iterate_launches", ctypes.c_longlong), ("density_launches", ctypes.c_longlong),
                 ("tonemap_launches", ctypes.c_longlong), ("samples", ctypes.c_longlong), ("cells", ctypes.c_longlong),
-                ("bin_ms", ctypes.c_double), ("bin_launches", ctypes.c_longlong)]
+                ("bin_ms", ctypes.c_double), ("bin_launches", ctypes.c_longlong), ("de_taps", ctypes.c_longlong)]
 
 
 def gpu_bench(args):
@@ -462,6 +462,13 @@ def main():
             "bin_GBps": (per_launch * BIN_BYTES_PER_SAMPLE / (bin_ms * 1e-3) / 1e9) if bin_ms > 0 else None,
             "density_ms": de_ms,
             "density_GBps": (st.cells * DE_BYTES_PER_CELL / (de_ms * 1e-3) / 1e9) if de_ms > 0 else None,
+            "density_hbm_frac": (st.cells * DE_BYTES_PER_CELL / (de_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if de_ms > 0
+            else None,
+            # the gather's work: each tap of a source's (2r+1)^2 footprint is
+            # 4 FP32 multiply-adds (8 flops) into the output cell
+            "density_taps": st.de_taps / max(1, st.density_launches),
+            "density_tap_fp32_frac": (st.de_taps / max(1, st.density_launches) * 8.0 / (de_ms * 1e-3)
+                                      / (FP32_PEAK_TFLOPS * 1e12)) if de_ms > 0 else None,
             "tonemap_ms": st.tonemap_ms / max(1, st.tonemap_launches)},
     }
     if not args.no_animation:

@@ -9,7 +9,7 @@ tuning.conf beside libflam3_amd.so (or the file FL3H_TUNING names).
 
 Workloads and the score each parameter is chosen on:
 
-* wg_per_cu, chunk_samples, de_split: frame time of the 1080p still
+* wg_per_cu, chunk_samples, de_halo: frame time of the 1080p still
   (1920x1080 supersample 2, 6 xforms + final xform, DE radius 9) at quality
   200, and of one animation frame (1280x720 supersample 2, 100 temporal
   samples, DE radius 7) at quality 100; the score is their sum.
@@ -40,7 +40,7 @@ from flam3_amd.render import make_frame, render_many  # noqa: E402
 CANDIDATES = {
     "wg_per_cu": [4, 6, 7, 8, 12],
     "chunk_samples": [1 << 27, 1 << 28, 1 << 29],
-    "de_split": [0, 1],
+    "de_halo": [0, 1],
     "min_samples": [128, 256, 512, 1024],
 }
 MARGIN = 0.02

@@ -358,10 +358,6 @@ __device__ __forceinline__ void opaque_pre(Pre<R> &f) {
                    "+v"(f.cosa), "+v"(f.atanyx));
 }
 
-#ifndef FL3_UNI_STRAIGHT
-#define FL3_UNI_STRAIGHT 1
-#endif
-
 // UNI: fn is the same in every lane of the wavefront (the final xform), so
 // the loop runs over that xform's own variations only.
 template <typename R, class RNG, int TIER = FL3_TIER_FULL, bool UNI = false>
@@ -399,21 +395,9 @@ __device__ bool apply_xform(const GenL<R> &G, int fn, const R *p, R *q, RNG &rng
    // compiler hoisting every case's loads (and registers) out of the loop.
    const uint32_t xoff = (uint32_t)((const unsigned char *)&X - fl3_lds);
    if (UNI) {
-      // The first FL3_UNI_STRAIGHT variations of the xform are dispatched in
-      // straight-line code (no loop, so nothing of the other cases can be
-      // hoisted and the inputs need no opaque copies); any further ones in
-      // a loop as below.
       const int fu = __builtin_amdgcn_readfirstlane(fn);
       const int nv = __builtin_amdgcn_readfirstlane(G.vcount[fu]);
-#pragma unroll
-      for (int k = 0; k < FL3_UNI_STRAIGHT; k++) {
-         if (k < nv) {
-            const int u = __builtin_amdgcn_readfirstlane((int)G.vlist[fu * G.nunion + k]);
-            const int v = __builtin_amdgcn_readfirstlane(G.uid[u]);
-            fl3_var_tier<R, RNG, TIER>(v, wrow[u], X, f, rng);
-         }
-      }
-      for (int k = FL3_UNI_STRAIGHT; k < nv; k++) {
+      for (int k = 0; k < nv; k++) {
          const int u = __builtin_amdgcn_readfirstlane((int)G.vlist[fu * G.nunion + k]);
          const int v = __builtin_amdgcn_readfirstlane(G.uid[u]);
          const R w = wrow[u];

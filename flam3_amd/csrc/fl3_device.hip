@@ -94,7 +94,8 @@ struct Ctx {
    DevBuf sorted, wstate, bins, jobtab, soff;
    DevBuf srcv, srcl;                  // density estimation per-source values
    DevBuf detab, deargs;               // batched density estimation: tables, per-render arguments
-   DevBuf deblk;                       // split DE gather: widest source per 16x16 block
+   DevBuf deblk;                       // DE: widest source kernel per 16x16 block
+   DevBuf detaps;                      // DE gather footprint counters (stage profile)
    unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
    void *staging = nullptr;                   // pinned host buffer for the image download
    size_t staging_cap = 0;
@@ -213,6 +214,14 @@ extern "C" void fl3h_profile_reset(int enable) {
    }
    g_prof.samples = 0;
    g_prof.cells = 0;
+   if (g_ctx.ready && g_ctx.detaps.ensure(64 * sizeof(unsigned long long)) == 0) {
+      (void)hipMemsetAsync(g_ctx.detaps.p, 0, 64 * sizeof(unsigned long long), g_ctx.stream);
+      (void)hipStreamSynchronize(g_ctx.stream);
+   }
+}
+
+static unsigned long long *de_taps() {
+   return g_prof.enabled && g_ctx.detaps.p ? (unsigned long long *)g_ctx.detaps.p : nullptr;
 }
 
 extern "C" int fl3h_profile_get(fl3h_stage_times *out) {
@@ -228,6 +237,12 @@ extern "C" int fl3h_profile_get(fl3h_stage_times *out) {
    out->cells = g_prof.cells;
    out->bin_ms = g_prof.ms[ST_BIN];
    out->bin_launches = g_prof.launches[ST_BIN];
+   out->de_taps = 0;
+   if (g_ctx.ready && g_ctx.detaps.p) {
+      unsigned long long t[64];
+      if (hipMemcpy(t, g_ctx.detaps.p, sizeof(t), hipMemcpyDeviceToHost) == hipSuccess)
+         for (int i = 0; i < 64; i++) out->de_taps += (long long)t[i];
+   }
    return 0;
 }
 
@@ -255,7 +270,6 @@ extern "C" int fl3h_copy_probe(long long bytes) {
 
 static size_t bucket_bytes(int bits) { return bits == 64 ? 40 : 20; }
 #define DG_REGION_HOST 16   // cells per side of a density-estimation workgroup (DG_REGION)
-#define DE_SPLIT_LDS_MAX 24576   // largest small-kernel table the split DE gather keeps in LDS
 static size_t accum_bytes(int bits);
 // the selected render's histogram and accumulator
 static void *cur_buckets() { return (unsigned char *)g_ctx.buckets.p + g_ctx.cell0 * bucket_bytes(g_ctx.bits); }
@@ -336,7 +350,7 @@ struct Tuning {
    int wg_per_cu = 8;               // walker workgroups of 256 per CU
    long long min_samples = 256;     // samples per walker at least
    long long chunk_samples = FL3_BIN_CHUNK_SAMPLES;   // binned deposit: samples per chunk
-   int de_split = 1;                // production DE: split gather (1) or one pass (0)
+   int de_halo = 1;                 // DE gather halo: per region from its sources (1) or the widest kernel's (0)
 };
 static Tuning g_tune;
 static std::once_flag g_tune_once;
@@ -346,7 +360,7 @@ static int tune_set(const char *key, double v) {
    else if (!strcmp(key, "min_samples") && v >= 1 && v <= 65536) g_tune.min_samples = (long long)v;
    else if (!strcmp(key, "chunk_samples") && v >= (1 << 20) && v <= (double)(1LL << 30))
       g_tune.chunk_samples = (long long)v;
-   else if (!strcmp(key, "de_split") && (v == 0 || v == 1)) g_tune.de_split = (int)v;
+   else if (!strcmp(key, "de_halo") && (v == 0 || v == 1)) g_tune.de_halo = (int)v;
    else return 1;
    return 0;
 }
@@ -386,7 +400,7 @@ extern "C" int fl3h_tuning(const char *key, const double *value, double *out) {
    if (!strcmp(key, "wg_per_cu")) v = g_tune.wg_per_cu;
    else if (!strcmp(key, "min_samples")) v = (double)g_tune.min_samples;
    else if (!strcmp(key, "chunk_samples")) v = (double)g_tune.chunk_samples;
-   else if (!strcmp(key, "de_split")) v = g_tune.de_split;
+   else if (!strcmp(key, "de_halo")) v = g_tune.de_halo;
    else return 1;
    if (out) *out = v;
    return 0;
@@ -1220,15 +1234,15 @@ extern "C" int fl3d_density(const fl3d_de_params *p, const fl3_de_filters *de) {
    if (g_ctx.srcl.ensure(sizeof(double) * (size_t)g_ctx.width * g_ctx.height)) return 1;
    a.srcv = g_ctx.srcv.p;
    a.srcl = (double *)g_ctx.srcl.p;
-   a.exact = p->exact;
+   a.exact = p->exact || g_ctx.bits == 64;   // (the FP64 gather is the exact one)
+   a.taps = de_taps();   // gather footprint, for the stage profile
    tune_init();
-   if (!a.exact && g_tune.de_split && fl3k_de_split_lds(a.max_filter_index, g_ctx.bits) <= DE_SPLIT_LDS_MAX) {
+   if (g_tune.de_halo) {
       a.nbx = (g_ctx.width + DG_REGION_HOST - 1) / DG_REGION_HOST;
       const size_t nblk = (size_t)a.nbx * ((g_ctx.height + DG_REGION_HOST - 1) / DG_REGION_HOST);
       if (g_ctx.deblk.ensure(sizeof(int) * nblk)) return 1;
       a.blkmax = (int *)g_ctx.deblk.p;
-   } else if (g_ctx.bits == 64)
-      a.exact = 1;   // (the one-pass FP64 gather is the exact one)
+   }
    const int pk = prof_begin(ST_DE);
    const int lrc = fl3k_launch_de_tiled(&a, g_ctx.stream, g_ctx.bits);
    prof_end(pk);
@@ -1295,18 +1309,14 @@ static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *d
       return 1;
    std::vector<DEArgs> as((size_t)nd);
    std::vector<unsigned> first(2 * (size_t)(nd + 1));
-   // split gather when every render's small-kernel table fits the LDS limit
-   int mfi_max = 0;
+   // per-block source radii for every render's gather halos
    long long nblk = 0;
-   for (int k = 0; k < nd; k++) {
-      mfi_max = std::max(mfi_max, de[dej[k]]->max_filter_index);
+   for (int k = 0; k < nd; k++)
       nblk += (long long)((g_ctx.rwidth[dej[k]] + DG_REGION_HOST - 1) / DG_REGION_HOST) *
               ((g_ctx.rheight[dej[k]] + DG_REGION_HOST - 1) / DG_REGION_HOST);
-   }
-   const size_t split_lds = fl3k_de_split_lds(mfi_max, 33);
    tune_init();
-   const bool split = g_tune.de_split && split_lds <= DE_SPLIT_LDS_MAX;
-   if (split && g_ctx.deblk.ensure(sizeof(int) * (size_t)nblk)) return 1;
+   const bool halo = g_tune.de_halo != 0;
+   if (halo && g_ctx.deblk.ensure(sizeof(int) * (size_t)nblk)) return 1;
    long long blk0 = 0;
    unsigned *pf = first.data(), *gf = first.data() + nd + 1;
    unsigned long long pb = 0, gb = 0;
@@ -1324,6 +1334,7 @@ static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *d
       for (const auto &u : uniq) {
          const fl3_de_filters *e = u.first;
          if (e->nfilters == d->nfilters && e->kernel_size == d->kernel_size &&
+             e->max_filter_index == d->max_filter_index &&
              !memcmp(e->filter_coefs, d->filter_coefs, nc * sizeof(double)) &&
              !memcmp(e->filter_widths, d->filter_widths, nw * sizeof(double))) {
             toff = u.second;
@@ -1353,6 +1364,7 @@ static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *d
       a.srcv = (unsigned char *)g_ctx.srcv.p + c0 * 16;
       a.srcl = (double *)g_ctx.srcl.p + c0;
       a.exact = 0;
+      a.taps = de_taps();
       a.width = g_ctx.rwidth[i];
       a.height = g_ctx.rheight[i];
       a.oversample = p[i].oversample;
@@ -1365,7 +1377,7 @@ static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *d
       int half = 0;
       while ((half + 1) * (half + 2) / 2 < d->kernel_size) half++;
       a.half = half;
-      if (split) {
+      if (halo) {
          a.nbx = (a.width + DG_REGION_HOST - 1) / DG_REGION_HOST;
          a.blkmax = (int *)g_ctx.deblk.p + blk0;
          blk0 += (long long)a.nbx * ((a.height + DG_REGION_HOST - 1) / DG_REGION_HOST);
@@ -1390,8 +1402,8 @@ static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *d
    const int pk = prof_begin(ST_DE);
    const unsigned *dfirst = (const unsigned *)((unsigned char *)g_ctx.deargs.p + ab);
    const int lrc = fl3k_launch_de_many((const DEArgs *)g_ctx.deargs.p, dfirst, (unsigned)pb, dfirst + nd + 1,
-                                       (unsigned)gb, nd, g_ctx.stream, split ? (int *)g_ctx.deblk.p : nullptr,
-                                       (long)nblk, split_lds);
+                                       (unsigned)gb, nd, g_ctx.stream, halo ? (int *)g_ctx.deblk.p : nullptr,
+                                       (long)nblk);
    prof_end(pk);
    if (lrc) {
       set_err("density estimation launch failed");

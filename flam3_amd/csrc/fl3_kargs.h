@@ -131,11 +131,12 @@ struct DEArgs {
    int max_filtered_counts, max_filter_index, kernel_size;
    int half;                // largest kernel radius (cells)
    double k1, k2, curve;
-   // split gather (sums without the reference's order): per 16x16 block of
-   // sources, the largest radius of its sources wider than DE_RS (0: none);
-   // nullptr: the one-pass gather
+   // per 16x16 block: 1 + the largest kernel radius of its sources (0:
+   // none), for each gather region's halo; nullptr: the widest kernel's halo
+   // everywhere
    int *blkmax;
    int nbx;                 // blocks per row
+   unsigned long long *taps;   // if set: 64 counters, summed: the gather footprint (2r + 1)^2 of all sources
 };
 
 struct TMArgs {

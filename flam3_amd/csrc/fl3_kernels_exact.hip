@@ -167,16 +167,13 @@ __global__ __launch_bounds__(256) void k_deposit_points(const unsigned char *job
 __device__ __forceinline__ double bucket_value(const void *b, long i, float) { return (double)((const uint32_t *)b)[i]; }
 __device__ __forceinline__ double bucket_value(const void *b, long i, double) { return ((const double *)b)[i]; }
 
-// Sources of radius at most DE_RS are gathered from an LDS window around
-// each workgroup's region; wider ones (sparse cells) from per-region lists.
-#define DE_RS 3
-#define DE_RS_TAPS 10   // coefficients of a kernel with hi <= DE_RS: hi (hi + 1) / 2 + lo
-
-// Per 16x16 block of cells (one DE workgroup region): the largest radius of
-// its sources wider than DE_RS.  Lanes 16k..16k+15 of a wavefront hold
-// cells of one block (rows of 256 cells, blocks 16 wide).
-__device__ __forceinline__ void de_mark_block(const DEArgs &a, int i, int j, int r_large) {
-   int m = r_large;
+// Per 16x16 block of cells (one DE workgroup region): 1 + the largest
+// kernel radius of its sources (0: no source), so that each gather
+// workgroup scans only the halo its neighbourhood's sources can reach from.
+// Lanes 16k..16k+15 of a wavefront hold cells of one block (rows of 256
+// cells, blocks 16 wide).
+__device__ __forceinline__ void de_mark_block(const DEArgs &a, int i, int j, int r1) {
+   int m = r1;
    for (int o = 1; o < 16; o <<= 1) m = max(m, __shfl_xor(m, o));
    if ((threadIdx.x & 15) == 0 && m > 0) atomicMax(&a.blkmax[(j >> 4) * a.nbx + (i >> 4)], m);
 }
@@ -235,24 +232,26 @@ __device__ __forceinline__ void de_prep_body(const DEArgs &a, int bx, int by) {
    const int ss = os / 2;
    const bool scf = !(os & 1);
    const double scfact = pow(os / (os + 1.0), 2.0);
-   const int W = a.width, H = a.height;
+   const int W = a.width;
    const int i = bx * blockDim.x + threadIdx.x;
    const int j = by;
-   if (a.blkmax) {
-      // (every lane takes part in the block reduction)
-      int rl = 0;
-      if (i < W) {
-         const unsigned f = de_prep_cell<BT, AT>(a, i, j, os, ss, scf, scfact);
-         if (f != DE_SKIP) {
-            const int r = (int)ceil(a.widths[f]) - 1;
-            rl = r > DE_RS ? r : 0;
-         }
-      }
-      de_mark_block(a, i < W ? i : W - 1, j, rl);
+   if (!a.blkmax && !a.taps) {
+      if (i < W) de_prep_cell<BT, AT>(a, i, j, os, ss, scf, scfact);
       return;
    }
-   if (i >= W) return;
-   de_prep_cell<BT, AT>(a, i, j, os, ss, scf, scfact);
+   // (every lane takes part in the block and wavefront reductions)
+   int r1 = 0;
+   if (i < W) {
+      const unsigned f = de_prep_cell<BT, AT>(a, i, j, os, ss, scf, scfact);
+      if (f != DE_SKIP) r1 = (int)ceil(a.widths[f]);
+   }
+   if (a.blkmax) de_mark_block(a, i < W ? i : W - 1, j, r1);
+   if (a.taps) {
+      unsigned long long t = r1 > 0 ? (unsigned long long)(2 * r1 - 1) * (unsigned long long)(2 * r1 - 1) : 0ull;
+      for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+      // (64 counters, so that the wavefronts' atomics do not all meet at one address)
+      if ((threadIdx.x & 63) == 0 && t) atomicAdd(a.taps + ((blockIdx.x * 4u + (threadIdx.x >> 6)) & 63u), t);
+   }
 }
 
 // Gather form: each thread owns one output cell and adds, in registers,
@@ -293,10 +292,32 @@ __device__ __forceinline__ void de_gather_body(const DEArgs &a, int bx, int by) 
    if (live)
       for (int c = 0; c < 4; c++) acc[c] = accp[c];
    for (int e = tid; e <= a.max_filter_index && e < 1024; e += DG_CHUNK) radius[e] = (int)ceil(a.widths[e]) - 1;
+   // The halo this region needs: the largest radius among the 16x16 blocks
+   // whose widest source can reach it (-1: none can), from the preparation
+   // pass's per-block maxima; without them the widest kernel's.  A smaller
+   // halo holds the same reaching sources in the same row-major order.
+   __shared__ int s_ext;
+   int E = R;
+   if (a.blkmax) {
+      if (tid == 0) s_ext = -1;
+      __syncthreads();
+      const int nbx = a.nbx, nby = (H + DG_REGION - 1) / DG_REGION;
+      const int span = (R + DG_REGION - 1) / DG_REGION, side = 2 * span + 1;
+      for (int k = tid; k < side * side; k += DG_CHUNK) {
+         const int qx = bx + k % side - span, qy = by + k / side - span;
+         if (qx < 0 || qy < 0 || qx >= nbx || qy >= nby) continue;
+         const int bm = a.blkmax[qy * nbx + qx] - 1;
+         const int qx0 = qx * DG_REGION, qy0 = qy * DG_REGION;
+         const int gx = max(max(rx0 - (qx0 + DG_REGION - 1), qx0 - rx1), 0);
+         const int gy = max(max(ry0 - (qy0 + DG_REGION - 1), qy0 - ry1), 0);
+         if (bm >= 0 && gx <= bm && gy <= bm) atomicMax(&s_ext, min(bm, R));
+      }
+   }
    __syncthreads();
-   const int hx0 = max(0, rx0 - R), hx1 = min(W, rx1 + 1 + R);
-   const int hy0 = max(0, ry0 - R), hy1 = min(H, ry1 + 1 + R);
-   const int hw = hx1 - hx0, nh = hw * (hy1 - hy0);
+   if (a.blkmax) E = s_ext;
+   const int hx0 = max(0, rx0 - E), hx1 = min(W, rx1 + 1 + E);
+   const int hy0 = max(0, ry0 - E), hy1 = min(H, ry1 + 1 + E);
+   const int hw = hx1 - hx0, nh = E < 0 ? 0 : hw * (hy1 - hy0);
    const AT *srcv = (const AT *)a.srcv;
    for (int base = 0; base < nh; base += DG_CHUNK) {
       const int s = base + tid;
@@ -384,167 +405,6 @@ __device__ __forceinline__ void de_gather_body(const DEArgs &a, int bx, int by) 
       for (int c = 0; c < 4; c++) accp[c] = acc[c];
 }
 
-// Split gather (production sums, not in the reference's order): each
-// thread owns one output cell of a 16x16 region.  Part A: sources of radius
-// at most DE_RS -- nearly every cell of a dense image -- are read once per
-// region from a (16 + 2 DE_RS)^2 window staged in LDS, with their kernels'
-// first DE_RS_TAPS coefficients in LDS, so the taps need no lists and no
-// global loads.  Part B: wider sources are gathered from the 16x16 blocks
-// whose widest source (blkmax, from the preparation pass) reaches the
-// region, one block of 256 cells at a time, compacted into LDS and listed
-// per 8x8 quarter as in the one-pass gather.  Deterministic order: window
-// row-major, then blocks row-major.  The coefficient table of small kernels
-// takes (max_filter_index + 1) x DE_RS_TAPS entries of dynamic LDS.
-#define DS_W (DG_REGION + 2 * DE_RS)
-template <typename AT>
-__device__ __forceinline__ void de_gather_split_body(const DEArgs &a, int bx, int by) {
-   extern __shared__ unsigned char de_dyn[];
-   AT *cs = (AT *)de_dyn;                  // [(max_filter_index + 1) * DE_RS_TAPS]
-   typedef AT V4[4];
-   __shared__ V4 s_win[DS_W * DS_W];
-   __shared__ int s_wfr[DS_W * DS_W];      // small source: kernel index * 4 + radius; else -1
-   __shared__ int radius[1024];
-   __shared__ int4 s_src[DG_CHUNK];
-   __shared__ V4 s_val[DG_CHUNK];
-   __shared__ unsigned wave_cnt[DG_CHUNK / 64];
-   __shared__ unsigned short s_list[4][DG_CHUNK];
-   __shared__ unsigned list_cnt[4][DG_CHUNK / 64];
-   const int W = a.width, H = a.height, R = a.half;
-   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-   const int rx0 = bx * DG_REGION, ry0 = by * DG_REGION;
-   const int rx1 = min(rx0 + DG_REGION, W) - 1, ry1 = min(ry0 + DG_REGION, H) - 1;
-   const int mx0 = rx0 + 8 * (wave & 1), my0 = ry0 + 8 * (wave >> 1);
-   const int lx = 8 * (wave & 1) + (lane & 7), ly = 8 * (wave >> 1) + (lane >> 3);
-   const int x = mx0 + (lane & 7), y = my0 + (lane >> 3);
-   const bool live = x < W && y < H;
-   AT *accp = (AT *)a.accum + (live ? (x + (long)y * W) * 4 : 0);
-   AT acc[4] = {(AT)0, (AT)0, (AT)0, (AT)0};
-   if (live)
-      for (int c = 0; c < 4; c++) acc[c] = accp[c];
-   const int nf = a.max_filter_index + 1;
-   for (int e = tid; e < nf && e < 1024; e += DG_CHUNK) radius[e] = (int)ceil(a.widths[e]) - 1;
-   const AT *coefs_t = (const AT *)a.coefs_t;
-   for (int e = tid; e < nf * DE_RS_TAPS; e += DG_CHUNK) {
-      const int f = e / DE_RS_TAPS, k = e - f * DE_RS_TAPS;
-      cs[e] = k < a.kernel_size ? coefs_t[(long)f * a.kernel_size + k] : (AT)0;
-   }
-   __syncthreads();
-   // ---- part A: the window of small sources
-   const AT *srcv = (const AT *)a.srcv;
-   for (int w = tid; w < DS_W * DS_W; w += DG_CHUNK) {
-      const int gx = rx0 - DE_RS + w % DS_W, gy = ry0 - DE_RS + w / DS_W;
-      int fr = -1;
-      if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-         const long cell = gx + (long)gy * W;
-         const unsigned f = a.findex[cell];
-         if (f != DE_SKIP) {
-            const int r = radius[f];
-            if (r <= DE_RS) {
-               fr = (int)f * 4 + r;
-               for (int c = 0; c < 4; c++) s_win[w][c] = srcv[cell * 4 + c];
-            }
-         }
-      }
-      s_wfr[w] = fr;
-   }
-   __syncthreads();
-#pragma unroll
-   for (int wy = -DE_RS; wy <= DE_RS; wy++) {
-#pragma unroll
-      for (int wx = -DE_RS; wx <= DE_RS; wx++) {
-         const int hi = max(wx < 0 ? -wx : wx, wy < 0 ? -wy : wy), lo = min(wx < 0 ? -wx : wx, wy < 0 ? -wy : wy);
-         const int w = (ly + DE_RS + wy) * DS_W + (lx + DE_RS + wx);
-         const int fr = s_wfr[w];
-         if (fr >= 0 && (fr & 3) >= hi) {
-            const AT c = cs[(fr >> 2) * DE_RS_TAPS + hi * (hi + 1) / 2 + lo];
-            if (c != (AT)0)
-               for (int k = 0; k < 4; k++) acc[k] += c * s_win[w][k];
-         }
-      }
-   }
-   // ---- part B: wider sources, from the blocks that can reach the region
-   const int nbx = a.nbx, nby = (H + DG_REGION - 1) / DG_REGION;
-   const int b0x = max(0, (rx0 - R) / DG_REGION), b1x = min(nbx - 1, (rx1 + R) / DG_REGION);
-   const int b0y = max(0, (ry0 - R) / DG_REGION), b1y = min(nby - 1, (ry1 + R) / DG_REGION);
-   for (int qy = b0y; qy <= b1y; qy++) {
-      for (int qx = b0x; qx <= b1x; qx++) {
-         const int bm = a.blkmax[qy * nbx + qx];
-         const int qx0 = qx * DG_REGION, qy0 = qy * DG_REGION;
-         const int gapx = max(max(rx0 - (qx0 + DG_REGION - 1), qx0 - rx1), 0);
-         const int gapy = max(max(ry0 - (qy0 + DG_REGION - 1), qy0 - ry1), 0);
-         if (bm == 0 || gapx > bm || gapy > bm) continue;   // (uniform over the workgroup)
-         const int sx = qx0 + (tid & 15), sy = qy0 + (tid >> 4);
-         int flag = 0, fi = 0;
-         if (sx < W && sy < H) {
-            const unsigned f = a.findex[sx + (long)sy * W];
-            if (f != DE_SKIP) {
-               fi = (int)f;
-               const int r = radius[fi];
-               const int gx = max(max(rx0 - sx, sx - rx1), 0), gy = max(max(ry0 - sy, sy - ry1), 0);
-               flag = r > DE_RS && gx <= r && gy <= r;
-            }
-         }
-         const unsigned long long m = __ballot(flag);
-         const unsigned rank = __popcll(m & ((1ull << lane) - 1ull));
-         if (lane == 0) wave_cnt[wave] = (unsigned)__popcll(m);
-         __syncthreads();
-         unsigned off = 0;
-         for (int w = 0; w < wave; w++) off += wave_cnt[w];
-         unsigned reach = 0;
-         if (flag) {
-            const int r = radius[fi];
-            for (int b = 0; b < 4; b++) {
-               const int bx0 = rx0 + 8 * (b & 1), by0 = ry0 + 8 * (b >> 1);
-               reach |= (unsigned)(sx + r >= bx0 && sx - r <= bx0 + 7 && sy + r >= by0 && sy - r <= by0 + 7) << b;
-            }
-         }
-         unsigned brank[4];
-         for (int b = 0; b < 4; b++) {
-            const unsigned long long mb = __ballot((reach >> b) & 1u);
-            brank[b] = (unsigned)__popcll(mb & ((1ull << lane) - 1ull));
-            if (lane == 0) list_cnt[b][wave] = (unsigned)__popcll(mb);
-         }
-         if (flag) {
-            const unsigned idx = off + rank;
-            const long cell = sx + (long)sy * W;
-            s_src[idx] = make_int4(sx, sy, fi * a.kernel_size, radius[fi]);
-            for (int c = 0; c < 4; c++) s_val[idx][c] = srcv[cell * 4 + c];
-         }
-         __syncthreads();
-         unsigned nb = 0;
-         for (int b = 0; b < 4; b++) {
-            unsigned boff = 0;
-            for (int w = 0; w < DG_CHUNK / 64; w++)
-               if (w < wave) boff += list_cnt[b][w];
-            if ((reach >> b) & 1u) s_list[b][boff + brank[b]] = (unsigned short)(off + rank);
-         }
-         for (int w = 0; w < DG_CHUNK / 64; w++) nb += list_cnt[wave][w];
-         __syncthreads();
-#pragma unroll 4
-         for (unsigned i = 0; i < nb; i++) {
-            const unsigned k = s_list[wave][i];
-            const int4 src = s_src[k];
-            const int dx = abs(x - src.x), dy = abs(y - src.y);
-            const int hi = max(dx, dy), lo = min(dx, dy);
-            const bool tap = live && hi <= src.w;
-            const long ci = tap ? (long)src.z + hi * (hi + 1) / 2 + lo : 0;
-            const AT coef = coefs_t[ci];
-            if (tap && coef != (AT)0)
-               for (int c = 0; c < 4; c++) acc[c] += coef * s_val[k][c];
-         }
-         __syncthreads();
-      }
-   }
-   if (live)
-      for (int c = 0; c < 4; c++) accp[c] = acc[c];
-}
-
-template <typename AT>
-__global__ __launch_bounds__(DG_CHUNK) void k_de_gather_split(DEArgs a) {
-   de_gather_split_body<AT>(a, blockIdx.x, blockIdx.y);
-}
-
-
 __global__ __launch_bounds__(256) void k_fill_i32(int *p, long n, int v) {
    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
    if (i < n) p[i] = v;
@@ -589,40 +449,22 @@ __global__ __launch_bounds__(DG_CHUNK) void k_de_gather_many(const DEArgs *as, c
    de_gather_body<AT, EXACT>(a, (int)(local % cols), (int)(local / cols));
 }
 
-template <typename AT>
-__global__ __launch_bounds__(DG_CHUNK) void k_de_gather_split_many(const DEArgs *as, const unsigned *first, int n) {
-   const int j = de_job(first, n, blockIdx.x);
-   const DEArgs a = as[j];
-   const unsigned local = blockIdx.x - first[j], cols = (unsigned)(a.width + DG_REGION - 1) / DG_REGION;
-   de_gather_split_body<AT>(a, (int)(local % cols), (int)(local / cols));
-}
-
 // Density estimation of n renders: per-render arguments in device memory,
 // workgroup prefixes prep_first[n + 1] (rows of 256 cells) and
 // gather_first[n + 1] (16x16 regions).  bits 32/33 with the FP32
 // accumulator (exact = 0) only.
-// With blkmax (a device array of nblk ints the renders' blkmax pointers
-// point into) the split gather runs, with split_lds bytes of dynamic LDS.
+// blkmax: a device array of nblk ints the renders' blkmax pointers point
+// into (cleared here), or nullptr.
 extern "C" int fl3k_launch_de_many(const DEArgs *as, const unsigned *prep_first, unsigned prep_blocks,
                                    const unsigned *gather_first, unsigned gather_blocks, int n, hipStream_t s,
-                                   int *blkmax, long nblk, size_t split_lds) {
+                                   int *blkmax, long nblk) {
    if (n <= 0) return 0;
    if (blkmax)
       hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, s, blkmax, nblk, 0);
    hipLaunchKernelGGL((k_de_prep_many<uint32_t, float>), dim3(prep_blocks), dim3(256), 0, s, as, prep_first, n);
-   if (blkmax)
-      hipLaunchKernelGGL((k_de_gather_split_many<float>), dim3(gather_blocks), dim3(DG_CHUNK), split_lds, s, as,
-                         gather_first, n);
-   else
-      hipLaunchKernelGGL((k_de_gather_many<float, false>), dim3(gather_blocks), dim3(DG_CHUNK), 0, s, as,
-                         gather_first, n);
+   hipLaunchKernelGGL((k_de_gather_many<float, false>), dim3(gather_blocks), dim3(DG_CHUNK), 0, s, as, gather_first,
+                      n);
    return hipGetLastError() != hipSuccess;
-}
-
-// Dynamic LDS of the split gather: the first DE_RS_TAPS coefficients of
-// every kernel.
-extern "C" size_t fl3k_de_split_lds(int max_filter_index, int bits) {
-   return (size_t)(max_filter_index + 1) * DE_RS_TAPS * (bits == 64 ? sizeof(double) : sizeof(float));
 }
 
 // Log scaling of every cell without DE (rect.c:947-974).
@@ -766,19 +608,9 @@ extern "C" int fl3k_launch_deposit(const unsigned char *job, const double *pts, 
 extern "C" int fl3k_launch_de_tiled(const DEArgs *a, hipStream_t s, int bits) {
    const dim3 g1((a->width + 255) / 256, a->height), b1(256);
    const dim3 g3((a->width + DG_REGION - 1) / DG_REGION, (a->height + DG_REGION - 1) / DG_REGION);
-   if (a->blkmax && !a->exact) {
-      // split gather: blkmax cleared, marked by the preparation pass
+   if (a->blkmax) {   // per-block radii, cleared here and marked by the preparation pass
       const long nblk = (long)g3.x * g3.y;
-      const size_t dyn = fl3k_de_split_lds(a->max_filter_index, bits);
       hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, s, a->blkmax, nblk, 0);
-      if (bits == 64) {
-         hipLaunchKernelGGL((k_de_prep<double, double>), g1, b1, 0, s, *a);
-         hipLaunchKernelGGL((k_de_gather_split<double>), g3, dim3(DG_CHUNK), dyn, s, *a);
-      } else {
-         hipLaunchKernelGGL((k_de_prep<uint32_t, float>), g1, b1, 0, s, *a);
-         hipLaunchKernelGGL((k_de_gather_split<float>), g3, dim3(DG_CHUNK), dyn, s, *a);
-      }
-      return hipGetLastError() != hipSuccess;
    }
    if (bits == 64) {
       hipLaunchKernelGGL((k_de_prep<double, double>), g1, b1, 0, s, *a);

@@ -21,6 +21,12 @@
 // per wavefront and the workgroup's points change lanes after every step.  In
 // record mode each workgroup also counts its records per tile in LDS and
 // stores them per super-tile at blk_hist[block][super-tile].
+// Steps between two exchanges of the points among a workgroup's lanes in
+// the one-xform-per-wavefront mode (a power of two).
+#ifndef FL3_PERM_EVERY
+#define FL3_PERM_EVERY 1u
+#endif
+
 template <int BITS, bool EXACT, int RECORDS, int TIER, bool WAVE>
 __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
    unsigned char *lds = fl3_lds;
@@ -131,31 +137,7 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
       const uint64_t kk = dsc ? dsc->key0 : (nj > 1 ? a.keys[2 * jb] : a.key0);
       const uint32_t bjob = dsc ? (uint32_t)(blockIdx.x - dsc->block0)
                                 : (nj > 1 ? (uint32_t)(blockIdx.x / nj) : (uint32_t)blk);
-      const uint32_t hseed = __builtin_amdgcn_readfirstlane(((uint32_t)kk ^ (uint32_t)(kk >> 32) * 0x85ebca6bu) ^
-                                                            bjob * 0xc2b2ae35u);
-      // The wavefront's xform draws come from a stream of its own, held in
-      // scalar registers (the same in every lane), so the lanes' streams
-      // serve the per-point random numbers only.  Seeded from the job key and
-      // the job-local wavefront; between chunks its four words are kept in
-      // the walker-state slots of the wavefront's first four lanes.
-      WalkerRng wrng;
-      {
-         const uint32_t wj0 = __builtin_amdgcn_readfirstlane((uint32_t)(wj >> 6));
-         if (a.resume) {
-            const long long wbase = wid - (long long)(threadIdx.x & 63);
-            wrng.s0 = __builtin_amdgcn_readfirstlane(ws[9 * W + wbase]);
-            wrng.s1 = __builtin_amdgcn_readfirstlane(ws[9 * W + wbase + 1]);
-            wrng.s2 = __builtin_amdgcn_readfirstlane(ws[9 * W + wbase + 2]);
-            wrng.s3 = __builtin_amdgcn_readfirstlane(ws[9 * W + wbase + 3]);
-         } else {
-            const uint64_t k1 = dsc ? dsc->key1 : (nj > 1 ? a.keys[2 * jb + 1] : a.key1);
-            wrng.seed(kk ^ 0x5851F42D4C957F2DULL, k1, (uint64_t)wj0);
-            wrng.s0 = __builtin_amdgcn_readfirstlane(wrng.s0);
-            wrng.s1 = __builtin_amdgcn_readfirstlane(wrng.s1);
-            wrng.s2 = __builtin_amdgcn_readfirstlane(wrng.s2);
-            wrng.s3 = __builtin_amdgcn_readfirstlane(wrng.s3);
-         }
-      }
+      const uint32_t hseed = ((uint32_t)kk ^ (uint32_t)(kk >> 32) * 0x85ebca6bu) ^ bjob * 0xc2b2ae35u;
       // the xform at the first slot of each 64-slot bucket of the 14-bit
       // selection slot: a draw starts there and steps over the few
       // breakpoints inside its bucket (select_xform's count, in fewer steps)
@@ -167,7 +149,7 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
          sel[threadIdx.x] = (unsigned short)f;
       }
       __syncthreads();
-      uint32_t st = __builtin_amdgcn_readfirstlane(a.resume ? (uint32_t)lastxf : 0u);
+      uint32_t st = a.resume ? (uint32_t)lastxf : 0u;
       // every lane takes one step at a time, so i is the same in all lanes
       // and the workgroup runs until its first walker (the one with the
       // most samples) is done
@@ -175,7 +157,8 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
       if (dsc && RECORDS) nwg = nwg - a.row0 < 0 ? 0 : (nwg - a.row0 < a.rows_out ? nwg - a.row0 : a.rows_out);
       long long i = a.resume ? 0 : -a.fuse;
       for (;;) {
-         const uint32_t xslot = wrng.next() & 16383u;
+         const uint32_t r = rng.next();
+         const uint32_t xslot = __builtin_amdgcn_readfirstlane(r) & 16383u;
          int fs = __builtin_amdgcn_readfirstlane(sel[xslot >> 6]);
          while (fs + 1 < G.nstd && xslot >= (uint32_t)__builtin_amdgcn_readfirstlane(G.starts[fs + 1])) fs++;
          int fn = fs;
@@ -184,14 +167,14 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
          // with a new xform, up to four times; the fifth bad value in a row
          // is kept.  The retries stay inside the step (the lanes without a
          // bad value wait), drawing each retry's xform from the first
-         // retrying lane's stream, so every lane still plots exactly one
+         // lane's stream, so every lane still plots exactly one
          // point per step and chunked launches follow the single launch.
          bool pend = apply_xform<float, WalkerRng, TIER, true>(G, fn, p, q, rng);
          int nbad = 0;
          while (__builtin_amdgcn_ballot_w64(pend)) {
-            // (the retry's xform is drawn by the whole wavefront, outside the
-            // branch, so the wavefront stream stays the same in every lane)
-            const uint32_t rs = wrng.next() & 16383u;
+            // (the retry's xform is drawn outside the branch, from the first
+            // lane's stream, like the step's own draw)
+            const uint32_t rs = __builtin_amdgcn_readfirstlane(rng.next()) & 16383u;
             int f2 = __builtin_amdgcn_readfirstlane(sel[rs >> 6]);
             while (f2 + 1 < G.nstd && rs >= (uint32_t)__builtin_amdgcn_readfirstlane(G.starts[f2 + 1])) f2++;
             if (pend) {
@@ -213,27 +196,28 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
          }
          if (i >= 0 && i < n) plot(i, q, fn);
          i++;
-         uint32_t h = st * 0x9E3779B9u ^ hseed;
-         h ^= h >> 16; h *= 0x7feb352du; h ^= h >> 15; h *= 0x846ca68bu; h ^= h >> 16;
-         // (only the low 8 bits of the product count: a 24-bit multiply)
-         const unsigned slot = (__umul24((h | 1u) & 255u, threadIdx.x ^ ((h >> 8) & 255u)) + (h >> 16)) & 255u;
-         float *buf = xb + (st & 1u) * 768;
-         st++;
-         buf[slot] = p[0];
-         buf[256 + slot] = p[1];
-         buf[512 + slot] = p[2];
-         // (two buffers: a wavefront writes buffer st & 1 only after every
-         // wavefront has passed the barrier behind its reads of it)
-         __syncthreads();
-         p[0] = buf[threadIdx.x];
-         p[1] = buf[256 + threadIdx.x];
-         p[2] = buf[512 + threadIdx.x];
+         // the points change lanes after every FL3_PERM_EVERY-th step (the
+         // same steps in every lane: st counts the workgroup's steps)
+         const uint32_t stp = st++;
+         if ((stp & (FL3_PERM_EVERY - 1u)) == FL3_PERM_EVERY - 1u) {
+            const uint32_t pc = stp / FL3_PERM_EVERY;
+            uint32_t h = pc * 0x9E3779B9u ^ hseed;
+            h ^= h >> 16; h *= 0x7feb352du; h ^= h >> 15; h *= 0x846ca68bu; h ^= h >> 16;
+            const unsigned slot = ((h | 1u) * (threadIdx.x ^ ((h >> 8) & 255u)) + (h >> 16)) & 255u;
+            float *buf = xb + (pc & 1u) * 768;
+            buf[slot] = p[0];
+            buf[256 + slot] = p[1];
+            buf[512 + slot] = p[2];
+            // (two buffers: a wavefront writes buffer pc & 1 only after every
+            // wavefront has passed the barrier behind its reads of it)
+            __syncthreads();
+            p[0] = buf[threadIdx.x];
+            p[1] = buf[256 + threadIdx.x];
+            p[2] = buf[512 + threadIdx.x];
+         }
          if (i >= nwg) break;
       }
       lastxf = (int)st;   // the step count is what resumes here
-      // the wavefront stream's words go to the first four lanes' spare slot
-      const unsigned ln = threadIdx.x & 63u;
-      if (ln < 4u) consec = (int)(ln == 0u ? wrng.s0 : ln == 1u ? wrng.s1 : ln == 2u ? wrng.s2 : wrng.s3);
    } else {
       for (long long i = a.resume ? 0 : -a.fuse; i < n;) {
          const uint32_t r = rng.next();

@@ -162,11 +162,10 @@ int fl3h_density_estimate(const void *buckets, int width, int height, int bits, 
                           int max_filter_index, int kernel_size, int nfilters,
                           const double *filter_widths, const double *filter_coefs, void *accum);
 
-/* The same stage with the production renders' arithmetic: the split gather
- * (sources of radius <= 3 from an LDS window, wider ones from per-region
- * lists), which adds each cell's contributions in another order than the
- * reference, in the accumulator type.  No counterpart in the reference: it
- * exposes the production path's density estimation to tests. */
+/* The same stage with the production renders' arithmetic: contributions
+ * formed in the accumulator type (the exact mode above forms each in double
+ * as the reference does).  No counterpart in the reference: it exposes the
+ * production path's density estimation to tests. */
 int fl3h_density_estimate_production(const void *buckets, int width, int height, int bits, int oversample,
                                      double k1, double k2, double curve, int max_filtered_counts,
                                      int max_filter_index, int kernel_size, int nfilters,
@@ -252,8 +251,8 @@ void fl3h_genome_clear(flam3_genome *g);
  * tuning.conf beside the library ("key value" lines), else defaults.
  * Keys: wg_per_cu (walker workgroups of 256 per CU, default 8), min_samples
  * (samples per walker at least, 256), chunk_samples (binned deposit chunk,
- * 2^28), de_split (production density estimation: split gather 1, one pass
- * 0).  With value non-NULL the key is set for the rest of the process; out
+ * 2^28), de_halo (density estimation: each gather region scans the halo its
+ * neighbourhood's kernels reach 1, the widest kernel's halo everywhere 0).  With value non-NULL the key is set for the rest of the process; out
  * (may be NULL) receives the current value.  Returns 0, or 1 for an unknown
  * key or a value out of range.  The walker parameters shape the sample
  * streams: renders made under different settings are different samples of
@@ -270,6 +269,7 @@ typedef struct {
    long long cells;        /* histogram cells of the last density pass */
    double bin_ms;          /* tile-binned deposit: sort + LDS accumulation */
    long long bin_launches;
+   long long de_taps;      /* density estimation: gather footprint, sum of (2r + 1)^2 over sources */
 } fl3h_stage_times;
 void fl3h_profile_reset(int enable);
 int fl3h_profile_get(fl3h_stage_times *out);

@@ -348,10 +348,10 @@ def test_density_estimation(prod, oracle, ref, bits, ss, rad):
                                                   (64, 2, 7.0, "mixed"), (33, 2, 9.0, "dense"), (33, 1, 9.0, "sparse"),
                                                   (64, 3, 15.0, "sparse")])
 def test_density_estimation_production(prod, oracle, ref, bits, ss, rad, density):
-    """The production renders' density estimation (split gather: small
-    kernels from an LDS window, wide ones from per-region lists, summed in
-    another order than the reference) against the oracle.  Sums of positive
-    terms in another order: within 2e-5 relative in FP32, 1e-12 in FP64."""
+    """The production renders' density estimation (contributions formed in
+    the accumulator type, each region's halo from its neighbourhood's kernel
+    radii) against the oracle, on dense, mixed and sparse histograms: within
+    2e-5 relative in FP32, 1e-12 in FP64 (where it is the exact mode)."""
     rng = np.random.default_rng(19 + ss)
     W, H = 170, 110
     if density == "mixed":
