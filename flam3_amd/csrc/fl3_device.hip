@@ -1926,6 +1926,139 @@ static int density_estimate_host(const void *buckets, int width, int height, int
    return 0;
 }
 
+// ------------------------------------------------------------------ density estimation without allocation
+//
+// fl3h_density_estimate_noalloc works on the caller's device buffers and
+// keeps its own intermediate data -- the per-cell kernel index and
+// per-source values of the two DE passes, and the device copy of the filter
+// bank -- in `scratch`, a fixed device array, so it allocates nothing.  Every
+// GPU the library runs on holds its own copy of the array until the process
+// ends.  The array's address on a device is looked up once, with that device
+// current, and kept in g_scratch; host code passes that address to the
+// kernels, never the array's name.  Calls on one device share the array and
+// must not overlap.
+#define FL3_SCRATCH_BYTES (64u << 20)
+#define FL3_SCRATCH_TABLES (4u << 20)   // filter bank: coefficients (double + float) and widths
+#define FL3_SCRATCH_BLOCKS (256u << 10)  // widest kernel per 16x16 block (65536 blocks)
+#define FL3_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[FL3_SCRATCH_BYTES];
+static unsigned char *g_scratch[FL3_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static unsigned char *scratch_on(int dev) {
+   std::lock_guard<std::mutex> lk(g_scratch_mu);
+   if (dev < 0 || dev >= FL3_MAX_DEVICES) {
+      set_err("fl3h_density_estimate_noalloc: device index out of range");
+      return nullptr;
+   }
+   if (!g_scratch[dev]) {
+      unsigned char *ptr = nullptr;
+      const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !ptr) {
+         snprintf(g_err, sizeof(g_err), "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+         return nullptr;
+      }
+      g_scratch[dev] = ptr;
+   }
+   return g_scratch[dev];
+}
+
+static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+extern "C" long long fl3h_density_estimate_noalloc_max_cells(int bits) {
+   const size_t per = sizeof(uint16_t) + accum_bytes(bits == 64 ? 64 : 33) + sizeof(double);
+   return (long long)((FL3_SCRATCH_BYTES - FL3_SCRATCH_TABLES - FL3_SCRATCH_BLOCKS - 3 * 256) / per);
+}
+
+extern "C" int fl3h_density_estimate_noalloc(const void *buckets, int width, int height, int bits, int oversample,
+                                             double k1, double k2, double curve, int max_filtered_counts,
+                                             int max_filter_index, int kernel_size, int nfilters,
+                                             const double *filter_widths, const double *filter_coefs,
+                                             void *accum) {
+   if (bits != 33 && bits != 64) { set_err("fl3h_density_estimate_noalloc: bits must be 33 or 64"); return 1; }
+   if (!buckets || !accum || width <= 0 || height <= 0 || width >= 65536 || height >= 65536) {
+      set_err("fl3h_density_estimate_noalloc: bad buffers or histogram size");
+      return 1;
+   }
+   const size_t cells = (size_t)width * height;
+   if ((long long)cells > fl3h_density_estimate_noalloc_max_cells(bits)) {
+      snprintf(g_err, sizeof(g_err),
+               "fl3h_density_estimate_noalloc: %zu cells exceed the %lld cells its %u-byte scratch holds at bits %d",
+               cells, fl3h_density_estimate_noalloc_max_cells(bits), FL3_SCRATCH_BYTES, bits);
+      return 1;
+   }
+   if (max_filter_index == 0) {   // plain log scaling: no intermediate data
+      if (fl3k_launch_logscale(buckets, accum, (long)cells, k1, k2, 0, bits)) {
+         set_err("k_logscale launch failed");
+         return 1;
+      }
+      HIPCHK(hipStreamSynchronize(0));
+      return 0;
+   }
+   const size_t nc = (size_t)nfilters * kernel_size, nw = (size_t)nfilters;
+   if (!filter_widths || !filter_coefs || nfilters <= 0 || kernel_size <= 0 || max_filter_index >= 1024 ||
+       max_filter_index >= nfilters ||
+       align256(sizeof(double) * (nc + nw)) + sizeof(float) * nc > FL3_SCRATCH_TABLES) {
+      set_err("fl3h_density_estimate_noalloc: filter bank missing, or larger than the scratch's table space");
+      return 1;
+   }
+   int dev = 0;
+   HIPCHK(hipGetDevice(&dev));
+   unsigned char *base = scratch_on(dev);
+   if (!base) return 1;
+   // layout: tables, the per-block kernel radii, then per cell the kernel
+   // index, the log terms and the values
+   double *coefs = (double *)base;
+   double *widths = coefs + nc;
+   float *coefs_f = (float *)(base + align256(sizeof(double) * (nc + nw)));
+   int *blkmax = (int *)(base + FL3_SCRATCH_TABLES);
+   unsigned char *cellp = base + FL3_SCRATCH_TABLES + FL3_SCRATCH_BLOCKS;
+   uint16_t *findex = (uint16_t *)cellp;
+   double *srcl = (double *)(cellp + align256(sizeof(uint16_t) * cells));
+   void *srcv = (void *)((unsigned char *)srcl + align256(sizeof(double) * cells));
+   HIPCHK(hipMemcpyAsync(coefs, filter_coefs, sizeof(double) * nc, hipMemcpyHostToDevice, 0));
+   HIPCHK(hipMemcpyAsync(widths, filter_widths, sizeof(double) * nw, hipMemcpyHostToDevice, 0));
+   if (fl3k_launch_f64_to_f32(coefs, coefs_f, (long)nc, 0)) {
+      set_err("k_f64_to_f32 launch failed");
+      return 1;
+   }
+   DEArgs a;
+   memset(&a, 0, sizeof(a));
+   a.buckets = buckets;
+   a.accum = accum;
+   a.coefs = coefs;
+   a.widths = widths;
+   a.coefs_t = bits == 64 ? (const void *)coefs : (const void *)coefs_f;
+   a.width = width;
+   a.height = height;
+   a.oversample = oversample;
+   a.max_filtered_counts = max_filtered_counts;
+   a.max_filter_index = max_filter_index;
+   a.kernel_size = kernel_size;
+   a.k1 = k1;
+   a.k2 = k2;
+   a.curve = curve;
+   int half = 0;
+   while ((half + 1) * (half + 2) / 2 < kernel_size) half++;
+   a.half = half;
+   a.findex = findex;
+   a.srcv = srcv;
+   a.srcl = srcl;
+   a.exact = 1;
+   a.nbx = (width + DG_REGION_HOST - 1) / DG_REGION_HOST;
+   if ((size_t)a.nbx * ((height + DG_REGION_HOST - 1) / DG_REGION_HOST) * sizeof(int) > FL3_SCRATCH_BLOCKS) {
+      set_err("fl3h_density_estimate_noalloc: too many 16x16 blocks for the scratch");   // (not reached: sides < 65536)
+      return 1;
+   }
+   a.blkmax = blkmax;
+   if (fl3k_launch_de_tiled(&a, 0, bits)) {
+      set_err("density estimation launch failed");
+      return 1;
+   }
+   HIPCHK(hipStreamSynchronize(0));
+   return 0;
+}
+
 extern "C" int fl3h_tonemap(const void *accum, int width, int height, int bits, const fl3h_tonemap_params *tp,
                             const double *filter, void *out) {
    std::lock_guard<std::recursive_mutex> lk(g_mu);

@@ -171,6 +171,25 @@ int fl3h_density_estimate_production(const void *buckets, int width, int height,
                                      int max_filter_index, int kernel_size, int nfilters,
                                      const double *filter_widths, const double *filter_coefs, void *accum);
 
+/* fl3h_density_estimate on device buffers without allocating anything: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory during the call (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation).  buckets and accum are DEVICE
+ * pointers (the caller's memory, same layouts as fl3h_density_estimate);
+ * the filter bank is host memory.  The intermediate per-cell data and the
+ * device copy of the filter bank live in a fixed 64 MiB device array of the
+ * library (one per GPU, held until the process ends): histograms above
+ * fl3h_density_estimate_noalloc_max_cells(bits) cells (2,409,678 at bits 33,
+ * 1,491,705 at bits 64) are rejected.  Runs on the null stream and returns
+ * when the accumulator is complete.  Calls that overlap on one GPU share
+ * that array and race.  Same arithmetic (the exact mode) and result as
+ * fl3h_density_estimate. */
+int fl3h_density_estimate_noalloc(const void *buckets, int width, int height, int bits, int oversample,
+                                  double k1, double k2, double curve, int max_filtered_counts,
+                                  int max_filter_index, int kernel_size, int nfilters,
+                                  const double *filter_widths, const double *filter_coefs, void *accum);
+long long fl3h_density_estimate_noalloc_max_cells(int bits);
+
 typedef struct {
    int image_width, image_height, oversample, de_offset, filter_width;
    int nchan, transp, bytes_per_channel, earlyclip;
