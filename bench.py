@@ -101,6 +101,8 @@ def parse_args():
     p.add_argument("--no-preview", action="store_true", help="skip the preview-serving measurement")
     p.add_argument("--no-still", action="store_true",
                    help="profiling aid: run only the animation and preview legs and print their results")
+    p.add_argument("--config5", action="store_true", help="also render the 8K bits-64 still once")
+    p.add_argument("--config5-quality", type=float, default=2000.0)
     p.add_argument("--anim-frames", type=int, default=120)
     p.add_argument("--preview-requests", type=int, default=10000)
     p.add_argument("--copy-probe", action="store_true",
@@ -156,6 +158,13 @@ def gpu_bench(args):
         t1 = time.perf_counter()
         st = fl3h_stage_times()
         L.fl3h_profile_get(ctypes.byref(st))
+        # the DE footprint (taps) of the same genome, counted in one more
+        # render outside the timed steps (the count slows the DE pass)
+        L.fl3h_profile_reset(2)
+        one("taps")
+        tp = fl3h_stage_times()
+        L.fl3h_profile_get(ctypes.byref(tp))
+        st.de_taps = tp.de_taps // max(1, tp.density_launches) * max(1, st.density_launches)
         L.fl3h_profile_reset(0)
         log("timed %d steps: %.3f s; iterate %.2f ms/launch (%d), bin %.2f ms/launch (%d), DE %.2f ms, tonemap %.3f ms"
             % (args.steps, t1 - t0, st.iterate_ms / max(1, st.iterate_launches), st.iterate_launches,
@@ -241,6 +250,45 @@ def animation_bench(args):
                         "frames only); plain: one flam3_render call per frame, its 100 temporal samples in one "
                         "walker launch; served: the render server, ~8 frames per shared launch with the host work "
                         "of the next batch overlapping the GPU"}
+    finally:
+        for gp in owned:
+            G.free_struct(gp)
+
+
+def config5_bench(args):
+    """The 8K still: 8 xforms + final xform + xaos, 7680x4320 supersample 3
+    (a 23134 x 13054 histogram), DE radius 15, bits 64 (FP64 buckets and
+    accumulator), one render at the requested quality with stage times."""
+    import numpy as np
+    import flam3_amd as F
+    from flam3_amd import abi
+    from flam3_amd import genome as G
+
+    L = F.lib()
+    g = G.synthetic_8k(seed=5, quality=args.config5_quality)
+    arr, owned = G.genome_array([g])
+    img = np.zeros((g.height, g.width, 3), dtype=np.uint8)
+    stats = abi.stat_struct()
+    try:
+        f = F.make_frame(ctypes.addressof(arr), 1, bits=64, nthreads=1, seed="8k")
+        L.fl3h_profile_reset(1)
+        t0 = time.perf_counter()
+        if L.flam3_render(ctypes.byref(f), img.ctypes.data_as(ctypes.c_void_p), 0, 3, 0, ctypes.byref(stats)) != 0:
+            raise RuntimeError("flam3_render failed: " + F.last_error())
+        dt = time.perf_counter() - t0
+        st = fl3h_stage_times()
+        L.fl3h_profile_get(ctypes.byref(st))
+        L.fl3h_profile_reset(0)
+        log("8K q%g bits 64: %.2f s, iterate %.1f ms, DE %.1f ms, tonemap %.1f ms" % (
+            args.config5_quality, dt, st.iterate_ms, st.density_ms, st.tonemap_ms))
+        return {"workload": "8K still: 8 xforms + final xform + xaos, 7680x4320 ss3, DE radius 15, bits 64, "
+                            "quality %g" % args.config5_quality,
+                "samples": stats.num_iters, "frame_s": dt, "samples_per_s": stats.num_iters / dt,
+                "iterate_ms": st.iterate_ms, "iterate_launches": st.iterate_launches,
+                "density_ms": st.density_ms, "tonemap_ms": st.tonemap_ms, "cells": st.cells,
+                "deposit": "per-sample FP64 atomics (bits 64 is not binned)",
+                "accumulation_hbm_frac": stats.num_iters * 80.0 / (st.iterate_ms * 1e-3) / (HBM_PEAK_GBS * 1e9)
+                if st.iterate_ms > 0 else None}
     finally:
         for gp in owned:
             G.free_struct(gp)
@@ -415,6 +463,8 @@ def main():
             legs["animation"] = animation_bench(args)
         if not args.no_preview:
             legs["preview"] = preview_bench(args)
+        if args.config5:
+            legs["config5"] = config5_bench(args)
         print(json.dumps(legs), flush=True)
         return
     r = gpu_bench(args)
@@ -471,6 +521,8 @@ def main():
                                       / (FP32_PEAK_TFLOPS * 1e12)) if de_ms > 0 else None,
             "tonemap_ms": st.tonemap_ms / max(1, st.tonemap_launches)},
     }
+    if args.config5:
+        out["config5"] = config5_bench(args)
     if not args.no_animation:
         out["animation"] = animation_bench(args)
     if not args.no_preview:

@@ -220,8 +220,10 @@ extern "C" void fl3h_profile_reset(int enable) {
    }
 }
 
+// (counted only with profiling level 2: the count adds a reduction and
+// atomics to the preparation pass)
 static unsigned long long *de_taps() {
-   return g_prof.enabled && g_ctx.detaps.p ? (unsigned long long *)g_ctx.detaps.p : nullptr;
+   return g_prof.enabled >= 2 && g_ctx.detaps.p ? (unsigned long long *)g_ctx.detaps.p : nullptr;
 }
 
 extern "C" int fl3h_profile_get(fl3h_stage_times *out) {
@@ -347,7 +349,8 @@ static int read_counters(double *badvals, int *saturated) {
 // walkers (and so the sample streams) the same way for every render of the
 // process, alone or shared.
 struct Tuning {
-   int wg_per_cu = 8;               // walker workgroups of 256 per CU
+   int wg_per_cu = 8;               // walker workgroups of 256 per CU (launches of several chunks)
+   int wg_per_cu_single = 8;        // the same for launches of one chunk (no sort to overlap)
    long long min_samples = 256;     // samples per walker at least
    long long chunk_samples = FL3_BIN_CHUNK_SAMPLES;   // binned deposit: samples per chunk
    int de_halo = 1;                 // DE gather halo: per region from its sources (1) or the widest kernel's (0)
@@ -357,6 +360,7 @@ static std::once_flag g_tune_once;
 
 static int tune_set(const char *key, double v) {
    if (!strcmp(key, "wg_per_cu") && v >= 1 && v <= 16) g_tune.wg_per_cu = (int)v;
+   else if (!strcmp(key, "wg_per_cu_single") && v >= 1 && v <= 16) g_tune.wg_per_cu_single = (int)v;
    else if (!strcmp(key, "min_samples") && v >= 1 && v <= 65536) g_tune.min_samples = (long long)v;
    else if (!strcmp(key, "chunk_samples") && v >= (1 << 20) && v <= (double)(1LL << 30))
       g_tune.chunk_samples = (long long)v;
@@ -398,6 +402,7 @@ extern "C" int fl3h_tuning(const char *key, const double *value, double *out) {
    if (value && tune_set(key, *value)) return 1;
    double v;
    if (!strcmp(key, "wg_per_cu")) v = g_tune.wg_per_cu;
+   else if (!strcmp(key, "wg_per_cu_single")) v = g_tune.wg_per_cu_single;
    else if (!strcmp(key, "min_samples")) v = (double)g_tune.min_samples;
    else if (!strcmp(key, "chunk_samples")) v = (double)g_tune.chunk_samples;
    else if (!strcmp(key, "de_halo")) v = g_tune.de_halo;
@@ -406,10 +411,15 @@ extern "C" int fl3h_tuning(const char *key, const double *value, double *out) {
    return 0;
 }
 
+// Walkers of a launch: enough for every CU to hold its tuned number of
+// workgroups (fewer when the launch runs in several chunks, whose sort
+// passes share the CUs with the next chunk's walkers), at least min_samples
+// samples each, a multiple of 256.
 static long long choose_walkers(long long nsamples, int requested) {
    if (requested > 0) return requested;
    tune_init();
-   long long maxw = (long long)g_ctx.ncu * g_tune.wg_per_cu * 256;
+   const int wg = nsamples > g_tune.chunk_samples ? g_tune.wg_per_cu : g_tune.wg_per_cu_single;
+   long long maxw = (long long)g_ctx.ncu * wg * 256;
    long long w = nsamples / g_tune.min_samples;
    if (w > maxw) w = maxw;
    if (w < 256) w = 256;

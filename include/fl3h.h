@@ -268,7 +268,9 @@ void fl3h_genome_clear(flam3_genome *g);
  * autotuner (flam3_amd/autotune.py) chooses, loaded once per process from
  * the file named by the FL3H_TUNING environment variable, else from
  * tuning.conf beside the library ("key value" lines), else defaults.
- * Keys: wg_per_cu (walker workgroups of 256 per CU, default 8), min_samples
+ * Keys: wg_per_cu (walker workgroups of 256 per CU for launches of several
+ * chunks, default 8), wg_per_cu_single (the same for launches of one chunk,
+ * 8), min_samples
  * (samples per walker at least, 256), chunk_samples (binned deposit chunk,
  * 2^28), de_halo (density estimation: each gather region scans the halo its
  * neighbourhood's kernels reach 1, the widest kernel's halo everywhere 0).  With value non-NULL the key is set for the rest of the process; out
@@ -279,8 +281,9 @@ void fl3h_genome_clear(flam3_genome *g);
 int fl3h_tuning(const char *key, const double *value, double *out);
 
 /* Stage timing (no counterpart in the reference): with profiling enabled
- * every kernel launch of the render path is bracketed by HIP events on the
- * stream it runs on; totals accumulate until the next reset. */
+ * (reset(1); reset(2) also counts the DE footprint) every kernel launch of
+ * the render path is bracketed by HIP events on the stream it runs on;
+ * totals accumulate until the next reset. */
 typedef struct {
    double iterate_ms, density_ms, tonemap_ms;
    long long iterate_launches, density_launches, tonemap_launches;
@@ -288,7 +291,8 @@ typedef struct {
    long long cells;        /* histogram cells of the last density pass */
    double bin_ms;          /* tile-binned deposit: sort + LDS accumulation */
    long long bin_launches;
-   long long de_taps;      /* density estimation: gather footprint, sum of (2r + 1)^2 over sources */
+   long long de_taps;      /* density estimation: gather footprint, sum of (2r + 1)^2 over sources
+                              (counted with fl3h_profile_reset(2) only: the count costs time) */
 } fl3h_stage_times;
 void fl3h_profile_reset(int enable);
 int fl3h_profile_get(fl3h_stage_times *out);

@@ -167,6 +167,55 @@ def test_de_filters_bit_exact(host, ref, args):
         assert abs(tot - 1.0) < 1e-12
 
 
+def test_de_filter_cache_hits_evictions_and_threads(host, ref):
+    """fl3_create_de_filters keeps the last eight filter banks (filters.c's
+    bank is rebuilt per call in the reference): over ten parameter sets,
+    each asked for twice in a row and again after others have evicted it,
+    and from eight threads at once, every returned bank equals the
+    reference's bit for bit, and every returned bank can be freed."""
+    import threading
+    host.fl3_free_de_filters.argtypes = [ctypes.POINTER(DEF)]
+    sets = [(9.0, 0.0, 0.4, 2), (9.0, 0.0, 0.4, 1), (7.0, 0.0, 0.4, 2), (15.0, 0.0, 0.4, 3), (5.0, 1.0, 0.6, 1),
+            (9.0, 0.0, 0.3, 2), (11.0, 0.5, 0.4, 1), (3.0, 0.0, 0.4, 4), (20.0, 0.0, 0.5, 1), (9.0, 2.0, 0.4, 2)]
+    want = {}
+    for a in sets:
+        mfc, mfi, ks = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        wp, cp_ = ctypes.POINTER(ctypes.c_double)(), ctypes.POINTER(ctypes.c_double)()
+        assert ref.ref().ref_de_filters(*a, ctypes.byref(mfc), ctypes.byref(mfi), ctypes.byref(ks),
+                                        ctypes.byref(wp), ctypes.byref(cp_)) == 0
+        nf = mfi.value + 1
+        want[a] = (mfc.value, mfi.value, ks.value, np.ctypeslib.as_array(wp, (nf,)).copy(),
+                   np.ctypeslib.as_array(cp_, (nf * ks.value,)).copy())
+
+    def check(a):
+        d = DEF()
+        assert host.fl3_create_de_filters(*a, ctypes.byref(d)) == 0
+        mfc, mfi, ks, w, c = want[a]
+        nf = mfi + 1
+        ok = ((d.max_filtered_counts, d.max_filter_index, d.kernel_size) == (mfc, mfi, ks) and
+              np.array_equal(np.ctypeslib.as_array(d.filter_widths, (nf,)), w) and
+              np.array_equal(np.ctypeslib.as_array(d.filter_coefs, (nf * ks,)), c))
+        host.fl3_free_de_filters(ctypes.byref(d))
+        return ok
+
+    for a in sets:              # fill, hit right away
+        assert check(a) and check(a), a
+    for a in sets + sets[::-1]:  # the first sets were evicted by the later ones
+        assert check(a), a
+    results = []
+
+    def worker(k):
+        rng = np.random.default_rng(k)
+        results.append(all(check(sets[int(i)]) for i in rng.integers(0, len(sets), 40)))
+
+    th = [threading.Thread(target=worker, args=(k,)) for k in range(8)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert results == [True] * 8
+
+
 @pytest.mark.parametrize("kernel", range(14))
 @pytest.mark.parametrize("ss,radius,aspect,field", [(1, 0.5, 1.0, 0), (2, 0.5, 1.0, 0), (3, 0.7, 1.0, 1),
                                                     (2, 1.3, 0.9, 2)])
