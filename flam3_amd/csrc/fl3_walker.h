@@ -169,6 +169,10 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
          // bad value wait), drawing each retry's xform from the first
          // lane's stream, so every lane still plots exactly one
          // point per step and chunked launches follow the single launch.
+         // Bad values are counted on plotted steps only: the reference's
+         // count holds its fuse steps too (15 per 10000-sample sub-batch,
+         // 0.15% of its steps), but a walker fuses 100 steps, which for
+         // small renders would be a large share of the count.
          bool pend = apply_xform<float, WalkerRng, TIER, true>(G, fn, p, q, rng);
          int nbad = 0;
          while (__builtin_amdgcn_ballot_w64(pend)) {
@@ -178,7 +182,7 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
             int f2 = __builtin_amdgcn_readfirstlane(sel[rs >> 6]);
             while (f2 + 1 < G.nstd && rs >= (uint32_t)__builtin_amdgcn_readfirstlane(G.starts[f2 + 1])) f2++;
             if (pend) {
-               if (i < n) bad++;
+               if (i >= 0 && i < n) bad++;
                if (++nbad < 5) {
                   p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
                   fn = f2;
@@ -224,7 +228,7 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
          const int fn = select_xform(G, G.chaos_enable ? lastxf : 0, r & 16383u);
          if (apply_xform<float, WalkerRng, TIER>(G, fn, p, q, rng)) {
             consec++;
-            bad++;
+            if (i >= 0) bad++;
             if (consec < 5) {
                p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
                continue;

@@ -310,10 +310,10 @@ def test_bad_values_follow_reference(prod, ref):
     m = 0.5 * (a.mean() + b.mean())
     print("bad values: gpu %d, reference %d of %d; rms ratio %.4f; mean %.3f vs %.3f/%.3f"
           % (st["badvals"], sa["badvals"], st["num_iters"], err / floor, got.mean(), a.mean(), b.mean()), flush=True)
-    # both count the bad values of their unplotted fuse steps too, and the
-    # walkers fuse 100 steps per walker against the reference's 15 per
-    # 10000-sample sub-batch
-    assert 0.1 * st["num_iters"] < sa["badvals"] <= st["badvals"] <= 1.5 * sa["badvals"], (st, sa)
+    # the walkers count the bad values of plotted steps only, the reference
+    # those of its fuse steps too (15 per 10000-sample sub-batch: 0.15% more)
+    assert 0.1 * st["num_iters"] < sa["badvals"], (st, sa)
+    assert 0.97 * sa["badvals"] <= st["badvals"] <= 1.03 * sa["badvals"], (st, sa)
     assert err <= 1.10 * floor, (err, floor)
     assert abs(got.mean() - m) <= 3 * abs(a.mean() - b.mean()) + 0.02 * m + 0.05, (got.mean(), a.mean(), b.mean())
 
