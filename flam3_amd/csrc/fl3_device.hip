@@ -608,7 +608,7 @@ static int narrow_round(const fl3_job_hdr *const *h, int n) {
 }
 
 // Largest records per accumulation work item: a cell's sums must fit 32
-// bits (and the packed pairs must not carry): at most 2^20 records, fewer
+// bits (and the packed pairs must not carry): at most 2^18 records, fewer
 // if a palette entry exceeds 255.
 static int item_records(const fl3_job_hdr *const *h, int n) {
    double maxd = 255.0;

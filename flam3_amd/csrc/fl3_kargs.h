@@ -68,14 +68,17 @@ struct IterArgs {
 
 // Binned deposit limits: tiles per histogram (the sort keeps a counter per
 // tile in LDS), samples per chunk of records, records per accumulation work
-// item (bounds a tile cell's LDS partial sum by 2^20 * 255 < 2^32) and the
-// batch size from which the library prefers binning to atomics.
+// item and the batch size from which the library prefers binning to atomics.
+// A work item is what one workgroup accumulates alone: 2^18 records keeps a
+// tile cell's partial sum below 2^18 * 255 < 2^32 and splits the densest
+// tiles of a frame (2.2e6 of a chunk's 2.6e8 records at 1080p, against a mean
+// of 1.3e5) over several workgroups, so that the last items do not trail.
 #define FL3_MAX_BINS 8192
 #define FL3_SUPER_SHIFT 5           // tiles per super-tile = 32
 
 #define FL3_BIN_CHUNK_SAMPLES (1LL << 28)
 #define FL3_CHUNK_BUFS 2            // record buffers in rotation: the walkers run one chunk ahead of the sort
-#define FL3_BIN_ITEM_RECORDS (1 << 20)
+#define FL3_BIN_ITEM_RECORDS (1 << 18)
 #define FL3_BIN_MIN_SAMPLES (1LL << 20)
 
 // Binned deposit (walker records -> tile-sorted records -> LDS tiles).

@@ -293,7 +293,7 @@ __global__ __launch_bounds__(256) void k_bin_part2(BinArgs a) {
 
 // Persistent workgroups take work items (tile, chunk of its records) from a
 // counter.  The tile's channels live in LDS as two 64-bit pairs (R|G, B|A)
-// and the hit count: a work item adds at most 2^20 deltas of at most 255 to
+// and the hit count: a work item adds at most 2^18 deltas of at most 255 to
 // a cell, so no 32-bit half carries into the other.  A tile that has one
 // work item is added to the histogram without atomics.
 template <typename RT, bool JOBREC>
@@ -411,7 +411,8 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum(BinArgs a) {
 // adds its palette entry and 255 hits): 4-byte records, or job records that
 // carry the entry's bytes (palettes of entries below 256).  The tile's cells
 // live in LDS as two packed 64-bit words, R:24 | G:24 | count:16 and B:32 | A:32, so a record
-// costs two LDS atomics; a round of at most `round` records keeps every
+// costs two LDS atomics (and, for 4-byte records, one 16-byte read of its
+// palette entry, kept in LDS in that packed form); a round of at most `round` records keeps every
 // field below its width, after which each thread folds the CPT cells it
 // owns into registers and clears them.  The tile is then added to the
 // histogram from the registers (non-atomic saturating add if the tile has
@@ -423,11 +424,17 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_narrow(BinArgs a)
    const int tile_cells = CPT * FL3_ACC_THREADS;
    unsigned long long *p0 = (unsigned long long *)fl3_lds;   // tile_cells
    unsigned long long *p1 = p0 + tile_cells;                  // tile_cells
-   uint32_t *dmapu = (uint32_t *)(p1 + tile_cells);           // 1024
+   ulonglong2 *pal = (ulonglong2 *)(p1 + tile_cells);          // 256
    __shared__ int item_sh;
    const double *pd = (const double *)(a.job + h->off_dmap);
    if (!JOB)
-      for (int e = threadIdx.x; e < 1024; e += blockDim.x) dmapu[e] = (uint32_t)pd[e];
+      for (int e = threadIdx.x; e < 256; e += blockDim.x) {
+         const uint32_t r = (uint32_t)pd[4 * e], g = (uint32_t)pd[4 * e + 1], bb = (uint32_t)pd[4 * e + 2], al = (uint32_t)pd[4 * e + 3];
+         ulonglong2 w;
+         w.x = (unsigned long long)r | ((unsigned long long)g << 24) | (1ull << 48);
+         w.y = (unsigned long long)bb | ((unsigned long long)al << 32);
+         pal[e] = w;
+      }
    for (int e = threadIdx.x; e < tile_cells; e += blockDim.x) {
       p0[e] = 0;
       p1[e] = 0;
@@ -476,9 +483,9 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_narrow(BinArgs a)
                   atomicAdd(&p1[local], (unsigned long long)((c >> 16) & 0xffu) | ((unsigned long long)(c >> 24) << 32));
                } else {
                   const unsigned local = ((unsigned)v[u] & 0xffffffu) - cell0;
-                  const uint32_t *du = dmapu + ((unsigned)v[u] >> 24) * 4;
-                  atomicAdd(&p0[local], (unsigned long long)du[0] | ((unsigned long long)du[1] << 24) | (1ull << 48));
-                  atomicAdd(&p1[local], (unsigned long long)du[2] | ((unsigned long long)du[3] << 32));
+                  const ulonglong2 w = pal[(unsigned)v[u] >> 24];
+                  atomicAdd(&p0[local], w.x);
+                  atomicAdd(&p1[local], w.y);
                }
             }
          }

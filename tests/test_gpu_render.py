@@ -258,6 +258,10 @@ BINNED_CASES = [
     # single launch's schedule
     ("bad_values_chunks", lambda: _with_blowup(G.synthetic_small(seed=90, width=128, height=96, quality=30)), {},
      1024, 1024 * 29 + 7),
+    # 1.2e6 samples on a 64x48 image: tiles of more records than one
+    # accumulation work item takes (2^18), summed by several workgroups
+    ("dense_tile_items", lambda: G.synthetic_small(seed=91, width=64, height=48, quality=400), {}, 1024,
+     1024 * 1300),
     # an xform with opacity < 1: 8-byte records carrying the xform index
     ("opacity_wide", lambda: _with_opacity(G.synthetic_small(seed=88, width=128, height=96, quality=30)), {},
      1024, 1024 * 23),
