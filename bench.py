@@ -101,7 +101,7 @@ def parse_args():
     p.add_argument("--no-preview", action="store_true", help="skip the preview-serving measurement")
     p.add_argument("--no-still", action="store_true",
                    help="profiling aid: run only the animation and preview legs and print their results")
-    p.add_argument("--config5", action="store_true", help="also render the 8K bits-64 still once")
+    p.add_argument("--no-config5", action="store_true", help="skip the 8K bits-64 still")
     p.add_argument("--config5-quality", type=float, default=2000.0)
     p.add_argument("--anim-frames", type=int, default=120)
     p.add_argument("--preview-requests", type=int, default=10000)
@@ -463,7 +463,7 @@ def main():
             legs["animation"] = animation_bench(args)
         if not args.no_preview:
             legs["preview"] = preview_bench(args)
-        if args.config5:
+        if not args.no_config5:
             legs["config5"] = config5_bench(args)
         print(json.dumps(legs), flush=True)
         return
@@ -521,7 +521,7 @@ def main():
                                       / (FP32_PEAK_TFLOPS * 1e12)) if de_ms > 0 else None,
             "tonemap_ms": st.tonemap_ms / max(1, st.tonemap_launches)},
     }
-    if args.config5:
+    if not args.no_config5:
         out["config5"] = config5_bench(args)
     if not args.no_animation:
         out["animation"] = animation_bench(args)

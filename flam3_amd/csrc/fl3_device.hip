@@ -96,6 +96,9 @@ struct Ctx {
    DevBuf detab, deargs;               // batched density estimation: tables, per-render arguments
    DevBuf deblk;                       // DE: widest source kernel per 16x16 block
    DevBuf detaps;                      // DE gather footprint counters (stage profile)
+   DevBuf blk2, soff2;                 // banded histograms: second-level block counts and offsets
+   unsigned *band_host = nullptr;      // pinned: first-level tile starts of the last chunk
+   hipEvent_t ev_band = nullptr;
    unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
    void *staging = nullptr;                   // pinned host buffer for the image download
    size_t staging_cap = 0;
@@ -129,6 +132,8 @@ static int ctx_init() {
       HIPCHK(hipEventCreateWithFlags(&g_ctx.ev_bin[i], hipEventDisableTiming));
    }
    HIPCHK(hipEventCreateWithFlags(&g_ctx.ev_join, hipEventDisableTiming));
+   HIPCHK(hipEventCreateWithFlags(&g_ctx.ev_band, hipEventDisableTiming));
+   HIPCHK(hipHostMalloc((void **)&g_ctx.band_host, sizeof(unsigned) * (FL3_MAX_BINS + 1), hipHostMallocDefault));
    HIPCHK(hipMalloc((void **)&g_ctx.counters, 64));
    g_ctx.ready = 1;
    g_ready.store(1, std::memory_order_release);
@@ -464,12 +469,41 @@ static int ensure_chunk_bufs(long long nchunks, size_t rec_bytes, size_t blk_byt
 // Tile shift of the binned deposit for the current histogram (cells per
 // tile = 1 << shift), or 0 when the histogram is too large for it.
 static int bin_shift_cells(long long ncells);
-static int bin_shift() { return bin_shift_cells((long long)g_ctx.width * g_ctx.height); }
 static int bin_shift_cells(long long ncells) {
    // 4096-cell tiles when LDS allows (fewer, larger scatter runs), else 2048
    for (int sh = 12; sh >= 11; sh--)
       if (((ncells + (1LL << sh) - 1) >> sh) <= FL3_MAX_BINS && fl3k_bin_lds_bytes(sh) <= g_ctx.lds_max) return sh;
    return 0;
+}
+
+// Tile shift of the accumulation into double buckets (bits 64: five doubles
+// per cell in LDS), 0 if LDS is too small.
+static int f64_tile_shift() {
+   for (int sh = 11; sh >= 10; sh--)
+      if (fl3k_bin_f64_lds_bytes(sh) <= g_ctx.lds_max) return sh;
+   return 0;
+}
+
+// Accumulation tile shift for a histogram of ncells cells and the bucket
+// type; *band: 0 if its tiles fit the tile limit, else the shift of a band
+// (FL3_MAX_BINS tiles), the unit the records are first grouped by.
+static int acc_tile_shift(long long ncells, int bits, int *band) {
+   int sh = 0;
+   if (bits == 64)
+      sh = f64_tile_shift();
+   else
+      for (int t = 12; t >= 11 && !sh; t--)
+         if (fl3k_bin_lds_bytes(t) <= g_ctx.lds_max) sh = t;
+   *band = 0;
+   if (!sh) return 0;
+   if (((ncells + (1LL << sh) - 1) >> sh) > FL3_MAX_BINS) {
+      const int bs = sh + 13;   // FL3_MAX_BINS = 2^13 tiles per band
+      // bands are the first level's super-tiles: at most 256 of them, and
+      // 32-bit record cells
+      if (((ncells + (1LL << bs) - 1) >> bs) > 256 || ncells >= (1LL << 31)) return 0;
+      *band = bs;
+   }
+   return sh;
 }
 
 // 4-byte records (cell | palette index << 24) suffice when every plotted
@@ -657,16 +691,81 @@ static size_t with_shuffle(IterArgs &a, size_t lds, bool want) {
 // Binned deposit: the walkers write records in chunks of `per` rows (the
 // walker state carries over between chunks, so the trajectories are those of
 // a single launch); each chunk is tile-sorted and accumulated.
-static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &js, const fl3d_iter_params *p,
-                          int shift) {
+// One chunk of a banded histogram on stream bs: the first level (scans of
+// the walkers' counts, records grouped by band into b1.sorted), then per
+// band with records: a counting pass, the two sort passes (through the
+// walker buffer) back into the band's range and the accumulation.  The
+// host reads the band boundaries, so it waits for this chunk's walkers.
+static int bin_banded(const BinArgs &b1, int format, int band, int shift, hipStream_t bs) {
+   if (fl3k_launch_bin_split(&b1, bs)) return 1;
+   HIPCHK(hipMemcpyAsync(g_ctx.band_host, b1.bin_start, sizeof(unsigned) * ((size_t)b1.nbins + 1),
+                         hipMemcpyDeviceToHost, bs));
+   HIPCHK(hipEventRecord(g_ctx.ev_band, bs));
+   HIPCHK(hipEventSynchronize(g_ctx.ev_band));
    const long long ncells = (long long)g_ctx.width * g_ctx.height;
-   const int nbins = (int)((ncells + (1LL << shift) - 1) >> shift);
-   bool narrow = narrow_records_ok(js.h[0], ncells);
+   const int rows2 = 64;   // second-level blocks: 64 x 256 records
+   const size_t rsz = format == FL3_BIN_F64_WIDE || format == 0 ? 8 : 4;
+   const unsigned *st = g_ctx.band_host;
+   for (int j = 0; j < b1.nsuper; j++) {
+      const int t0 = j << FL3_SUPER_SHIFT, t1 = std::min(b1.nbins, t0 + (1 << FL3_SUPER_SHIFT));
+      const long long n = (long long)st[t1] - st[t0];
+      if (n <= 0) continue;
+      BinArgs b = b1;
+      b.cell_base = (unsigned)((long long)j << band);
+      b.rec = (const unsigned char *)b1.sorted + rsz * st[t0];
+      b.rec_count = n;
+      b.rows = rows2;
+      b.nblk = (int)((n + 256LL * rows2 - 1) / (256LL * rows2));
+      b.walkers = 256LL * b.nblk;
+      b.shift = shift;
+      const long long cells = std::min(ncells - (long long)b.cell_base, 1LL << band);
+      b.nbins = (int)((cells + (1LL << shift) - 1) >> shift);
+      b.nsuper = (b.nbins + (1 << FL3_SUPER_SHIFT) - 1) >> FL3_SUPER_SHIFT;
+      if (g_ctx.blk2.ensure(sizeof(unsigned) * (size_t)b.nblk * b.nsuper) ||
+          g_ctx.soff2.ensure(sizeof(unsigned) * (size_t)b.nblk * b.nsuper))
+         return 1;
+      b.blk_hist = (unsigned *)g_ctx.blk2.p;
+      b.soff = (unsigned *)g_ctx.soff2.p;
+      // (the first level's counters are spent: the band's reuse their buffer)
+      unsigned *bw = (unsigned *)g_ctx.bins.p;
+      b.bin_total = bw;
+      b.bin_start = bw + b.nbins;
+      b.item_start = bw + 2 * b.nbins + 1;
+      b.item_counter = bw + 3 * b.nbins + 2;
+      b.cursor = bw + 3 * b.nbins + 4;
+      b.sorted = (void *)b1.rec;                                // the walker buffer, free after the band pass
+      b.final = (unsigned char *)b1.sorted + rsz * st[t0];      // the band's own range
+      HIPCHK(hipMemsetAsync(b.bin_total, 0, sizeof(unsigned) * (size_t)b.nbins, bs));
+      HIPCHK(hipMemsetAsync(b.item_counter, 0, sizeof(unsigned), bs));
+      if (fl3k_launch_bin_count(&b, bs) || fl3k_launch_bin(&b, g_ctx.ncu, format, bs)) return 1;
+   }
+   return 0;
+}
+
+// shift: the accumulation tiles' (cells per tile = 1 << shift); band: 0, or
+// the band shift of a histogram of more than FL3_MAX_BINS tiles: the
+// walkers then count their records per 2^(band - FL3_SUPER_SHIFT)-cell tile,
+// the first sort pass groups them by band, and each band is counted, sorted
+// and accumulated on its own (iterate_banded_chunk).
+static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &js, const fl3d_iter_params *p,
+                          int shift, int band) {
+   const long long ncells = (long long)g_ctx.width * g_ctx.height;
+   const int wshift = band ? band - FL3_SUPER_SHIFT : shift;   // the walkers' counting tiles
+   const int nbins = (int)((ncells + (1LL << wshift) - 1) >> wshift);
+   bool narrow = !band && narrow_records_ok(js.h[0], ncells);
    for (int j = 1; narrow && j < js.n; j++) narrow = same_narrow_tables(js.h[0], js.h[j]);
    // one palette for all jobs: narrow or wide records; else the job index
-   // rides in the record (step palettes only: it takes the fraction's bits)
-   const int format = narrow ? 1 : (js.same_tables ? 0 : 2);
+   // rides in the record (step palettes only: it takes the fraction's bits);
+   // double buckets take narrow or wide records
+   const int format = g_ctx.bits == 64 ? (narrow ? FL3_BIN_F64_NARROW : FL3_BIN_F64_WIDE)
+                                       : (narrow ? 1 : (js.same_tables ? 0 : 2));
    long long per = p->chunk > 0 ? (p->chunk + W - 1) / W : (g_tune.chunk_samples / W);
+   // banded: chunks of at least 4 samples per cell (up to 2^31 records), as
+   // every chunk adds nearly every tile of the histogram once
+   if (band && p->chunk <= 0) {
+      const long long want = 4 * ncells < (1LL << 28) ? (1LL << 28) : (4 * ncells > (1LL << 31) ? (1LL << 31) : 4 * ncells);
+      per = (want + W - 1) / W;
+   }
    if (per < 1) per = 1;
    const long long rows_total = (p->nsamples + Wj - 1) / Wj;
    if (per > rows_total) per = rows_total;
@@ -683,7 +782,8 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    if (ensure_chunk_bufs(nchunks, rec_bytes, sizeof(unsigned) * (size_t)nblk * nsup, sizeof(unsigned) * (size_t)nbins))
       return 1;
    const int nsuper = (nbins + (1 << FL3_SUPER_SHIFT) - 1) >> FL3_SUPER_SHIFT;
-   if (g_ctx.bins.ensure(sizeof(unsigned) * (4 * (size_t)nbins + 8))) return 1;
+   // (the bands' counters reuse these: room for FL3_MAX_BINS tiles)
+   if (g_ctx.bins.ensure(sizeof(unsigned) * (4 * (size_t)(band ? FL3_MAX_BINS : nbins) + 8))) return 1;
    if (g_ctx.soff.ensure(sizeof(unsigned) * (size_t)nblk * nsuper)) return 1;
    unsigned *bw = (unsigned *)g_ctx.bins.p;
    BinArgs b;
@@ -702,7 +802,7 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    b.cursor = bw + 3 * nbins + 4;
    b.soff = (unsigned *)g_ctx.soff.p;
    b.nsuper = nsuper;
-   b.shift = shift;
+   b.shift = wshift;
    b.nbins = nbins;
    b.width = g_ctx.width;
    b.height = g_ctx.height;
@@ -711,7 +811,7 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    if (format == 2 && upload_job_tables(js.h, js.n, js.nxf_max, &b, &pal8)) return 1;
    a.rec_mode = narrow ? 2 : (format == 2 ? 3 : 1);
    a.wstate = (uint32_t *)g_ctx.wstate.p;
-   a.bin_shift = shift;
+   a.bin_shift = wshift;
    a.nbins = nbins;
    a.hist_off = (unsigned)((js.lds_rec + 15) & ~(size_t)15);
    size_t lds_rec = a.hist_off + sizeof(unsigned) * (size_t)nbins;
@@ -756,7 +856,10 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
       b.bin_total = a.bin_total;
       b.rows = a.rows_out;
       pk = prof_begin(ST_BIN, bs);
-      lrc = fl3k_launch_bin(&b, g_ctx.ncu, format, bs);
+      if (band)
+         lrc = bin_banded(b, format, band, shift, bs);
+      else
+         lrc = fl3k_launch_bin(&b, g_ctx.ncu, format, bs);
       prof_end(pk, bs);
       if (lrc) {
          set_err("binned deposit launch failed");
@@ -837,19 +940,23 @@ extern "C" int fl3d_iterate_batch(const void *const *jobs, const size_t *job_byt
       // walkers: a multiple of 256 per job
       const long long Wj = job_walkers(p->nsamples, njobs, p->nwalkers);
       const long long W = Wj * njobs;
-      const int shift = bin_shift();
-      const bool binnable = g_ctx.bits != 64 && js.nxf_max <= 256 && shift > 0 && !(njobs > 1 && js.linear &&
-                                                                                    !js.same_tables);
+      int band = 0;
+      const int shift = acc_tile_shift((long long)g_ctx.width * g_ctx.height, g_ctx.bits, &band);
+      // double buckets and banded histograms take one palette and one set
+      // of visibilities for all jobs (4- or 8-byte records, no job records)
+      const bool binnable = js.nxf_max <= 256 && shift > 0 && !(njobs > 1 && js.linear && !js.same_tables) &&
+                            (js.same_tables || (g_ctx.bits != 64 && !band));
       if (p->deposit == 2 && !binnable) {
-         set_err("tile-binned deposit needs bits 32/33, at most 256 xforms, at most %s%ld tiles and, for "
-                 "temporal samples with linear palettes, one palette for all", "", (long)FL3_MAX_BINS);
+         set_err("tile-binned deposit needs at most 256 xforms, at most 256 bands of %s%ld tiles and, for "
+                 "bits 64, banded histograms or temporal samples with linear palettes, one palette for all",
+                 "", (long)FL3_MAX_BINS);
          return 1;
       }
       const bool binned =
           binnable && (p->deposit == 2 || (p->deposit == 0 && p->nsamples * njobs >= FL3_BIN_MIN_SAMPLES));
       g_prof.samples += p->nsamples * njobs;
       if (binned) {
-         if (iterate_binned(a, W, Wj, js, p, shift)) return 1;
+         if (iterate_binned(a, W, Wj, js, p, shift, band)) return 1;
       } else {
          const int pk = prof_begin(ST_ITER);
          const size_t lw = with_shuffle(a, js.lds, jobs_wave_ok(js.h, js.n));

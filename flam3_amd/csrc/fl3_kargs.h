@@ -81,6 +81,11 @@ struct IterArgs {
 #define FL3_BIN_ITEM_RECORDS (1 << 18)
 #define FL3_BIN_MIN_SAMPLES (1LL << 20)
 
+// Record formats of fl3k_launch_bin beyond 0 (8-byte records), 1 (4-byte
+// records) and 2 (job records): the same records into double buckets.
+#define FL3_BIN_F64_WIDE 3
+#define FL3_BIN_F64_NARROW 4
+
 // Binned deposit (walker records -> tile-sorted records -> LDS tiles).
 struct BinArgs {
    const unsigned char *job;       // for palette / visibility tables
@@ -109,6 +114,11 @@ struct BinArgs {
    const float *dmapf_all;         // [job][1024]
    const float *vis_all;           // [job][vis_stride]
    int vis_stride;
+   // banded histograms: the records' cells are relative to cell_base; the
+   // input of the first sort pass (or count pass) holds rec_count records
+   // (0: every slot of the blocks is a record or FL3_REC_NONE)
+   unsigned cell_base;
+   long long rec_count;
 };
 
 struct PointsArgs {

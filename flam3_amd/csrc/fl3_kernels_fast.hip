@@ -107,6 +107,16 @@ template <typename RT>
 __device__ __forceinline__ bool rec_none(RT v) {
    return (unsigned)v == FL3_REC_NONE;
 }
+// the record's cell relative to the band being sorted
+template <typename RT>
+__device__ __forceinline__ unsigned rel_cell(const BinArgs &a, RT v) {
+   return rec_cell(v) - a.cell_base;
+}
+// slot idx of the input blocks: past rec_count (when set) there are no records
+template <typename RT>
+__device__ __forceinline__ RT rec_at(const BinArgs &a, const RT *rec, long long idx) {
+   return a.rec_count == 0 || idx < a.rec_count ? rec[idx] : (RT)FL3_REC_NONE;
+}
 
 // First slot of each walker workgroup's records of each super-tile in the
 // super-tile-grouped array: workgroup s scans super-tile s's counts over the
@@ -171,17 +181,17 @@ __global__ __launch_bounds__(256) void k_bin_part1(BinArgs a) {
    long long i = 0;
    for (; i + FL3_SCATTER_UNROLL <= a.rows; i += FL3_SCATTER_UNROLL) {
       RT v[FL3_SCATTER_UNROLL];
-      for (int k = 0; k < FL3_SCATTER_UNROLL; k++) v[k] = rec[(blockIdx.x * a.rows + i + k) * 256 + threadIdx.x];
+      for (int k = 0; k < FL3_SCATTER_UNROLL; k++) v[k] = rec_at(a, rec, (blockIdx.x * a.rows + i + k) * 256 + threadIdx.x);
       for (int k = 0; k < FL3_SCATTER_UNROLL; k++) {
          if (rec_none(v[k])) continue;
-         const unsigned s = rec_cell(v[k]) >> ss;
+         const unsigned s = rel_cell(a, v[k]) >> ss;
          out[base[s] + atomicAdd(&cnt[s], 1u)] = v[k];
       }
    }
    for (; i < a.rows; i++) {
-      const RT v = rec[(blockIdx.x * a.rows + i) * 256 + threadIdx.x];
+      const RT v = rec_at(a, rec, (blockIdx.x * a.rows + i) * 256 + threadIdx.x);
       if (rec_none(v)) continue;
-      const unsigned s = rec_cell(v) >> ss;
+      const unsigned s = rel_cell(a, v) >> ss;
       out[base[s] + atomicAdd(&cnt[s], 1u)] = v;
    }
 }
@@ -211,20 +221,20 @@ __global__ __launch_bounds__(256) void k_bin_part1_staged(BinArgs a) {
       unsigned rk[P];
       for (int k = 0; k < P; k++) {
          const long long row = i0 + k;
-         v[k] = row < a.rows ? rec[(blockIdx.x * a.rows + row) * 256 + t] : (RT)FL3_REC_NONE;
+         v[k] = row < a.rows ? rec_at(a, rec, (blockIdx.x * a.rows + row) * 256 + t) : (RT)FL3_REC_NONE;
       }
       for (int k = 0; k < P; k++)
-         if (!rec_none(v[k])) rk[k] = atomicAdd(&lcnt[rec_cell(v[k]) >> ss], 1u);
+         if (!rec_none(v[k])) rk[k] = atomicAdd(&lcnt[rel_cell(a, v[k]) >> ss], 1u);
       __syncthreads();
       unsigned tot;
       loff[t] = wg_excl_scan256(lcnt[t], wsum, &tot);
       __syncthreads();
       for (int k = 0; k < P; k++)
-         if (!rec_none(v[k])) stage[loff[rec_cell(v[k]) >> ss] + rk[k]] = v[k];
+         if (!rec_none(v[k])) stage[loff[rel_cell(a, v[k]) >> ss] + rk[k]] = v[k];
       __syncthreads();
       for (unsigned j = t; j < tot; j += 256) {
          const RT w = stage[j];
-         const unsigned sp = rec_cell(w) >> ss;
+         const unsigned sp = rel_cell(a, w) >> ss;
          out[gnext[sp] + (j - loff[sp])] = w;
       }
       __syncthreads();
@@ -261,11 +271,11 @@ __global__ __launch_bounds__(256) void k_bin_part2(BinArgs a) {
       v[k] = r < total ? in[r] : (RT)FL3_REC_NONE;
    }
    for (int k = 0; k < FL3_PART2_PER; k++)
-      if (!rec_none(v[k])) rank[k] = atomicAdd(&cnt[rec_cell(v[k]) >> a.shift], 1u);
+      if (!rec_none(v[k])) rank[k] = atomicAdd(&cnt[rel_cell(a, v[k]) >> a.shift], 1u);
    __syncthreads();
    const long long rl = (total < r0 + FL3_PART2_RECORDS ? total : r0 + FL3_PART2_RECORDS) - 1;
-   const int t_lo = (int)((rec_cell(in[r0]) >> a.shift) & ~((1u << FL3_SUPER_SHIFT) - 1u));
-   const int t_hi = (int)((rec_cell(in[rl]) >> a.shift) | ((1u << FL3_SUPER_SHIFT) - 1u));
+   const int t_lo = (int)((rel_cell(a, in[r0]) >> a.shift) & ~((1u << FL3_SUPER_SHIFT) - 1u));
+   const int t_hi = (int)((rel_cell(a, in[rl]) >> a.shift) | ((1u << FL3_SUPER_SHIFT) - 1u));
    if (sizeof(RT) == 4 && t_hi - t_lo < 256) {
       const int b = t_lo + (int)threadIdx.x;
       const unsigned c = b <= t_hi && b < a.nbins ? cnt[b] : 0u;
@@ -275,11 +285,11 @@ __global__ __launch_bounds__(256) void k_bin_part2(BinArgs a) {
       gofs[threadIdx.x] = c ? atomicAdd(&a.cursor[b], c) : 0u;
       __syncthreads();
       for (int k = 0; k < FL3_PART2_PER; k++)
-         if (!rec_none(v[k])) stage[lofs[(rec_cell(v[k]) >> a.shift) - t_lo] + rank[k]] = v[k];
+         if (!rec_none(v[k])) stage[lofs[(rel_cell(a, v[k]) >> a.shift) - t_lo] + rank[k]] = v[k];
       __syncthreads();
       for (unsigned j = threadIdx.x; j < tot; j += 256) {
          const RT w = stage[j];
-         const unsigned bb = (rec_cell(w) >> a.shift) - (unsigned)t_lo;
+         const unsigned bb = (rel_cell(a, w) >> a.shift) - (unsigned)t_lo;
          out[gofs[bb] + (j - lofs[bb])] = w;
       }
       return;
@@ -288,7 +298,7 @@ __global__ __launch_bounds__(256) void k_bin_part2(BinArgs a) {
       if (cnt[b]) cnt[b] = atomicAdd(&a.cursor[b], cnt[b]);
    __syncthreads();
    for (int k = 0; k < FL3_PART2_PER; k++)
-      if (!rec_none(v[k])) out[cnt[rec_cell(v[k]) >> a.shift] + rank[k]] = v[k];
+      if (!rec_none(v[k])) out[cnt[rel_cell(a, v[k]) >> a.shift] + rank[k]] = v[k];
 }
 
 // Persistent workgroups take work items (tile, chunk of its records) from a
@@ -339,7 +349,7 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum(BinArgs a) {
       const long long r0 = (long long)a.bin_start[b] + k * a.chunk;
       const long long r1 = min((long long)a.bin_start[b + 1], r0 + a.chunk);
       const bool single = (a.item_start[b + 1] - a.item_start[b]) == 1;
-      const unsigned cell0 = (unsigned)b << a.shift;
+      const unsigned cell0 = a.cell_base + ((unsigned)b << a.shift);
       for (long long rb = r0; rb < r1; rb += FL3_ACC_UNROLL * blockDim.x) {
          RT v[FL3_ACC_UNROLL];
          for (int u = 0; u < FL3_ACC_UNROLL; u++) {
@@ -459,7 +469,7 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_narrow(BinArgs a)
       const long long r0 = (long long)a.bin_start[b] + k * a.chunk;
       const long long r1 = min((long long)a.bin_start[b + 1], r0 + a.chunk);
       const bool single = (a.item_start[b + 1] - a.item_start[b]) == 1;
-      const unsigned cell0 = (unsigned)b << a.shift;
+      const unsigned cell0 = a.cell_base + ((unsigned)b << a.shift);
       uint32_t acc[CPT][5];
       for (int j = 0; j < CPT; j++)
          for (int c = 0; c < 5; c++) acc[j][c] = 0;
@@ -523,6 +533,127 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_narrow(BinArgs a)
    }
 }
 
+// Counting pass of a band (the walkers' counting for records sorted by band
+// first): workgroup k counts the records of its block of rows x 256 slots
+// per tile in LDS, adds the counts to bin_total and stores its counts per
+// super-tile at blk_hist[k][super-tile].
+template <typename RT>
+__global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
+   unsigned *hist = (unsigned *)fl3_lds;   // [nbins]
+   for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) hist[b] = 0;
+   __syncthreads();
+   const RT *rec = (const RT *)a.rec;
+   for (long long i = 0; i < a.rows; i++) {
+      const RT v = rec_at(a, rec, (blockIdx.x * a.rows + i) * 256 + threadIdx.x);
+      if (!rec_none(v)) atomicAdd(&hist[rel_cell(a, v) >> a.shift], 1u);
+   }
+   __syncthreads();
+   for (int b = threadIdx.x; b < a.nbins; b += blockDim.x)
+      if (hist[b]) atomicAdd(&a.bin_total[b], hist[b]);
+   for (int sp = threadIdx.x; sp < a.nsuper; sp += blockDim.x) {
+      const int b0 = sp << FL3_SUPER_SHIFT, b1 = min(a.nbins, b0 + (1 << FL3_SUPER_SHIFT));
+      unsigned c = 0;
+      for (int b = b0; b < b1; b++) c += hist[b];
+      a.blk_hist[(size_t)blockIdx.x * a.nsuper + sp] = c;
+   }
+}
+
+// Accumulation into bits-64 (double) buckets: the tile's five channels live
+// in LDS as doubles and each record adds what the direct deposit adds for
+// its point (rect.c:492-497 in double: the palette entry, blended for a
+// linear palette, times the xform's visibility, and 255 times it to the
+// count), so a cell's sum differs from the per-sample atomics' only in the
+// order of its additions.  4-byte records (stepped palette, visibility 1)
+// or 8-byte records (make_record).  Persistent workgroups take work items
+// as in k_bin_accum; a tile with one item is added without atomics.
+template <typename RT>
+__global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_f64(BinArgs a) {
+   const fl3_job_hdr *h = (const fl3_job_hdr *)a.job;
+   const int tile_cells = 1 << a.shift;
+   double *acc = (double *)fl3_lds;               // [5][tile_cells]
+   double *dm = acc + 5 * tile_cells;             // [1024]
+   double *vis = dm + 1024;                       // [256]
+   __shared__ int item_sh;
+   const double *pd = (const double *)(a.job + h->off_dmap);
+   const fl3_xform_pack *px = (const fl3_xform_pack *)(a.job + h->off_xf);
+   for (int e = threadIdx.x; e < 1024; e += blockDim.x) dm[e] = pd[e];
+   // (the walkers carry a visibility as a float)
+   for (int e = threadIdx.x; e < 256; e += blockDim.x) vis[e] = e < h->nxf ? (double)(float)px[e].vis : 1.0;
+   const int linear = h->palette_mode == 1;
+   const long long total_items = a.item_start[a.nbins];
+   const RT *sorted = (const RT *)a.final;
+   for (;;) {
+      __syncthreads();
+      if (threadIdx.x == 0) item_sh = (int)atomicAdd(a.item_counter, 1u);
+      for (int e = threadIdx.x; e < 5 * tile_cells; e += blockDim.x) acc[e] = 0.0;
+      __syncthreads();
+      const long long item = item_sh;
+      if (item >= total_items) break;
+      int lo = 0, hi = a.nbins - 1;
+      while (lo < hi) {  // last tile whose first item is <= item
+         const int mid = (lo + hi + 1) >> 1;
+         if ((long long)a.item_start[mid] <= item) lo = mid;
+         else hi = mid - 1;
+      }
+      const int b = lo;
+      const long long k = item - a.item_start[b];
+      const long long r0 = (long long)a.bin_start[b] + k * a.chunk;
+      const long long r1 = min((long long)a.bin_start[b + 1], r0 + a.chunk);
+      const bool single = (a.item_start[b + 1] - a.item_start[b]) == 1;
+      const unsigned cell0 = a.cell_base + ((unsigned)b << a.shift);
+      for (long long rb = r0; rb < r1; rb += FL3_ACC_UNROLL * blockDim.x) {
+         RT v[FL3_ACC_UNROLL];
+         for (int u = 0; u < FL3_ACC_UNROLL; u++) {
+            const long long r = rb + u * blockDim.x + threadIdx.x;
+            v[u] = r < r1 ? sorted[r] : (RT)FL3_REC_NONE;
+         }
+         for (int u = 0; u < FL3_ACC_UNROLL; u++) {
+            if (rec_none(v[u])) continue;
+            double c[4], w = 1.0;
+            unsigned local;
+            if (sizeof(RT) == 4) {
+               local = ((unsigned)v[u] & 0xffffffu) - cell0;
+               const double *e = dm + ((unsigned)v[u] >> 24) * 4;
+               for (int ch = 0; ch < 4; ch++) c[ch] = e[ch];
+            } else {
+               const unsigned long long x = (unsigned long long)v[u];
+               local = (unsigned)x - cell0;
+               const int ci = (int)((x >> 32) & 0xff), fn = (int)((x >> 40) & 0xff);
+               const double *e = dm + ci * 4;
+               if (linear) {
+                  const double f = (double)((float)(x >> 48) * (1.0f / 65535.0f));
+                  for (int ch = 0; ch < 4; ch++) c[ch] = e[ch] * (1.0 - f) + e[4 + ch] * f;
+               } else
+                  for (int ch = 0; ch < 4; ch++) c[ch] = e[ch];
+               w = vis[fn];
+            }
+            if (w == 1.0) {
+               for (int ch = 0; ch < 4; ch++) atomicAdd(&acc[ch * tile_cells + local], c[ch]);
+               atomicAdd(&acc[4 * tile_cells + local], 255.0);
+            } else {
+               for (int ch = 0; ch < 4; ch++) atomicAdd(&acc[ch * tile_cells + local], w * c[ch]);
+               atomicAdd(&acc[4 * tile_cells + local], w * 255.0);
+            }
+         }
+      }
+      __syncthreads();
+      double *bk = (double *)a.buckets + (size_t)cell0 * 5;
+      const long long ncells = (long long)a.width * a.height;
+      const int nvalid = (int)min((long long)tile_cells, ncells - (long long)cell0);
+      for (int e = threadIdx.x; e < nvalid * 5; e += blockDim.x) {
+         const int cl = e / 5, ch = e % 5;
+         const double x = acc[ch * tile_cells + cl];
+         if (x == 0.0) continue;
+         if (single)
+            bk[e] += x;
+         else
+            unsafeAtomicAdd(bk + e, x);
+      }
+   }
+}
+
+extern "C" size_t fl3k_bin_f64_lds_bytes(int shift) { return (size_t)40 * ((size_t)1 << shift) + 8 * 1280 + 64; }
+
 extern "C" size_t fl3k_bin_narrow_lds_bytes(int shift) { return (size_t)16 * ((size_t)1 << shift) + 4 * 1024 + 64; }
 extern "C" int fl3k_bin_pass_records(void) { return FL3_ACC_UNROLL * FL3_ACC_THREADS; }
 
@@ -544,6 +675,30 @@ extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_
    const unsigned gacc = (unsigned)ncu * (lds > 65536 ? 1 : 3);
    const int big = lds > 65536;
    const int cpt = (1 << a->shift) / FL3_ACC_THREADS;
+   if (format == FL3_BIN_F64_WIDE || format == FL3_BIN_F64_NARROW) {
+      // double buckets: 8- or 4-byte records, accumulated in double
+      const size_t lf = fl3k_bin_f64_lds_bytes(a->shift);
+      const unsigned gf = (unsigned)ncu * (lf > 81920 ? 1 : (lf > 54000 ? 2 : 3));
+      if (format == FL3_BIN_F64_NARROW) {
+         if (a->rows >= FL3_PART1_STAGED_ROWS)
+            hipLaunchKernelGGL(k_bin_part1_staged<unsigned>, dim3(a->nblk), dim3(256), 0, s, *a);
+         else
+            hipLaunchKernelGGL(k_bin_part1<unsigned>, dim3(a->nblk), dim3(256), 0, s, *a);
+         if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned>, dim3(g2), dim3(256), lds2_4, s, *a);
+         if (lf > 65536)
+            (void)hipFuncSetAttribute((const void *)k_bin_accum_f64<unsigned>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lf);
+         hipLaunchKernelGGL(k_bin_accum_f64<unsigned>, dim3(gf), dim3(FL3_ACC_THREADS), lf, s, *a);
+      } else {
+         hipLaunchKernelGGL(k_bin_part1<unsigned long long>, dim3(a->nblk), dim3(256), 0, s, *a);
+         if (g2) hipLaunchKernelGGL(k_bin_part2<unsigned long long>, dim3(g2), dim3(256), lds2_8, s, *a);
+         if (lf > 65536)
+            (void)hipFuncSetAttribute((const void *)k_bin_accum_f64<unsigned long long>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lf);
+         hipLaunchKernelGGL(k_bin_accum_f64<unsigned long long>, dim3(gf), dim3(FL3_ACC_THREADS), lf, s, *a);
+      }
+      return hipGetLastError() != hipSuccess;
+   }
    if ((format == 1 || format == 2) && a->round > 0 && (cpt == 2 || cpt == 4)) {
       const bool job = format == 2;
       if (job) {
@@ -594,6 +749,28 @@ extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_
          hipLaunchKernelGGL((k_bin_accum<unsigned long long, false>), dim3(gacc), dim3(FL3_ACC_THREADS), lds, s, *a);
       }
    }
+   return hipGetLastError() != hipSuccess;
+}
+
+// Banded histograms, first level: the walkers' 8-byte records grouped by
+// super-tile (the bands) into a->sorted: scans of the walkers' counts and the
+// first sort pass only.
+extern "C" int fl3k_launch_bin_split(const BinArgs *a, hipStream_t s) {
+   hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, *a);
+   hipLaunchKernelGGL(k_super_off, dim3(a->nsuper), dim3(256), 0, s, *a);
+   hipLaunchKernelGGL(k_bin_part1<unsigned long long>, dim3(a->nblk), dim3(256), 0, s, *a);
+   return hipGetLastError() != hipSuccess;
+}
+
+// Banded histograms, second level: per-tile and per-block counts of one
+// band's 8-byte records (a->rec, a->rec_count of them, as a->nblk blocks of
+// a->rows x 256), before fl3k_launch_bin sorts and accumulates them.
+extern "C" int fl3k_launch_bin_count(const BinArgs *a, hipStream_t s) {
+   const size_t l = sizeof(unsigned) * (size_t)a->nbins;
+   if (l > 65536)
+      (void)hipFuncSetAttribute((const void *)k_bin_count<unsigned long long>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)l);
+   hipLaunchKernelGGL(k_bin_count<unsigned long long>, dim3(a->nblk), dim3(256), l, s, *a);
    return hipGetLastError() != hipSuccess;
 }
 

@@ -17,8 +17,14 @@ int fl3k_launch_apply_fp32(const unsigned char *job, int xi, const double *pts, 
                            unsigned long long key, size_t lds, hipStream_t s, int tier);
 int fl3k_launch_apply_fp32_tier(const unsigned char *job, int xi, const double *pts, long long n, double *out,
                                 unsigned long long key, size_t lds, hipStream_t s, int tier);
-/* format: 0 8-byte records, 1 4-byte records, 2 8-byte job records */
+/* format: 0 8-byte records, 1 4-byte records, 2 8-byte job records;
+ * FL3_BIN_F64_WIDE / FL3_BIN_F64_NARROW: 8- / 4-byte records into double buckets */
 int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_t s);
+/* banded histograms: the first level (scans and the band pass), the second
+ * level's counting pass */
+int fl3k_launch_bin_split(const BinArgs *a, hipStream_t s);
+int fl3k_launch_bin_count(const BinArgs *a, hipStream_t s);
+size_t fl3k_bin_f64_lds_bytes(int shift);
 int fl3k_bin_pass_records(void);   // records one accumulation workgroup loads per pass
 int fl3k_launch_copy_probe(const void *src, void *dst, long long n16, hipStream_t s);
 size_t fl3k_bin_lds_bytes(int shift);

@@ -343,10 +343,64 @@ def test_binned_deposit_linear_palette_close(prod):
     assert d.max() <= 2 and (d > 0).mean() < 0.03
 
 
-def test_binned_deposit_rejects_bits64(prod):
-    g = G.synthetic_small(seed=87, width=32, height=24, quality=2)
+# (name, genome, walkers, samples per chunk): bits-64 renders through the
+# tile-binned deposit into double buckets
+BITS64_CASES = [
+    # stepped palette, visibility 1: 4-byte records
+    ("narrow", lambda: G.synthetic_small(seed=87, width=128, height=96, quality=120), 1024, 1024 * 400 + 9),
+    # an xform of opacity 0.5: 8-byte records carrying the xform
+    ("wide_opacity", lambda: _with_opacity(G.synthetic_small(seed=84, width=128, height=96, quality=120)), 1024,
+     1024 * 500),
+    # 4200 x 4100 cells: more than 2^13 tiles of 2048 cells, so the records
+    # are grouped by band (2^24 cells) first and each band is sorted alone
+    ("banded", lambda: G.synthetic_small(seed=83, width=4200, height=4100, quality=0.12), 1024, 1024 * 700),
+]
+
+
+@pytest.mark.parametrize("name,mk,walkers,chunk", BITS64_CASES, ids=[c[0] for c in BITS64_CASES])
+def test_binned_bits64_equals_atomic_bits64(prod, name, mk, walkers, chunk):
+    """Bits-64 buckets through the tile-binned deposit: each record adds in
+    double what the per-sample double atomics add for its point, so the
+    buckets differ only in the order of the additions (a few ulps), and the
+    8-bit images with density estimation off agree but for a rare rounding
+    boundary."""
+    g = mk()
+    g.estimator_radius = 0.0
+    a, sa = prod.render(g, seed="b64", bits=64, nwalkers=walkers, deposit=1)
+    b, sb = prod.render(g, seed="b64", bits=64, nwalkers=walkers, deposit=2, chunk_samples=chunk)
+    assert sa["num_iters"] == sb["num_iters"] and sa["badvals"] == sb["badvals"]
+    assert sa["num_iters"] >= 2 * chunk   # several chunks
+    d = np.abs(a.astype(int) - b.astype(int))
+    print("%s: %d of %d channels differ, max %d" % (name, (d > 0).sum(), d.size, d.max()), flush=True)
+    assert d.max() <= 1 and (d > 0).mean() < 1e-4
+    assert a.mean() > 0.5
+
+
+def test_banded_bits33_equals_atomic(prod):
+    """A 6000 x 6000 histogram (more than 2^13 tiles of 4096 cells): records
+    grouped by band, then per band counted, sorted and accumulated; integer
+    sums, so the image equals the per-sample atomics' exactly."""
+    g = G.synthetic_small(seed=82, width=6000, height=6000, quality=0.06)
+    g.estimator_radius = 0.0
+    a, sa = prod.render(g, seed="band", nwalkers=1024, deposit=1)
+    b, sb = prod.render(g, seed="band", nwalkers=1024, deposit=2, chunk_samples=1024 * 600)
+    assert sa["num_iters"] == sb["num_iters"] and sa["num_iters"] >= 2 * 1024 * 600
+    assert sa["badvals"] == sb["badvals"]
+    assert np.array_equal(a, b), np.abs(a.astype(int) - b.astype(int)).max()
+    assert a.mean() > 0.05
+
+
+def test_binned_bits64_rejects_differing_tables(prod):
+    """Double buckets take one palette for all temporal samples: an
+    animation frame between control points of different palettes is refused
+    by the binned deposit (and renders with atomics by default)."""
+    gs = _animated_pair(95, 0.0)
+    for g in gs:
+        g.width, g.height, g.quality = 256, 192, 30
     with pytest.raises(RuntimeError):
-        prod.render(g, seed="b64", bits=64, deposit=2)
+        prod.render(gs, seed="b64", bits=64, time=0.5, deposit=2)
+    img, _ = prod.render(gs, seed="b64", bits=64, time=0.5)
+    assert img.mean() > 0.5
 
 
 def _animated_pair(seed, de):
