@@ -279,16 +279,19 @@ def config5_bench(args):
         st = fl3h_stage_times()
         L.fl3h_profile_get(ctypes.byref(st))
         L.fl3h_profile_reset(0)
-        log("8K q%g bits 64: %.2f s, iterate %.1f ms, DE %.1f ms, tonemap %.1f ms" % (
-            args.config5_quality, dt, st.iterate_ms, st.density_ms, st.tonemap_ms))
+        log("8K q%g bits 64: %.2f s, iterate %.1f ms (%d launches), bin %.1f ms (%d chunks), DE %.1f ms, "
+            "tonemap %.1f ms" % (args.config5_quality, dt, st.iterate_ms, st.iterate_launches, st.bin_ms,
+                                 st.bin_launches, st.density_ms, st.tonemap_ms))
         return {"workload": "8K still: 8 xforms + final xform + xaos, 7680x4320 ss3, DE radius 15, bits 64, "
                             "quality %g" % args.config5_quality,
                 "samples": stats.num_iters, "frame_s": dt, "samples_per_s": stats.num_iters / dt,
                 "iterate_ms": st.iterate_ms, "iterate_launches": st.iterate_launches,
+                "bin_ms": st.bin_ms, "bin_chunks": st.bin_launches,
                 "density_ms": st.density_ms, "tonemap_ms": st.tonemap_ms, "cells": st.cells,
-                "deposit": "per-sample FP64 atomics (bits 64 is not binned)",
-                "accumulation_hbm_frac": stats.num_iters * 80.0 / (st.iterate_ms * 1e-3) / (HBM_PEAK_GBS * 1e9)
-                if st.iterate_ms > 0 else None}
+                "deposit": "tile-binned: 8-byte records grouped by band (2^24 cells), then per band counted, "
+                           "sorted by tile and summed in double LDS tiles",
+                "accumulation_hbm_frac": stats.num_iters * 80.0 / (st.bin_ms * 1e-3) / (HBM_PEAK_GBS * 1e9)
+                if st.bin_ms > 0 else None}
     finally:
         for gp in owned:
             G.free_struct(gp)

@@ -96,7 +96,7 @@ struct Ctx {
    DevBuf detab, deargs;               // batched density estimation: tables, per-render arguments
    DevBuf deblk;                       // DE: widest source kernel per 16x16 block
    DevBuf detaps;                      // DE gather footprint counters (stage profile)
-   DevBuf blk2, soff2;                 // banded histograms: second-level block counts and offsets
+   DevBuf blk2, soff2, bins2;          // banded histograms: second-level block counts, offsets, counters
    unsigned *band_host = nullptr;      // pinned: first-level tile starts of the last chunk
    hipEvent_t ev_band = nullptr;
    unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
@@ -726,17 +726,16 @@ static int bin_banded(const BinArgs &b1, int format, int band, int shift, hipStr
          return 1;
       b.blk_hist = (unsigned *)g_ctx.blk2.p;
       b.soff = (unsigned *)g_ctx.soff2.p;
-      // (the first level's counters are spent: the band's reuse their buffer)
-      unsigned *bw = (unsigned *)g_ctx.bins.p;
+      // the bands' counters, at fixed places for any tile count (the tile
+      // counts start at zero and each scan leaves them there)
+      unsigned *bw = (unsigned *)g_ctx.bins2.p;
       b.bin_total = bw;
-      b.bin_start = bw + b.nbins;
-      b.item_start = bw + 2 * b.nbins + 1;
-      b.item_counter = bw + 3 * b.nbins + 2;
-      b.cursor = bw + 3 * b.nbins + 4;
+      b.bin_start = bw + FL3_MAX_BINS;
+      b.item_start = bw + 2 * FL3_MAX_BINS + 1;
+      b.item_counter = bw + 3 * FL3_MAX_BINS + 2;
+      b.cursor = bw + 3 * FL3_MAX_BINS + 4;
       b.sorted = (void *)b1.rec;                                // the walker buffer, free after the band pass
       b.final = (unsigned char *)b1.sorted + rsz * st[t0];      // the band's own range
-      HIPCHK(hipMemsetAsync(b.bin_total, 0, sizeof(unsigned) * (size_t)b.nbins, bs));
-      HIPCHK(hipMemsetAsync(b.item_counter, 0, sizeof(unsigned), bs));
       if (fl3k_launch_bin_count(&b, bs) || fl3k_launch_bin(&b, g_ctx.ncu, format, bs)) return 1;
    }
    return 0;
@@ -782,8 +781,11 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    if (ensure_chunk_bufs(nchunks, rec_bytes, sizeof(unsigned) * (size_t)nblk * nsup, sizeof(unsigned) * (size_t)nbins))
       return 1;
    const int nsuper = (nbins + (1 << FL3_SUPER_SHIFT) - 1) >> FL3_SUPER_SHIFT;
-   // (the bands' counters reuse these: room for FL3_MAX_BINS tiles)
-   if (g_ctx.bins.ensure(sizeof(unsigned) * (4 * (size_t)(band ? FL3_MAX_BINS : nbins) + 8))) return 1;
+   if (g_ctx.bins.ensure(sizeof(unsigned) * (4 * (size_t)nbins + 8))) return 1;
+   if (band) {
+      if (g_ctx.bins2.ensure(sizeof(unsigned) * (4 * (size_t)FL3_MAX_BINS + 8))) return 1;
+      HIPCHK(hipMemsetAsync(g_ctx.bins2.p, 0, sizeof(unsigned) * FL3_MAX_BINS, g_ctx.stream));
+   }
    if (g_ctx.soff.ensure(sizeof(unsigned) * (size_t)nblk * nsuper)) return 1;
    unsigned *bw = (unsigned *)g_ctx.bins.p;
    BinArgs b;
@@ -849,7 +851,6 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
       }
       HIPCHK(hipEventRecord(g_ctx.ev_walk[buf], ws));
       HIPCHK(hipStreamWaitEvent(bs, g_ctx.ev_walk[buf], 0));
-      HIPCHK(hipMemsetAsync(b.item_counter, 0, sizeof(unsigned), bs));
       b.rec = a.rec;
       b.final = a.rec;   // the walker buffer takes the tile-grouped records
       b.blk_hist = a.blk_hist;
@@ -1228,8 +1229,7 @@ extern "C" int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fus
          }
          HIPCHK(hipEventRecord(g_ctx.ev_walk[buf], ws));
          HIPCHK(hipStreamWaitEvent(bs, g_ctx.ev_walk[buf], 0));
-         HIPCHK(hipMemsetAsync(b.item_counter, 0, sizeof(unsigned), bs));
-         b.rec = ar.rec;
+            b.rec = ar.rec;
          b.final = ar.rec;
          b.blk_hist = ar.blk_hist;
          b.bin_total = ar.bin_total;

@@ -65,10 +65,12 @@ extern "C" int fl3k_launch_apply_fp32(const unsigned char *job, int xi, const do
 #endif
 
 // One workgroup: exclusive scans of the tile counts and of the work items
-// (ceil(count / chunk) per tile).
+// (ceil(count / chunk) per tile).  It also leaves the tile counts at zero
+// for their next use and resets the accumulation's item counter.
 __global__ __launch_bounds__(1024) void k_bin_scan(BinArgs a) {
    __shared__ unsigned long long part[1024], ipart[1024];
    const int t = threadIdx.x, per = (a.nbins + 1023) / 1024;
+   if (t == 0) *a.item_counter = 0u;
    const int b0 = t * per, b1 = min(a.nbins, b0 + per);
    unsigned long long s = 0, si = 0;
    for (int b = b0; b < b1; b++) {
@@ -92,6 +94,7 @@ __global__ __launch_bounds__(1024) void k_bin_scan(BinArgs a) {
       a.item_start[b] = (unsigned)irun;
       run += a.bin_total[b];
       irun += (a.bin_total[b] + a.chunk - 1) / a.chunk;
+      a.bin_total[b] = 0u;
    }
    if (t == 1023) {
       a.bin_start[a.nbins] = (unsigned)part[1023];
@@ -516,19 +519,63 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_narrow(BinArgs a)
       uint32_t *bk = (uint32_t *)a.buckets + (size_t)cell0 * 5;
       const long long ncells = (long long)a.width * a.height;
       const int nvalid = (int)min((long long)tile_cells, ncells - (long long)cell0);
-      for (int j = 0; j < CPT; j++) {
-         const int e = threadIdx.x + j * FL3_ACC_THREADS;
-         if (e >= nvalid) continue;
-         for (int c = 0; c < 5; c++) {
-            const uint32_t v = acc[j][c];
-            if (!v) continue;
-            uint32_t *dst = bk + (size_t)e * 5 + c;
-            if (single) {
-               const unsigned long long s2 = (unsigned long long)*dst + v;
-               *dst = s2 > 0xffffffffull ? 0xffffffffu : (uint32_t)s2;
-            } else
-               sat_add(dst, v);
+      // every load (or atomic add) of the thread's cells is issued before
+      // any result is used, so that their latencies overlap
+      uint32_t old[CPT][5];
+      if (nvalid == tile_cells) {
+         // (a tile inside the histogram: straight-line code over all its
+         // cells, so that no store waits for an earlier one; a tile of one
+         // work item owns its cells, and adding zero leaves a cell as it is)
+         uint32_t *bt = bk + (size_t)threadIdx.x * 5;
+         if (single) {
+            for (int j = 0; j < CPT; j++)
+               for (int c = 0; c < 5; c++) old[j][c] = bt[(size_t)j * FL3_ACC_THREADS * 5 + c];
+            for (int j = 0; j < CPT; j++)
+               for (int c = 0; c < 5; c++) {
+                  const unsigned long long s2 = (unsigned long long)old[j][c] + acc[j][c];
+                  old[j][c] = s2 > 0xffffffffull ? 0xffffffffu : (uint32_t)s2;
+               }
+            for (int j = 0; j < CPT; j++)
+               for (int c = 0; c < 5; c++) bt[(size_t)j * FL3_ACC_THREADS * 5 + c] = old[j][c];
+         } else {
+            for (int j = 0; j < CPT; j++)
+               for (int c = 0; c < 5; c++) old[j][c] = atomicAdd(bt + (size_t)j * FL3_ACC_THREADS * 5 + c, acc[j][c]);
+            for (int j = 0; j < CPT; j++)
+               for (int c = 0; c < 5; c++)
+                  if (old[j][c] > 0xffffffffu - acc[j][c])
+                     atomicExch(bt + (size_t)j * FL3_ACC_THREADS * 5 + c, 0xffffffffu);
          }
+      } else if (single) {
+         for (int j = 0; j < CPT; j++)
+            for (int c = 0; c < 5; c++) {
+               const int e = threadIdx.x + j * FL3_ACC_THREADS;
+               old[j][c] = e < nvalid && acc[j][c] ? bk[(size_t)e * 5 + c] : 0u;
+            }
+         // (the sums first, each in a register of its own, so that no store
+         // waits for an earlier one to release its data register)
+         for (int j = 0; j < CPT; j++)
+            for (int c = 0; c < 5; c++) {
+               const unsigned long long s2 = (unsigned long long)old[j][c] + acc[j][c];
+               old[j][c] = s2 > 0xffffffffull ? 0xffffffffu : (uint32_t)s2;
+            }
+         for (int j = 0; j < CPT; j++)
+            for (int c = 0; c < 5; c++) {
+               const int e = threadIdx.x + j * FL3_ACC_THREADS;
+               if (e < nvalid && acc[j][c]) bk[(size_t)e * 5 + c] = old[j][c];
+            }
+      } else {
+         for (int j = 0; j < CPT; j++)
+            for (int c = 0; c < 5; c++) {
+               const int e = threadIdx.x + j * FL3_ACC_THREADS;
+               old[j][c] = e < nvalid && acc[j][c] ? atomicAdd(bk + (size_t)e * 5 + c, acc[j][c]) : 0u;
+            }
+         // (a saturating add: the sum stops at UINT_MAX, as sat_add)
+         for (int j = 0; j < CPT; j++)
+            for (int c = 0; c < 5; c++) {
+               const int e = threadIdx.x + j * FL3_ACC_THREADS;
+               const uint32_t v = acc[j][c];
+               if (e < nvalid && v && old[j][c] > 0xffffffffu - v) atomicExch(bk + (size_t)e * 5 + c, 0xffffffffu);
+            }
       }
    }
 }
@@ -640,15 +687,23 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_f64(BinArgs a) {
       double *bk = (double *)a.buckets + (size_t)cell0 * 5;
       const long long ncells = (long long)a.width * a.height;
       const int nvalid = (int)min((long long)tile_cells, ncells - (long long)cell0);
-      for (int e = threadIdx.x; e < nvalid * 5; e += blockDim.x) {
-         const int cl = e / 5, ch = e % 5;
-         const double x = acc[ch * tile_cells + cl];
-         if (x == 0.0) continue;
-         if (single)
-            bk[e] += x;
-         else
-            unsafeAtomicAdd(bk + e, x);
+      // a thread's values of the tile (at most 5 x 2048 / FL3_ACC_THREADS),
+      // their loads all issued before any is used
+      constexpr int K = 5 * 2048 / FL3_ACC_THREADS;
+      double val[K];
+      for (int k = 0; k < K; k++) {
+         const int e = threadIdx.x + k * FL3_ACC_THREADS;
+         val[k] = e < nvalid * 5 ? acc[(e % 5) * tile_cells + e / 5] : 0.0;
       }
+      if (single) {
+         double old[K];
+         for (int k = 0; k < K; k++) old[k] = val[k] != 0.0 ? bk[threadIdx.x + k * FL3_ACC_THREADS] : 0.0;
+         for (int k = 0; k < K; k++) old[k] += val[k];
+         for (int k = 0; k < K; k++)
+            if (val[k] != 0.0) bk[threadIdx.x + k * FL3_ACC_THREADS] = old[k];
+      } else
+         for (int k = 0; k < K; k++)
+            if (val[k] != 0.0) unsafeAtomicAdd(bk + threadIdx.x + k * FL3_ACC_THREADS, val[k]);
    }
 }
 
