@@ -510,7 +510,7 @@ static int acc_tile_shift(long long ncells, int bits, int *band) {
 // point has visibility 1 and the palette is stepped: the deposit is then a
 // function of the cell and the palette index alone.
 static bool narrow_records_ok(const fl3_job_hdr *h, long long ncells) {
-   if (h->palette_mode != 0 || ncells >= (1LL << 24) - 1) return false;
+   if (h->palette_mode != 0 || (ncells >= 0 && ncells >= (1LL << 24) - 1)) return false;
    const fl3_xform_pack *px = (const fl3_xform_pack *)((const unsigned char *)h + h->off_xf);
    for (int e = 0; e < h->nstd; e++)
       if (px[e].vis != 1.0) return false;
@@ -697,14 +697,17 @@ static size_t with_shuffle(IterArgs &a, size_t lds, bool want) {
 // walker buffer) back into the band's range and the accumulation.  The
 // host reads the band boundaries, so it waits for this chunk's walkers.
 static int bin_banded(const BinArgs &b1, int format, int band, int shift, hipStream_t bs) {
-   if (fl3k_launch_bin_split(&b1, bs)) return 1;
+   // double buckets of 2^24-cell bands, visibility 1, stepped palette: the
+   // band pass writes 4-byte records (cell within the band | palette index)
+   const int narrow = format == FL3_BIN_F64_NARROW;
+   if (fl3k_launch_bin_split(&b1, narrow, bs)) return 1;
    HIPCHK(hipMemcpyAsync(g_ctx.band_host, b1.bin_start, sizeof(unsigned) * ((size_t)b1.nbins + 1),
                          hipMemcpyDeviceToHost, bs));
    HIPCHK(hipEventRecord(g_ctx.ev_band, bs));
    HIPCHK(hipEventSynchronize(g_ctx.ev_band));
    const long long ncells = (long long)g_ctx.width * g_ctx.height;
    const int rows2 = 64;   // second-level blocks: 64 x 256 records
-   const size_t rsz = format == FL3_BIN_F64_WIDE || format == 0 ? 8 : 4;
+   const size_t rsz = narrow ? 4 : 8;
    const unsigned *st = g_ctx.band_host;
    for (int j = 0; j < b1.nsuper; j++) {
       const int t0 = j << FL3_SUPER_SHIFT, t1 = std::min(b1.nbins, t0 + (1 << FL3_SUPER_SHIFT));
@@ -712,6 +715,7 @@ static int bin_banded(const BinArgs &b1, int format, int band, int shift, hipStr
       if (n <= 0) continue;
       BinArgs b = b1;
       b.cell_base = (unsigned)((long long)j << band);
+      b.rec_rel = narrow;
       b.rec = (const unsigned char *)b1.sorted + rsz * st[t0];
       b.rec_count = n;
       b.rows = rows2;
@@ -736,7 +740,7 @@ static int bin_banded(const BinArgs &b1, int format, int band, int shift, hipStr
       b.cursor = bw + 3 * FL3_MAX_BINS + 4;
       b.sorted = (void *)b1.rec;                                // the walker buffer, free after the band pass
       b.final = (unsigned char *)b1.sorted + rsz * st[t0];      // the band's own range
-      if (fl3k_launch_bin_count(&b, bs) || fl3k_launch_bin(&b, g_ctx.ncu, format, bs)) return 1;
+      if (fl3k_launch_bin_count(&b, narrow, bs) || fl3k_launch_bin(&b, g_ctx.ncu, format, bs)) return 1;
    }
    return 0;
 }
@@ -753,10 +757,15 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    const int nbins = (int)((ncells + (1LL << wshift) - 1) >> wshift);
    bool narrow = !band && narrow_records_ok(js.h[0], ncells);
    for (int j = 1; narrow && j < js.n; j++) narrow = same_narrow_tables(js.h[0], js.h[j]);
+   // double buckets of a banded histogram: the walkers write 8-byte records,
+   // the band pass 4-byte ones (cells within a 2^24-cell band) when the
+   // tables allow
+   bool band_narrow = band == 24 && g_ctx.bits == 64 && narrow_records_ok(js.h[0], -1);
+   for (int j = 1; band_narrow && j < js.n; j++) band_narrow = same_narrow_tables(js.h[0], js.h[j]);
    // one palette for all jobs: narrow or wide records; else the job index
    // rides in the record (step palettes only: it takes the fraction's bits);
    // double buckets take narrow or wide records
-   const int format = g_ctx.bits == 64 ? (narrow ? FL3_BIN_F64_NARROW : FL3_BIN_F64_WIDE)
+   const int format = g_ctx.bits == 64 ? (narrow || band_narrow ? FL3_BIN_F64_NARROW : FL3_BIN_F64_WIDE)
                                        : (narrow ? 1 : (js.same_tables ? 0 : 2));
    long long per = p->chunk > 0 ? (p->chunk + W - 1) / W : (g_tune.chunk_samples / W);
    // banded: chunks of at least 4 samples per cell (up to 2^31 records), as

@@ -119,6 +119,7 @@ struct BinArgs {
    // (0: every slot of the blocks is a record or FL3_REC_NONE)
    unsigned cell_base;
    long long rec_count;
+   int rec_rel;                    // records hold cells relative to cell_base (4-byte band records)
 };
 
 struct PointsArgs {

@@ -110,10 +110,11 @@ template <typename RT>
 __device__ __forceinline__ bool rec_none(RT v) {
    return (unsigned)v == FL3_REC_NONE;
 }
-// the record's cell relative to the band being sorted
+// the record's cell relative to the band being sorted (records of a band
+// pass hold that already)
 template <typename RT>
 __device__ __forceinline__ unsigned rel_cell(const BinArgs &a, RT v) {
-   return rec_cell(v) - a.cell_base;
+   return a.rec_rel ? rec_cell(v) : rec_cell(v) - a.cell_base;
 }
 // slot idx of the input blocks: past rec_count (when set) there are no records
 template <typename RT>
@@ -170,7 +171,17 @@ __device__ __forceinline__ unsigned wg_excl_scan256(unsigned v, unsigned *wsum, 
 // at most 256 super-tiles per workgroup the partly written lines stay in L2
 // until they fill.
 #define FL3_SCATTER_UNROLL 16
-template <typename RT>
+// The record as pass 1 writes it: unchanged, or (OT 4 bytes from 8-byte
+// records, the band pass of a histogram of 2^24-cell bands, visibility 1
+// and a stepped palette) its cell within the super-tile | palette index << 24.
+template <typename RT, typename OT>
+__device__ __forceinline__ OT part1_out(RT v, int ss) {
+   if (sizeof(OT) == sizeof(RT)) return (OT)v;
+   const unsigned long long x = (unsigned long long)v;
+   return (OT)(((unsigned)x & ((1u << ss) - 1u)) | (((unsigned)(x >> 32) & 0xffu) << 24));
+}
+
+template <typename RT, typename OT = RT>
 __global__ __launch_bounds__(256) void k_bin_part1(BinArgs a) {
    __shared__ unsigned base[256], cnt[256];
    const int ss = a.shift + FL3_SUPER_SHIFT;
@@ -180,7 +191,7 @@ __global__ __launch_bounds__(256) void k_bin_part1(BinArgs a) {
    }
    __syncthreads();
    const RT *rec = (const RT *)a.rec;
-   RT *out = (RT *)a.sorted;
+   OT *out = (OT *)a.sorted;
    long long i = 0;
    for (; i + FL3_SCATTER_UNROLL <= a.rows; i += FL3_SCATTER_UNROLL) {
       RT v[FL3_SCATTER_UNROLL];
@@ -188,14 +199,14 @@ __global__ __launch_bounds__(256) void k_bin_part1(BinArgs a) {
       for (int k = 0; k < FL3_SCATTER_UNROLL; k++) {
          if (rec_none(v[k])) continue;
          const unsigned s = rel_cell(a, v[k]) >> ss;
-         out[base[s] + atomicAdd(&cnt[s], 1u)] = v[k];
+         out[base[s] + atomicAdd(&cnt[s], 1u)] = part1_out<RT, OT>(v[k], ss);
       }
    }
    for (; i < a.rows; i++) {
       const RT v = rec_at(a, rec, (blockIdx.x * a.rows + i) * 256 + threadIdx.x);
       if (rec_none(v)) continue;
       const unsigned s = rel_cell(a, v) >> ss;
-      out[base[s] + atomicAdd(&cnt[s], 1u)] = v;
+      out[base[s] + atomicAdd(&cnt[s], 1u)] = part1_out<RT, OT>(v, ss);
    }
 }
 
@@ -246,47 +257,54 @@ __global__ __launch_bounds__(256) void k_bin_part1_staged(BinArgs a) {
 }
 
 // Pass 2: workgroup c moves the super-tile-grouped records
-// [c * FL3_PART2_RECORDS, ...) to their tiles: ranks within the workgroup
-// from LDS counters, each tile's slots reserved with one global atomic.  The
-// order within a tile is arbitrary (the accumulation adds integers).  When
-// 4-byte records span at most 256 tiles (8 super-tiles, the usual case) they
-// are ordered by tile in LDS first and written out from there, consecutive
-// lanes to consecutive slots (8-byte records would need twice the LDS and
-// halve the workgroups a CU holds, which costs more than it saves).
+// [c * FL3_PART2_RECORDS, ...) to their tiles.  The records of a workgroup
+// usually span fewer than 256 tiles (a few super-tiles): ranks within the
+// workgroup from LDS counters of that span, each tile's slots reserved with
+// one global atomic, and 4-byte records ordered by tile in LDS first and
+// written out from there, consecutive lanes to consecutive slots (8-byte
+// records would need twice the LDS and halve the workgroups a CU holds, so
+// they are written from registers).  A wider span (records of many sparse
+// tiles) reserves one slot per record.  The order within a tile is
+// arbitrary: the accumulation adds integers (or doubles, whose sums then
+// differ in rounding only).
 #define FL3_PART2_PER 32
 #define FL3_PART2_RECORDS (256 * FL3_PART2_PER)
 template <typename RT>
 __global__ __launch_bounds__(256) void k_bin_part2(BinArgs a) {
-   unsigned *cnt = (unsigned *)fl3_lds;   // [nbins]
-   RT *stage = (RT *)(fl3_lds + ((sizeof(unsigned) * (size_t)a.nbins + 15) & ~(size_t)15));   // [FL3_PART2_RECORDS]
-   __shared__ unsigned lofs[256], gofs[256], wsum[4];
+   RT *stage = (RT *)fl3_lds;   // 4-byte records: [FL3_PART2_RECORDS]
+   __shared__ unsigned lcnt[256], lofs[256], gofs[256], wsum[4];
    const long long total = a.bin_start[a.nbins];
    const long long r0 = (long long)blockIdx.x * FL3_PART2_RECORDS;
    if (r0 >= total) return;
-   for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) cnt[b] = 0;
-   __syncthreads();
    const RT *in = (const RT *)a.sorted;
    RT *out = (RT *)a.final;
    RT v[FL3_PART2_PER];
-   unsigned rank[FL3_PART2_PER];
    for (int k = 0; k < FL3_PART2_PER; k++) {
       const long long r = r0 + k * 256 + threadIdx.x;
       v[k] = r < total ? in[r] : (RT)FL3_REC_NONE;
    }
-   for (int k = 0; k < FL3_PART2_PER; k++)
-      if (!rec_none(v[k])) rank[k] = atomicAdd(&cnt[rel_cell(a, v[k]) >> a.shift], 1u);
-   __syncthreads();
    const long long rl = (total < r0 + FL3_PART2_RECORDS ? total : r0 + FL3_PART2_RECORDS) - 1;
    const int t_lo = (int)((rel_cell(a, in[r0]) >> a.shift) & ~((1u << FL3_SUPER_SHIFT) - 1u));
    const int t_hi = (int)((rel_cell(a, in[rl]) >> a.shift) | ((1u << FL3_SUPER_SHIFT) - 1u));
-   if (sizeof(RT) == 4 && t_hi - t_lo < 256) {
-      const int b = t_lo + (int)threadIdx.x;
-      const unsigned c = b <= t_hi && b < a.nbins ? cnt[b] : 0u;
-      unsigned tot;
-      const unsigned l = wg_excl_scan256(c, wsum, &tot);
-      lofs[threadIdx.x] = l;
-      gofs[threadIdx.x] = c ? atomicAdd(&a.cursor[b], c) : 0u;
-      __syncthreads();
+   if (t_hi - t_lo >= 256) {
+      for (int k = 0; k < FL3_PART2_PER; k++)
+         if (!rec_none(v[k])) out[atomicAdd(&a.cursor[rel_cell(a, v[k]) >> a.shift], 1u)] = v[k];
+      return;
+   }
+   lcnt[threadIdx.x] = 0;
+   __syncthreads();
+   unsigned rank[FL3_PART2_PER];
+   for (int k = 0; k < FL3_PART2_PER; k++)
+      if (!rec_none(v[k])) rank[k] = atomicAdd(&lcnt[(rel_cell(a, v[k]) >> a.shift) - t_lo], 1u);
+   __syncthreads();
+   const int b = t_lo + (int)threadIdx.x;
+   const unsigned c = lcnt[threadIdx.x];   // (zero past t_hi and the last tile)
+   unsigned tot;
+   const unsigned l = wg_excl_scan256(c, wsum, &tot);
+   lofs[threadIdx.x] = l;
+   gofs[threadIdx.x] = c ? atomicAdd(&a.cursor[b], c) : 0u;
+   __syncthreads();
+   if (sizeof(RT) == 4) {
       for (int k = 0; k < FL3_PART2_PER; k++)
          if (!rec_none(v[k])) stage[lofs[(rel_cell(a, v[k]) >> a.shift) - t_lo] + rank[k]] = v[k];
       __syncthreads();
@@ -295,13 +313,9 @@ __global__ __launch_bounds__(256) void k_bin_part2(BinArgs a) {
          const unsigned bb = (rel_cell(a, w) >> a.shift) - (unsigned)t_lo;
          out[gofs[bb] + (j - lofs[bb])] = w;
       }
-      return;
-   }
-   for (int b = threadIdx.x; b < a.nbins; b += blockDim.x)
-      if (cnt[b]) cnt[b] = atomicAdd(&a.cursor[b], cnt[b]);
-   __syncthreads();
-   for (int k = 0; k < FL3_PART2_PER; k++)
-      if (!rec_none(v[k])) out[cnt[rel_cell(a, v[k]) >> a.shift] + rank[k]] = v[k];
+   } else
+      for (int k = 0; k < FL3_PART2_PER; k++)
+         if (!rec_none(v[k])) out[gofs[(rel_cell(a, v[k]) >> a.shift) - t_lo] + rank[k]] = v[k];
 }
 
 // Persistent workgroups take work items (tile, chunk of its records) from a
@@ -647,7 +661,8 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_f64(BinArgs a) {
       const long long r0 = (long long)a.bin_start[b] + k * a.chunk;
       const long long r1 = min((long long)a.bin_start[b + 1], r0 + a.chunk);
       const bool single = (a.item_start[b + 1] - a.item_start[b]) == 1;
-      const unsigned cell0 = a.cell_base + ((unsigned)b << a.shift);
+      const unsigned tb = (unsigned)b << a.shift;   // the tile's first cell in the band
+      const unsigned cell0 = a.cell_base + tb;
       for (long long rb = r0; rb < r1; rb += FL3_ACC_UNROLL * blockDim.x) {
          RT v[FL3_ACC_UNROLL];
          for (int u = 0; u < FL3_ACC_UNROLL; u++) {
@@ -657,14 +672,12 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_f64(BinArgs a) {
          for (int u = 0; u < FL3_ACC_UNROLL; u++) {
             if (rec_none(v[u])) continue;
             double c[4], w = 1.0;
-            unsigned local;
+            const unsigned local = rel_cell(a, v[u]) - tb;
             if (sizeof(RT) == 4) {
-               local = ((unsigned)v[u] & 0xffffffu) - cell0;
                const double *e = dm + ((unsigned)v[u] >> 24) * 4;
                for (int ch = 0; ch < 4; ch++) c[ch] = e[ch];
             } else {
                const unsigned long long x = (unsigned long long)v[u];
-               local = (unsigned)x - cell0;
                const int ci = (int)((x >> 32) & 0xff), fn = (int)((x >> 40) & 0xff);
                const double *e = dm + ci * 4;
                if (linear) {
@@ -720,12 +733,8 @@ extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_
    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, *a);
    hipLaunchKernelGGL(k_super_off, dim3(a->nsuper), dim3(256), 0, s, *a);
    const unsigned g2 = (unsigned)(((long long)a->walkers * a->rows + FL3_PART2_RECORDS - 1) / FL3_PART2_RECORDS);
-   // tile counters, then the staging area of the 4- or 8-byte records
-   const size_t lds2c = (sizeof(unsigned) * (size_t)a->nbins + 15) & ~(size_t)15;
-   const size_t lds2_4 = lds2c + 4 * (size_t)FL3_PART2_RECORDS, lds2_8 = lds2c;
-   if (lds2_4 > 65536)
-      (void)hipFuncSetAttribute((const void *)k_bin_part2<unsigned>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds2_4);
+   // the staging area of 4-byte records (8-byte records: none)
+   const size_t lds2_4 = 4 * (size_t)FL3_PART2_RECORDS, lds2_8 = 0;
    const size_t lds = fl3k_bin_lds_bytes(a->shift);
    const unsigned gacc = (unsigned)ncu * (lds > 65536 ? 1 : 3);
    const int big = lds > 65536;
@@ -810,22 +819,25 @@ extern "C" int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_
 // Banded histograms, first level: the walkers' 8-byte records grouped by
 // super-tile (the bands) into a->sorted: scans of the walkers' counts and the
 // first sort pass only.
-extern "C" int fl3k_launch_bin_split(const BinArgs *a, hipStream_t s) {
+extern "C" int fl3k_launch_bin_split(const BinArgs *a, int narrow_out, hipStream_t s) {
    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, *a);
    hipLaunchKernelGGL(k_super_off, dim3(a->nsuper), dim3(256), 0, s, *a);
-   hipLaunchKernelGGL(k_bin_part1<unsigned long long>, dim3(a->nblk), dim3(256), 0, s, *a);
+   if (narrow_out)
+      hipLaunchKernelGGL((k_bin_part1<unsigned long long, unsigned>), dim3(a->nblk), dim3(256), 0, s, *a);
+   else
+      hipLaunchKernelGGL(k_bin_part1<unsigned long long>, dim3(a->nblk), dim3(256), 0, s, *a);
    return hipGetLastError() != hipSuccess;
 }
 
 // Banded histograms, second level: per-tile and per-block counts of one
 // band's 8-byte records (a->rec, a->rec_count of them, as a->nblk blocks of
 // a->rows x 256), before fl3k_launch_bin sorts and accumulates them.
-extern "C" int fl3k_launch_bin_count(const BinArgs *a, hipStream_t s) {
-   const size_t l = sizeof(unsigned) * (size_t)a->nbins;
-   if (l > 65536)
-      (void)hipFuncSetAttribute((const void *)k_bin_count<unsigned long long>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)l);
-   hipLaunchKernelGGL(k_bin_count<unsigned long long>, dim3(a->nblk), dim3(256), l, s, *a);
+extern "C" int fl3k_launch_bin_count(const BinArgs *a, int narrow, hipStream_t s) {
+   const size_t l = sizeof(unsigned) * (size_t)a->nbins;   // at most FL3_MAX_BINS: 32 KiB
+   if (narrow)
+      hipLaunchKernelGGL(k_bin_count<unsigned>, dim3(a->nblk), dim3(256), l, s, *a);
+   else
+      hipLaunchKernelGGL(k_bin_count<unsigned long long>, dim3(a->nblk), dim3(256), l, s, *a);
    return hipGetLastError() != hipSuccess;
 }
 

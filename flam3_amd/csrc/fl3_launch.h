@@ -20,10 +20,10 @@ int fl3k_launch_apply_fp32_tier(const unsigned char *job, int xi, const double *
 /* format: 0 8-byte records, 1 4-byte records, 2 8-byte job records;
  * FL3_BIN_F64_WIDE / FL3_BIN_F64_NARROW: 8- / 4-byte records into double buckets */
 int fl3k_launch_bin(const BinArgs *a, int ncu, int format, hipStream_t s);
-/* banded histograms: the first level (scans and the band pass), the second
- * level's counting pass */
-int fl3k_launch_bin_split(const BinArgs *a, hipStream_t s);
-int fl3k_launch_bin_count(const BinArgs *a, hipStream_t s);
+/* banded histograms: the first level (scans and the band pass, writing
+ * 4-byte band records when narrow_out), the second level's counting pass */
+int fl3k_launch_bin_split(const BinArgs *a, int narrow_out, hipStream_t s);
+int fl3k_launch_bin_count(const BinArgs *a, int narrow, hipStream_t s);
 size_t fl3k_bin_f64_lds_bytes(int shift);
 int fl3k_bin_pass_records(void);   // records one accumulation workgroup loads per pass
 int fl3k_launch_copy_probe(const void *src, void *dst, long long n16, hipStream_t s);

@@ -352,8 +352,12 @@ BITS64_CASES = [
     ("wide_opacity", lambda: _with_opacity(G.synthetic_small(seed=84, width=128, height=96, quality=120)), 1024,
      1024 * 500),
     # 4200 x 4100 cells: more than 2^13 tiles of 2048 cells, so the records
-    # are grouped by band (2^24 cells) first and each band is sorted alone
+    # are grouped by band (2^24 cells) first, into 4-byte band records, and
+    # each band is sorted alone
     ("banded", lambda: G.synthetic_small(seed=83, width=4200, height=4100, quality=0.12), 1024, 1024 * 700),
+    # the same with an xform of opacity 0.5: 8-byte records in the bands
+    ("banded_opacity", lambda: _with_opacity(G.synthetic_small(seed=81, width=4200, height=4100, quality=0.12)),
+     1024, 1024 * 700),
 ]
 
 
@@ -373,7 +377,7 @@ def test_binned_bits64_equals_atomic_bits64(prod, name, mk, walkers, chunk):
     d = np.abs(a.astype(int) - b.astype(int))
     print("%s: %d of %d channels differ, max %d" % (name, (d > 0).sum(), d.size, d.max()), flush=True)
     assert d.max() <= 1 and (d > 0).mean() < 1e-4
-    assert a.mean() > 0.5
+    assert a.mean() > 0.05   # (the sparse 4200 x 4100 images are dark)
 
 
 def test_banded_bits33_equals_atomic(prod):
