@@ -14,8 +14,11 @@ Workloads and the score each parameter is chosen on:
   500 (several chunks), plus that of one animation frame (1280x720
   supersample 2, 100 temporal samples, DE radius 7) at quality 200.
 * wg_per_cu_single (launches of one chunk): the animation frame alone.
-* min_samples (walker count of small renders): requests per second of 64
-  preview renders (64^2..512^2, quality 10..100) through fl3h_render_many.
+
+min_samples_small (samples per walker of small launches) is not tuned here: it
+trades preview latency against preview throughput (fewer samples per
+walker: shorter walkers, more fuse steps in all), a policy rather than a
+speed optimum; the library's default favours latency.
 
 Each timing is the median of seven repeats after a warm-up render.  A
 parameter keeps its current value unless a candidate beats it by more than
@@ -36,14 +39,13 @@ import numpy as np  # noqa: E402
 from flam3_amd import abi  # noqa: E402
 from flam3_amd import genome as G  # noqa: E402
 from flam3_amd._lib import lib  # noqa: E402
-from flam3_amd.render import make_frame, render_many  # noqa: E402
+from flam3_amd.render import make_frame  # noqa: E402
 
 CANDIDATES = {
     "wg_per_cu": [4, 6, 7, 8, 12],
     "wg_per_cu_single": [4, 6, 7, 8, 12],
     "chunk_samples": [1 << 27, 1 << 28, 1 << 29],
     "de_halo": [0, 1],
-    "min_samples": [128, 256, 512, 1024],
 }
 MARGIN = 0.03
 
@@ -70,7 +72,6 @@ class Frames:
     def __init__(self, quick):
         self.still = G.synthetic_1080p(seed=2, quality=100.0 if quick else 500.0)
         self.anim = G.synthetic_animation(seed=3, quality=50.0 if quick else 200.0)
-        self.previews = [G.synthetic_preview(1000 + k) for k in range(32 if quick else 64)]
         self.repeats = 3 if quick else 7
 
     def _render(self, gs, t=0.0):
@@ -101,15 +102,6 @@ class Frames:
         self._render(self.anim, 37.0)
         return statistics.median(self._render(self.anim, 37.0) for _ in range(self.repeats))
 
-    def preview_score(self):
-        items = [{"genomes": g, "seed": "tune-%d" % k} for k, g in enumerate(self.previews)]
-        render_many(items)
-        ts = []
-        for _ in range(self.repeats):
-            t0 = time.perf_counter()
-            render_many(items)
-            ts.append(time.perf_counter() - t0)
-        return statistics.median(ts)
 
 
 def tune(quick=False, log=print):
@@ -117,8 +109,7 @@ def tune(quick=False, log=print):
     best = {k: _get(k) for k in CANDIDATES}
     for rnd in range(2):
         for key, cands in CANDIDATES.items():
-            score_fn = {"min_samples": frames.preview_score,
-                        "wg_per_cu_single": frames.single_score}.get(key, frames.frame_score)
+            score_fn = {"wg_per_cu_single": frames.single_score}.get(key, frames.frame_score)
             _set(key, best[key])
             base = score_fn()
             results = {best[key]: base}
@@ -150,7 +141,7 @@ def main():
     a = p.parse_args()
     os.environ["FL3H_TUNING"] = os.devnull   # start from the defaults, not an earlier result
     best = tune(a.quick, log=lambda *m: print(*m, flush=True))
-    write_conf(a.out, best, "1080p q500 still + animation frame q200; 64 preview renders")
+    write_conf(a.out, best, "1080p q500 still + animation frame q200")
     print("wrote", a.out, best, flush=True)
 
 

@@ -39,6 +39,8 @@ typedef struct {
    int nwalkers;        /* 0: chosen by the library */
    int deposit;         /* 0 auto, 1 atomics, 2 tile-binned */
    int64_t chunk;       /* binned: samples per chunk, 0 = library choice */
+   const int64_t *job_nsamples;   /* fl3d_iterate_batch: job j plots job_nsamples[j] (at most
+                                     nsamples); NULL: nsamples each */
 } fl3d_iter_params;
 
 /* Iterate one job (one genome at one temporal sample) and deposit into the
@@ -66,7 +68,8 @@ long long fl3d_job_walkers(int64_t nsamples, int njobs, int requested);
 int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fuse, int exact, int nrenders,
                        double *badvals, int *saturated);
 /* The same for njobs jobs in one launch (the temporal samples of a batch):
- * p->nsamples samples each, job j's walkers keyed by keys[2j], keys[2j+1]. */
+ * p->nsamples samples each (or p->job_nsamples[j]), job j's walkers keyed by
+ * keys[2j], keys[2j+1]. */
 int fl3d_iterate_batch(const void *const *jobs, const size_t *job_bytes, int njobs, const fl3d_iter_params *p,
                        const uint64_t *keys, double *badvals, int *saturated);
 

@@ -356,7 +356,15 @@ static int read_counters(double *badvals, int *saturated) {
 struct Tuning {
    int wg_per_cu = 8;               // walker workgroups of 256 per CU (launches of several chunks)
    int wg_per_cu_single = 8;        // the same for launches of one chunk (no sort to overlap)
-   long long min_samples = 256;     // samples per walker at least
+   // samples per walker at least, for launches of at least small_samples
+   // samples; smaller ones (previews) take at least min_samples_small each:
+   // more walkers with fewer samples, so fewer steps in a row.  Every walker
+   // also pays its fuse steps, so fewer samples each cost throughput
+   // (preview serving with 64 / 128 / 256 for every launch: p50 1.27 / 1.63 /
+   // 2.48 ms at 1000 requests/s, 4700 / 5900 / 6900 requests/s all at once)
+   long long min_samples = 256;
+   long long min_samples_small = 128;
+   long long small_samples = 1LL << 23;
    long long chunk_samples = FL3_BIN_CHUNK_SAMPLES;   // binned deposit: samples per chunk
    int de_halo = 1;                 // DE gather halo: per region from its sources (1) or the widest kernel's (0)
 };
@@ -367,6 +375,8 @@ static int tune_set(const char *key, double v) {
    if (!strcmp(key, "wg_per_cu") && v >= 1 && v <= 16) g_tune.wg_per_cu = (int)v;
    else if (!strcmp(key, "wg_per_cu_single") && v >= 1 && v <= 16) g_tune.wg_per_cu_single = (int)v;
    else if (!strcmp(key, "min_samples") && v >= 1 && v <= 65536) g_tune.min_samples = (long long)v;
+   else if (!strcmp(key, "min_samples_small") && v >= 1 && v <= 65536) g_tune.min_samples_small = (long long)v;
+   else if (!strcmp(key, "small_samples") && v >= 0 && v <= (double)(1LL << 40)) g_tune.small_samples = (long long)v;
    else if (!strcmp(key, "chunk_samples") && v >= (1 << 20) && v <= (double)(1LL << 30))
       g_tune.chunk_samples = (long long)v;
    else if (!strcmp(key, "de_halo") && (v == 0 || v == 1)) g_tune.de_halo = (int)v;
@@ -409,6 +419,8 @@ extern "C" int fl3h_tuning(const char *key, const double *value, double *out) {
    if (!strcmp(key, "wg_per_cu")) v = g_tune.wg_per_cu;
    else if (!strcmp(key, "wg_per_cu_single")) v = g_tune.wg_per_cu_single;
    else if (!strcmp(key, "min_samples")) v = (double)g_tune.min_samples;
+   else if (!strcmp(key, "min_samples_small")) v = (double)g_tune.min_samples_small;
+   else if (!strcmp(key, "small_samples")) v = (double)g_tune.small_samples;
    else if (!strcmp(key, "chunk_samples")) v = (double)g_tune.chunk_samples;
    else if (!strcmp(key, "de_halo")) v = g_tune.de_halo;
    else return 1;
@@ -419,13 +431,14 @@ extern "C" int fl3h_tuning(const char *key, const double *value, double *out) {
 // Walkers of a launch: enough for every CU to hold its tuned number of
 // workgroups (fewer when the launch runs in several chunks, whose sort
 // passes share the CUs with the next chunk's walkers), at least min_samples
-// samples each, a multiple of 256.
+// samples each (min_samples_small for launches below small_samples), a
+// multiple of 256.
 static long long choose_walkers(long long nsamples, int requested) {
    if (requested > 0) return requested;
    tune_init();
    const int wg = nsamples > g_tune.chunk_samples ? g_tune.wg_per_cu : g_tune.wg_per_cu_single;
    long long maxw = (long long)g_ctx.ncu * wg * 256;
-   long long w = nsamples / g_tune.min_samples;
+   long long w = nsamples / (nsamples < g_tune.small_samples ? g_tune.min_samples_small : g_tune.min_samples);
    if (w > maxw) w = maxw;
    if (w < 256) w = 256;
    return (w + 255) / 256 * 256;
@@ -539,6 +552,7 @@ struct JobSet {
    const unsigned char *dev_all = nullptr; // n > 1: concatenated jobs
    const long long *dev_off = nullptr;     // n > 1: byte offsets
    const unsigned long long *dev_keys = nullptr;
+   const long long *dev_ns = nullptr;      // n > 1 with counts of their own: samples per job
    size_t lds = 0;                         // largest genome LDS image
    size_t lds_rec = 0;                     // the same without the palette (record modes)
    int tier = 0, nxf_max = 0;
@@ -845,6 +859,7 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
       const long long c = p->nsamples - done < Wj * per ? p->nsamples - done : Wj * per;
       a.nsamples = c;
       a.rows_out = (c + Wj - 1) / Wj;
+      a.ns_row0 = done / Wj;   // (done: whole rows of every walker)
       a.resume = done > 0;
       a.rec = (unsigned long long *)g_ctx.crec[buf].p;
       a.blk_hist = (unsigned *)g_ctx.cblk[buf].p;
@@ -919,17 +934,31 @@ extern "C" int fl3d_iterate_batch(const void *const *jobs, const size_t *job_byt
          off[j] = (long long)total;
          total += (job_bytes[j] + 255) & ~(size_t)255;
       }
-      const size_t tab = sizeof(long long) * njobs + sizeof(uint64_t) * 2 * njobs;
+      // after the jobs: byte offsets, walker keys and (if given) sample counts
+      const size_t tab = sizeof(long long) * njobs + sizeof(uint64_t) * 2 * njobs +
+                         (p->job_nsamples ? sizeof(long long) * njobs : 0);
       std::vector<unsigned char> host(total + tab);
       for (int j = 0; j < njobs; j++) memcpy(host.data() + off[j], jobs[j], job_bytes[j]);
       memcpy(host.data() + total, off.data(), sizeof(long long) * njobs);
       memcpy(host.data() + total + sizeof(long long) * njobs, keys, sizeof(uint64_t) * 2 * njobs);
+      if (p->job_nsamples)
+         for (int j = 0; j < njobs; j++) {
+            if (p->job_nsamples[j] < 0 || p->job_nsamples[j] > p->nsamples) {
+               set_err("fl3d_iterate_batch: a job's sample count exceeds the launch's");
+               return 1;
+            }
+            const long long v = p->job_nsamples[j];
+            memcpy(host.data() + total + (sizeof(long long) + 2 * sizeof(uint64_t)) * njobs + sizeof(long long) * j,
+                   &v, sizeof(long long));
+         }
       if (g_ctx.job.ensure(total + tab)) return 1;
       HIPCHK(hipMemcpyAsync(g_ctx.job.p, host.data(), total + tab, hipMemcpyHostToDevice, g_ctx.stream));
       HIPCHK(hipStreamSynchronize(g_ctx.stream));  // host is a local buffer
       js.dev_all = (const unsigned char *)g_ctx.job.p;
       js.dev_off = (const long long *)(js.dev_all + total);
       js.dev_keys = (const unsigned long long *)(js.dev_all + total + sizeof(long long) * njobs);
+      if (p->job_nsamples)
+         js.dev_ns = (const long long *)(js.dev_all + total + (sizeof(long long) + 2 * sizeof(uint64_t)) * njobs);
    }
    HIPCHK(hipMemsetAsync(g_ctx.counters, 0, 16, g_ctx.stream));
    if (p->nsamples > 0) {
@@ -940,6 +969,7 @@ extern "C" int fl3d_iterate_batch(const void *const *jobs, const size_t *job_byt
       a.job_off = js.dev_off;
       a.keys = js.dev_keys;
       a.njobs = njobs;
+      a.job_ns = js.dev_ns;
       a.buckets = cur_buckets();
       a.nsamples = p->nsamples;
       a.fuse = p->fuse;
@@ -964,7 +994,10 @@ extern "C" int fl3d_iterate_batch(const void *const *jobs, const size_t *job_byt
       }
       const bool binned =
           binnable && (p->deposit == 2 || (p->deposit == 0 && p->nsamples * njobs >= FL3_BIN_MIN_SAMPLES));
-      g_prof.samples += p->nsamples * njobs;
+      if (p->job_nsamples && njobs > 1)
+         for (int j = 0; j < njobs; j++) g_prof.samples += p->job_nsamples[j];
+      else
+         g_prof.samples += p->nsamples * njobs;
       if (binned) {
          if (iterate_binned(a, W, Wj, js, p, shift, band)) return 1;
       } else {

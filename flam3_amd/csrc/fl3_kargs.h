@@ -30,6 +30,10 @@ struct IterArgs {
    const long long *job_off;
    const unsigned long long *keys;
    int njobs;
+   // several jobs with their own sample counts (at most nsamples' share):
+   // job j's plotted samples in all, and the rows of earlier chunks
+   const long long *job_ns;
+   long long ns_row0;
    void *buckets;
    long long nsamples;      // fast: total plotted samples in the launch
    int fuse;

@@ -41,19 +41,29 @@
  * 24-bit cell index. */
 #define FL3_SHARED_NARROW_CELLS ((1LL << 24) - 2)
 
-/* One launch over the *n temporal samples collected so far (nsamples each,
- * keys[2t], keys[2t+1] for the t-th); frees their jobs and empties the set. */
-static int flush_batch(void **jobs, size_t *bytes, int *n, int64_t nsamples, const uint64_t *keys, int exact,
+/* One launch over the *n temporal samples collected so far (nsamples[t]
+ * plotted samples and keys[2t], keys[2t+1] for the t-th); frees their jobs
+ * and empties the set.  Interpolated sample densities differ in their last
+ * bits between temporal samples (rect.c:797, 830-833), so the counts may
+ * differ by a few samples: one launch takes them all, its walkers sized by
+ * the largest. */
+static int flush_batch(void **jobs, size_t *bytes, int *n, const int64_t *nsamples, const uint64_t *keys, int exact,
                        const fl3h_render_opts *opts, double *bad, int *sat) {
    fl3d_iter_params ip;
-   int rc, t;
+   int rc, t, same = 1;
+   int64_t nmax = nsamples[0];
+   for (t = 1; t < *n; t++) {
+      if (nsamples[t] != nsamples[0]) same = 0;
+      if (nsamples[t] > nmax) nmax = nsamples[t];
+   }
    memset(&ip, 0, sizeof(ip));
    /* The reference fuses 15 steps (100 with earlyclip) before each
     * 10000-sample sub-batch; a production walker fuses once before a few
     * hundred to a few thousand samples, so it fuses 100 steps to keep the
     * start transient out of the image. */
    ip.fuse = FUSE_WALKER;
-   ip.nsamples = nsamples;
+   ip.nsamples = nmax;
+   ip.job_nsamples = same ? NULL : nsamples;
    ip.key[0] = keys[0];
    ip.key[1] = keys[1];
    ip.exact = exact;
@@ -511,8 +521,8 @@ int fl3h_render(flam3_frame *spec, void *out, int field, int nchan, int transp, 
    int batch_num, ts, i, rv = 1, aborted = 0, nts;
    void **bjobs = NULL;   /* temporal samples waiting for their launch */
    size_t *bbytes = NULL;
+   int64_t *bns = NULL;   /* and their sample counts */
    int bn = 0, bts0 = 0;
-   int64_t bns = 0;
 
    if (plan_open(&P, spec, out, field, nchan, transp, stats, opts)) return 1;
    nts = P.nts;
@@ -522,7 +532,8 @@ int fl3h_render(flam3_frame *spec, void *out, int field, int nchan, int transp, 
    }
    bjobs = (void **)calloc((size_t)(nts < MAX_BATCH_JOBS ? nts : MAX_BATCH_JOBS), sizeof(void *));
    bbytes = (size_t *)calloc((size_t)(nts < MAX_BATCH_JOBS ? nts : MAX_BATCH_JOBS), sizeof(size_t));
-   if (!bjobs || !bbytes) goto fail;
+   bns = (int64_t *)calloc((size_t)(nts < MAX_BATCH_JOBS ? nts : MAX_BATCH_JOBS), sizeof(int64_t));
+   if (!bjobs || !bbytes || !bns) goto fail;
 
    for (batch_num = 0; batch_num < P.nbatches && !aborted; batch_num++) {
       fl3_de_filters de;
@@ -576,19 +587,17 @@ int fl3h_render(flam3_frame *spec, void *out, int field, int nchan, int transp, 
                P.vib_gam_n++;
                plan_keys(&P, ts, batch_num * nts + ts);
                nsamp = plan_nsamples(&P, r->batch_size);
-               if (bn > 0 && (nsamp != bns || bn == max_batch_jobs(opts))) {
+               if (bn == max_batch_jobs(opts)) {
                   if (flush_batch(bjobs, bbytes, &bn, bns, P.keys + 2 * bts0, exact, opts, &bad, &sat)) {
                      failed = 2;
                      break;
                   }
                   if (sat) saturated = 1;
                }
-               if (bn == 0) {
-                  bts0 = ts;
-                  bns = nsamp;
-               }
+               if (bn == 0) bts0 = ts;
                bjobs[bn] = r->job;
                bbytes[bn] = r->bytes;
+               bns[bn] = nsamp;
                bn++;
                res[ts].job = NULL;
                P.badvals += bad;
@@ -632,20 +641,18 @@ int fl3h_render(flam3_frame *spec, void *out, int field, int nchan, int transp, 
             } else {
                int64_t nsamp;
                if (pass == 0) plan_keys(&P, ts, tidx);
-               /* Temporal samples with the same sample count go to the GPU
-                * together, one launch for the whole run of them. */
+               /* Temporal samples go to the GPU together, one launch for up
+                * to max_batch_jobs of them. */
                nsamp = plan_nsamples(&P, batch_size);
-               if (bn > 0 && (nsamp != bns || bn == max_batch_jobs(opts))) {
+               if (bn == max_batch_jobs(opts)) {
                   if (flush_batch(bjobs, bbytes, &bn, bns, P.keys + 2 * bts0, exact, opts, &bad, &sat))
                      goto devfail_de;
                   if (sat) saturated = 1;
                }
-               if (bn == 0) {
-                  bts0 = ts;
-                  bns = nsamp;
-               }
+               if (bn == 0) bts0 = ts;
                bjobs[bn] = job;
                bbytes[bn] = job_bytes;
+               bns[bn] = nsamp;
                bn++;
                job = NULL;
             }
@@ -702,6 +709,7 @@ cleanup:
    for (i = 0; i < bn; i++) free(bjobs[i]);
    free(bjobs);
    free(bbytes);
+   free(bns);
    plan_close(&P);
    return rv;
 }
@@ -847,15 +855,19 @@ void fl3_many_free(fl3_many *B) {
    free(B);
 }
 
-/* Walkers per run of consecutive temporal samples with equal sample counts,
- * as fl3h_render launches them. */
+/* Walkers per run of up to max_batch_jobs consecutive temporal samples, as
+ * fl3h_render launches them (sized by the run's largest sample count). */
 static int assign_walkers(fl3d_multi_job *m, int n, const fl3h_render_opts *opts) {
    int ts = 0, i;
    while (ts < n) {
       int e = ts;
       long long w;
-      while (e < n && e - ts < max_batch_jobs(opts) && m[e].nsamples == m[ts].nsamples) e++;
-      w = fl3d_job_walkers(m[ts].nsamples, e - ts, opts ? opts->nwalkers : 0);
+      int64_t nmax = 0;
+      while (e < n && e - ts < max_batch_jobs(opts)) {
+         if (m[e].nsamples > nmax) nmax = m[e].nsamples;
+         e++;
+      }
+      w = fl3d_job_walkers(nmax, e - ts, opts ? opts->nwalkers : 0);
       if (w < 256) return 1;
       for (i = ts; i < e; i++) {
          m[i].walkers = (int)w;
@@ -1170,8 +1182,9 @@ static int many_small(fl3_many *B) {
    return rc;
 }
 
-/* A large prepared render: fl3h_render's launches (runs of temporal samples
- * with equal sample counts), density estimation and tone map. */
+/* A large prepared render: fl3h_render's launches (runs of up to
+ * max_batch_jobs temporal samples, each with its own sample count), density
+ * estimation and tone map. */
 static int many_large(fl3_many *B, int r) {
    render_plan *Q = &B->P[r];
    const fl3h_render_opts *opts = B->has_opts ? &B->opts : NULL;
@@ -1189,18 +1202,24 @@ static int many_large(fl3_many *B, int r) {
          const void *jobs[MAX_BATCH_JOBS];
          size_t bytes[MAX_BATCH_JOBS];
          uint64_t keys[2 * MAX_BATCH_JOBS];
+         int64_t ns[MAX_BATCH_JOBS], nmax = 0;
+         int same = 1;
          fl3d_iter_params ip;
          double bad = 0.0;
          int sat = 0;
-         for (e = ts; e < B->njob[r] && e - ts < max_batch_jobs(opts) && m[e].nsamples == m[ts].nsamples; e++) {
+         for (e = ts; e < B->njob[r] && e - ts < max_batch_jobs(opts); e++) {
             jobs[e - ts] = m[e].job;
             bytes[e - ts] = m[e].bytes;
             keys[2 * (e - ts)] = m[e].key[0];
             keys[2 * (e - ts) + 1] = m[e].key[1];
+            ns[e - ts] = m[e].nsamples;
+            if (m[e].nsamples > nmax) nmax = m[e].nsamples;
+            if (m[e].nsamples != m[ts].nsamples) same = 0;
          }
          memset(&ip, 0, sizeof(ip));
          ip.fuse = FUSE_WALKER;
-         ip.nsamples = m[ts].nsamples;
+         ip.nsamples = nmax;
+         ip.job_nsamples = same ? NULL : ns;
          ip.key[0] = keys[0];
          ip.key[1] = keys[1];
          ip.exact = exact;

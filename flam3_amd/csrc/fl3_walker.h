@@ -59,7 +59,11 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
    const long long Wj = dsc ? (long long)dsc->nblocks * blockDim.x : (long long)(gridDim.x / nj) * blockDim.x;
    const long long wj = dsc ? (long long)(blockIdx.x - dsc->block0) * blockDim.x + threadIdx.x
                             : (long long)(blockIdx.x / nj) * blockDim.x + threadIdx.x;
-   const long long nsj = dsc ? dsc->nsamples : a.nsamples;
+   long long nsj = dsc ? dsc->nsamples : a.nsamples;
+   if (!dsc && a.job_ns) {   // this chunk's share of the job's own count
+      const long long rem = a.job_ns[jb] - a.ns_row0 * Wj;
+      nsj = rem < 0 ? 0 : (rem < nsj ? rem : nsj);
+   }
    long long n = nsj / Wj + (wj < nsj % Wj ? 1 : 0);
    // independent renders with records run in chunks of rows [row0, row0 + rows_out)
    if (dsc && RECORDS) n = n - a.row0 < 0 ? 0 : (n - a.row0 < a.rows_out ? n - a.row0 : a.rows_out);
@@ -192,7 +196,10 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
             }
          }
          p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
-         if (use_final) {
+         // the final xform's point is only plotted, never iterated on: the
+         // fuse steps skip it (the reference applies it there and drops the
+         // result, flam3.c:280-291)
+         if (use_final && i >= 0) {
             if (G.final_opacity == 1.0f || u01<float>(rng) < G.final_opacity) {
                apply_xform<float, WalkerRng, TIER, true>(G, G.final_index, p, q, rng);
                q[3] = p[3];
@@ -238,7 +245,7 @@ __global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
             consec = 0;
          lastxf = fn + 1;
          p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
-         if (use_final) {
+         if (use_final && i >= 0) {   // (not in the fuse steps, as above)
             if (G.final_opacity == 1.0f || u01<float>(rng) < G.final_opacity) {
                apply_xform<float, WalkerRng, TIER, true>(G, G.final_index, p, q, rng);
                q[3] = p[3];

@@ -271,7 +271,9 @@ void fl3h_genome_clear(flam3_genome *g);
  * Keys: wg_per_cu (walker workgroups of 256 per CU for launches of several
  * chunks, default 8), wg_per_cu_single (the same for launches of one chunk,
  * 8), min_samples
- * (samples per walker at least, 256), chunk_samples (binned deposit chunk,
+ * (samples per walker at least, 256), min_samples_small (the same for
+ * launches of fewer than small_samples samples, 128; small_samples 2^23),
+ * chunk_samples (binned deposit chunk,
  * 2^28), de_halo (density estimation: each gather region scans the halo its
  * neighbourhood's kernels reach 1, the widest kernel's halo everywhere 0).  With value non-NULL the key is set for the rest of the process; out
  * (may be NULL) receives the current value.  Returns 0, or 1 for an unknown
