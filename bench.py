@@ -23,6 +23,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
 import argparse
 import ctypes
+import gc
 import json
 import os
 import sys
@@ -359,6 +360,10 @@ def preview_bench(args):
             inputs = [build(k) for k in range(n)]
             srv = F.Server()
             ids = []
+            # (the load generator must not stall: a garbage-collection pause
+            # would submit the requests due meanwhile in one burst)
+            gc.collect()
+            gc.disable()
             t0 = time.perf_counter()
             for k in range(n):
                 d = t0 + arrive[k] - time.perf_counter()
@@ -374,6 +379,7 @@ def preview_bench(args):
                     raise RuntimeError("preview request failed: " + F.last_error())
                 lat.append(lv.value)
             wall = time.perf_counter() - t0
+            gc.enable()
             batches, _ = srv.counts()
             srv.close()
             lat = np.array(lat)

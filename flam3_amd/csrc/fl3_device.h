@@ -65,6 +65,9 @@ typedef struct {
                        one fl3d_iterate_batch launch of that render alone would */
 } fl3d_multi_job;
 long long fl3d_job_walkers(int64_t nsamples, int njobs, int requested);
+/* Size the histogram, accumulator and density-estimation buffers for renders
+ * of hist_bytes histogram and accumulator bytes in all (no later growth). */
+int fl3d_reserve(double hist_bytes);
 int fl3d_iterate_multi(const fl3d_multi_job *jobs, int njobs, int fuse, int exact, int nrenders,
                        double *badvals, int *saturated);
 /* The same for njobs jobs in one launch (the temporal samples of a batch):

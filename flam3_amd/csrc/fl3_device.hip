@@ -64,12 +64,17 @@ struct DevBuf {
    size_t cap = 0;
    int ensure(size_t bytes) {
       if (bytes <= cap) return 0;
+      const size_t cap0 = cap;
       if (p) (void)hipFree(p);
       p = nullptr;
       cap = 0;
+      // (a buffer that has to grow again at least doubles, up to 2 GiB: each
+      // growth frees and allocates, which stalls the device for milliseconds)
       size_t want = bytes + bytes / 8;
+      if (cap0 && cap0 < ((size_t)1 << 31) && want < 2 * cap0) want = 2 * cap0;
       if (hipMalloc(&p, want) != hipSuccess) {
          snprintf(g_err, sizeof(g_err), "hipMalloc of %zu bytes failed", want);
+         (void)hipGetLastError();   // (reported here; later launches must not see it)
          p = nullptr;
          return 1;
       }
@@ -309,6 +314,28 @@ extern "C" int fl3d_begin_many(const int *width, const int *height, int n, int b
    HIPCHK(hipMemsetAsync(g_ctx.accum.p, 0, cells * accum_bytes(bits), g_ctx.stream));
    HIPCHK(hipMemsetAsync(g_ctx.buckets.p, 0, cells * bucket_bytes(bits), g_ctx.stream));
    return fl3d_select(0);
+}
+
+// Device buffers sized up front for renders whose histograms and
+// accumulators take hist_bytes in all (a server's admission budget), so
+// that no batch within it stalls on growing them: histogram, accumulator
+// and the density estimation's per-cell buffers.
+extern "C" int fl3d_reserve(double hist_bytes) {
+   std::lock_guard<std::recursive_mutex> lk(g_mu);
+   if (ctx_init()) return 1;
+   if (!(hist_bytes > 0)) return 0;
+   // per cell 20 + 16 B (bits 32/33) or 40 + 32 B (bits 64): the buckets
+   // take 5/9 of either, the accumulator 4/9; at most hist_bytes / 36 cells.
+   // A batch's groups share the buffers one after another, each of at most
+   // the shared launch's 32e6 cells (a larger lone render grows them).
+   if (hist_bytes > 32e6 * 36.0) hist_bytes = 32e6 * 36.0;
+   const size_t cells = (size_t)(hist_bytes / 36.0) + 1;
+   if (g_ctx.buckets.ensure((size_t)(hist_bytes * 5.0 / 9.0) + 16) ||
+       g_ctx.accum.ensure((size_t)(hist_bytes * 4.0 / 9.0) + 16) || g_ctx.findex.ensure(sizeof(uint16_t) * cells) ||
+       g_ctx.srcv.ensure(16 * cells) || g_ctx.srcl.ensure(sizeof(double) * cells) ||
+       g_ctx.deblk.ensure(sizeof(int) * (cells / (DG_REGION_HOST * DG_REGION_HOST) + 4096)))
+      return 1;
+   return 0;
 }
 
 extern "C" int fl3d_select(int r) {
@@ -1739,11 +1766,13 @@ extern "C" int fl3d_images_fetch(const fl3d_image_dst *d, int n) {
    // one copy into pinned staging, then rows into the callers' buffers
    // (a strided copy straight to pageable memory goes row by row)
    if (g_ctx.staging_cap < end) {
+      // (grown at least twofold: pinned allocations are slow)
+      const size_t want = end > 2 * g_ctx.staging_cap ? end : 2 * g_ctx.staging_cap;
       if (g_ctx.staging) (void)hipHostFree(g_ctx.staging);
       g_ctx.staging = nullptr;
       g_ctx.staging_cap = 0;
-      HIPCHK(hipHostMalloc(&g_ctx.staging, end, hipHostMallocDefault));
-      g_ctx.staging_cap = end;
+      HIPCHK(hipHostMalloc(&g_ctx.staging, want, hipHostMallocDefault));
+      g_ctx.staging_cap = want;
    }
    HIPCHK(hipMemcpyAsync(g_ctx.staging, g_ctx.image.p, end, hipMemcpyDeviceToHost, g_ctx.stream));
    HIPCHK(hipStreamSynchronize(g_ctx.stream));
@@ -1771,11 +1800,14 @@ extern "C" int fl3d_images_download(int slot, size_t bytes, const unsigned char 
       return 1;
    }
    if (g_ctx.slot_cap[slot] < bytes) {
+      // (grown at least twofold, within the image arena: pinned allocations are slow)
+      size_t want = bytes > 2 * g_ctx.slot_cap[slot] ? bytes : 2 * g_ctx.slot_cap[slot];
+      if (want > g_ctx.image.cap) want = g_ctx.image.cap;
       if (g_ctx.slot[slot]) (void)hipHostFree(g_ctx.slot[slot]);
       g_ctx.slot[slot] = nullptr;
       g_ctx.slot_cap[slot] = 0;
-      HIPCHK(hipHostMalloc(&g_ctx.slot[slot], bytes, hipHostMallocDefault));
-      g_ctx.slot_cap[slot] = bytes;
+      HIPCHK(hipHostMalloc(&g_ctx.slot[slot], want, hipHostMallocDefault));
+      g_ctx.slot_cap[slot] = want;
    }
    if (bytes) HIPCHK(hipMemcpyAsync(g_ctx.slot[slot], g_ctx.image.p, bytes, hipMemcpyDeviceToHost, g_ctx.stream));
    HIPCHK(hipStreamSynchronize(g_ctx.stream));

@@ -346,6 +346,15 @@ fl3h_server *fl3h_server_create(const fl3h_server_opts *o) {
    pthread_cond_init(&s->cv_fin, NULL);
    pthread_cond_init(&s->cv_slot, NULL);
    if (fl3d_device_memory(&s->dev_avail, NULL)) s->dev_avail = 0.0;
+   {
+      /* the device buffers of a full batch, allocated now: growing them
+       * between batches stalls the device for milliseconds (an unlucky
+       * batch's requests then wait tens of ms); if it fails, they grow as
+       * needed */
+      double budget = s->o.hist_budget_bytes > 0 ? s->o.hist_budget_bytes : 32e6 * 36.0;
+      if (s->dev_avail > 0 && budget > SRV_MEM_FRACTION * s->dev_avail) budget = SRV_MEM_FRACTION * s->dev_avail;
+      (void)fl3d_reserve(budget);
+   }
    if (pthread_create(&s->finisher, NULL, finisher_main, s)) goto bad;
    s->threads = 1;
    if (pthread_create(&s->planner, NULL, planner_main, s)) {
