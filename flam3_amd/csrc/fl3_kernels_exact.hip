@@ -375,28 +375,43 @@ __device__ __forceinline__ void de_gather_body(const DEArgs &a, int bx, int by) 
       }
       for (int w = 0; w < DG_CHUNK / 64; w++) nb += list_cnt[wave][w];
       __syncthreads();
-      // (the coefficient is loaded for every listed source, from index 0
-      // when the tap misses this cell, so that the loads of consecutive
-      // sources overlap; a missed tap adds nothing, as before)
+      if constexpr (EXACT) {
+         // (the coefficient is loaded for every listed source, from index 0
+         // when the tap misses this cell, so that the loads of consecutive
+         // sources overlap; a missed tap adds nothing)
 #pragma unroll 4
-      for (unsigned i = 0; i < nb; i++) {
-         const unsigned k = s_list[wave][i];
-         const int4 src = s_src[k];
-         const int dx = abs(x - src.x), dy = abs(y - src.y);
-         const int hi = max(dx, dy), lo = min(dx, dy);
-         const bool tap = live && hi <= src.w;
-         const long ci = tap ? (long)src.z + hi * (hi + 1) / 2 + lo : 0;
-         if constexpr (EXACT) {
+         for (unsigned i = 0; i < nb; i++) {
+            const unsigned k = s_list[wave][i];
+            const int4 src = s_src[k];
+            const int dx = abs(x - src.x), dy = abs(y - src.y);
+            const int hi = max(dx, dy), lo = min(dx, dy);
+            const bool tap = live && hi <= src.w;
+            const long ci = tap ? (long)src.z + hi * (hi + 1) / 2 + lo : 0;
             const double coef = a.coefs[ci];
             if (tap && coef != 0.0) {
                const double c3 = s_val[k][3];
                const double ls = coef * s_val[k][4] / c3;
                for (int c = 0; c < 4; c++) acc[c] = (AT)((double)acc[c] + s_val[k][c] * ls);
             }
-         } else {
-            const AT coef = ((const AT *)a.coefs_t)[ci];
-            if (tap && coef != (AT)0)
-               for (int c = 0; c < 4; c++) acc[c] += coef * s_val[k][c];
+         }
+      } else {
+         // Straight-line taps, no branches, so that the coefficient loads of
+         // consecutive sources are in flight together: a tap that misses
+         // this cell loads index 0 and adds it times 0 (the per-source
+         // values are finite), in 32-bit index arithmetic, fused
+         // multiply-adds.
+         const AT *ct = (const AT *)a.coefs_t;
+#pragma unroll 4
+         for (unsigned i = 0; i < nb; i++) {
+            const unsigned k = s_list[wave][i];
+            const int4 src = s_src[k];
+            const int dx = abs(x - src.x), dy = abs(y - src.y);
+            const int hi = max(dx, dy), lo = min(dx, dy);
+            const bool tap = live && hi <= src.w;
+            const unsigned ci = tap ? (unsigned)src.z + ((unsigned)(hi * (hi + 1)) >> 1) + (unsigned)lo : 0u;
+            const AT cv = ct[ci];
+            const AT coef = tap ? cv : (AT)0;
+            for (int c = 0; c < 4; c++) acc[c] = fma(coef, s_val[k][c], acc[c]);
          }
       }
       __syncthreads();
