@@ -27,8 +27,15 @@
 #define FL3_PERM_EVERY 1u
 #endif
 
+// Waves per SIMD the walker kernel is compiled for: the classic tier (the
+// variations of most genomes) at 8, which holds it to 64 VGPRs (a few
+// spills; 7 waves at its natural 67-78 VGPRs ran 11% slower per launch),
+// the larger tiers at their natural register counts.
+template <int TIER>
+constexpr int walker_waves() { return TIER == 0 ? 8 : 1; }
+
 template <int BITS, bool EXACT, int RECORDS, int TIER, bool WAVE>
-__global__ __launch_bounds__(256) void k_iter_fast(IterArgs a) {
+__global__ __launch_bounds__(256, walker_waves<TIER>()) void k_iter_fast(IterArgs a) {
    unsigned char *lds = fl3_lds;
    GenL<float> G;
    // independent renders: direct deposits, or job or 4-byte records in one
