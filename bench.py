@@ -290,7 +290,8 @@ def config5_bench(args):
                 "bin_ms": st.bin_ms, "bin_chunks": st.bin_launches,
                 "density_ms": st.density_ms, "tonemap_ms": st.tonemap_ms, "cells": st.cells,
                 "deposit": "tile-binned: 8-byte records grouped by band (2^24 cells), then per band counted, "
-                           "sorted by tile and summed in double LDS tiles",
+                           "sorted by tile and summed in LDS tiles as 64-bit fixed point (2^-32), "
+                           "then added to the double buckets",
                 "accumulation_hbm_frac": stats.num_iters * 80.0 / (st.bin_ms * 1e-3) / (HBM_PEAK_GBS * 1e9)
                 if st.bin_ms > 0 else None}
     finally:

@@ -684,14 +684,18 @@ static int narrow_round(const fl3_job_hdr *const *h, int n) {
 
 // Largest records per accumulation work item: a cell's sums must fit 32
 // bits (and the packed pairs must not carry): at most 2^18 records, fewer
-// if a palette entry exceeds 255.
-static int item_records(const fl3_job_hdr *const *h, int n) {
+// if a palette entry exceeds 255.  Double buckets (fixed64) sum multiples of
+// 2^-32 in signed 64-bit integers: records x largest |entry| < 2^31.
+static int item_records(const fl3_job_hdr *const *h, int n, int fixed64 = 0) {
    double maxd = 255.0;
    for (int j = 0; j < n; j++) {
       const double *dm = (const double *)((const unsigned char *)h[j] + h[j]->off_dmap);
-      for (int e = 0; e < 1024; e++) maxd = dm[e] > maxd ? dm[e] : maxd;
+      for (int e = 0; e < 1024; e++) {
+         const double v = fixed64 ? fabs(dm[e]) : dm[e];
+         maxd = v > maxd ? v : maxd;
+      }
    }
-   const double lim = 4294967295.0 / (maxd + 1.0);
+   const double lim = (fixed64 ? 2147483647.0 : 4294967295.0) / (maxd + 1.0);
    const int c = lim < (double)FL3_BIN_ITEM_RECORDS ? (int)lim : FL3_BIN_ITEM_RECORDS;
    return c < 1 ? 1 : c;
 }
@@ -858,7 +862,7 @@ static int iterate_binned(IterArgs a, long long W, long long Wj, const JobSet &j
    b.nbins = nbins;
    b.width = g_ctx.width;
    b.height = g_ctx.height;
-   b.chunk = item_records(js.h, js.n);
+   b.chunk = item_records(js.h, js.n, format >= FL3_BIN_F64_WIDE);
    const uint32_t *pal8 = nullptr;
    if (format == 2 && upload_job_tables(js.h, js.n, js.nxf_max, &b, &pal8)) return 1;
    a.rec_mode = narrow ? 2 : (format == 2 ? 3 : 1);

@@ -619,34 +619,45 @@ __global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
    }
 }
 
-// Accumulation into bits-64 (double) buckets: the tile's five channels live
-// in LDS as doubles and each record adds what the direct deposit adds for
-// its point (rect.c:492-497 in double: the palette entry, blended for a
-// linear palette, times the xform's visibility, and 255 times it to the
-// count), so a cell's sum differs from the per-sample atomics' only in the
-// order of its additions.  4-byte records (stepped palette, visibility 1)
-// or 8-byte records (make_record).  Persistent workgroups take work items
-// as in k_bin_accum; a tile with one item is added without atomics.
+// Accumulation into bits-64 (double) buckets.  Each record adds what the
+// direct deposit adds for its point (rect.c:492-497 in double: the palette
+// entry, blended for a linear palette, times the xform's visibility, and
+// 255 times it to the count), rounded to a multiple of 2^-32 and summed in
+// LDS as signed 64-bit integers (integer atomics outpace double ones; a
+// work item of at most 2^31 / (largest |entry| + 1) records keeps a cell's
+// sum below 2^63 in size); the item's sums are then added to the double
+// buckets.  Against the
+// per-sample double atomics a cell's sum differs by at most 2^-33 per
+// record (about 1e-12 relative), besides the order of the additions.
+// 4-byte records (stepped palette, visibility 1) or 8-byte records
+// (make_record).  Persistent workgroups take work items as in k_bin_accum;
+// a tile with one item is added without atomics.
+#define FL3_F64_FIX 4294967296.0    // 2^32: the fixed-point scale of the LDS sums
 template <typename RT>
 __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_f64(BinArgs a) {
    const fl3_job_hdr *h = (const fl3_job_hdr *)a.job;
    const int tile_cells = 1 << a.shift;
-   double *acc = (double *)fl3_lds;               // [5][tile_cells]
-   double *dm = acc + 5 * tile_cells;             // [1024]
-   double *vis = dm + 1024;                       // [256]
+   unsigned long long *acc = (unsigned long long *)fl3_lds;   // [5][tile_cells]
+   unsigned long long *dfx = acc + 5 * tile_cells;            // [1024] the palette, fixed point
+   double *dm = (double *)(dfx + 1024);                       // [1024]
+   double *vis = dm + 1024;                                   // [256]
    __shared__ int item_sh;
    const double *pd = (const double *)(a.job + h->off_dmap);
    const fl3_xform_pack *px = (const fl3_xform_pack *)(a.job + h->off_xf);
-   for (int e = threadIdx.x; e < 1024; e += blockDim.x) dm[e] = pd[e];
+   for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
+      dm[e] = pd[e];
+      dfx[e] = (unsigned long long)__double2ll_rn(pd[e] * FL3_F64_FIX);
+   }
    // (the walkers carry a visibility as a float)
    for (int e = threadIdx.x; e < 256; e += blockDim.x) vis[e] = e < h->nxf ? (double)(float)px[e].vis : 1.0;
    const int linear = h->palette_mode == 1;
+   const unsigned long long hits = (unsigned long long)__double2ll_rn(255.0 * FL3_F64_FIX);
    const long long total_items = a.item_start[a.nbins];
    const RT *sorted = (const RT *)a.final;
    for (;;) {
       __syncthreads();
       if (threadIdx.x == 0) item_sh = (int)atomicAdd(a.item_counter, 1u);
-      for (int e = threadIdx.x; e < 5 * tile_cells; e += blockDim.x) acc[e] = 0.0;
+      for (int e = threadIdx.x; e < 5 * tile_cells; e += blockDim.x) acc[e] = 0ull;
       __syncthreads();
       const long long item = item_sh;
       if (item >= total_items) break;
@@ -671,29 +682,33 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_f64(BinArgs a) {
          }
          for (int u = 0; u < FL3_ACC_UNROLL; u++) {
             if (rec_none(v[u])) continue;
-            double c[4], w = 1.0;
             const unsigned local = rel_cell(a, v[u]) - tb;
+            unsigned long long d[5];
             if (sizeof(RT) == 4) {
-               const double *e = dm + ((unsigned)v[u] >> 24) * 4;
-               for (int ch = 0; ch < 4; ch++) c[ch] = e[ch];
+               const unsigned long long *e = dfx + ((unsigned)v[u] >> 24) * 4;
+               for (int ch = 0; ch < 4; ch++) d[ch] = e[ch];
+               d[4] = hits;
             } else {
                const unsigned long long x = (unsigned long long)v[u];
                const int ci = (int)((x >> 32) & 0xff), fn = (int)((x >> 40) & 0xff);
-               const double *e = dm + ci * 4;
-               if (linear) {
-                  const double f = (double)((float)(x >> 48) * (1.0f / 65535.0f));
-                  for (int ch = 0; ch < 4; ch++) c[ch] = e[ch] * (1.0 - f) + e[4 + ch] * f;
-               } else
-                  for (int ch = 0; ch < 4; ch++) c[ch] = e[ch];
-               w = vis[fn];
+               const double w = vis[fn];
+               if (!linear && w == 1.0) {
+                  for (int ch = 0; ch < 4; ch++) d[ch] = dfx[ci * 4 + ch];
+                  d[4] = hits;
+               } else {
+                  const double *e = dm + ci * 4;
+                  double c[4];
+                  if (linear) {
+                     const double f = (double)((float)(x >> 48) * (1.0f / 65535.0f));
+                     for (int ch = 0; ch < 4; ch++) c[ch] = e[ch] * (1.0 - f) + e[4 + ch] * f;
+                  } else
+                     for (int ch = 0; ch < 4; ch++) c[ch] = e[ch];
+                  for (int ch = 0; ch < 4; ch++)
+                     d[ch] = (unsigned long long)__double2ll_rn((w == 1.0 ? c[ch] : w * c[ch]) * FL3_F64_FIX);
+                  d[4] = w == 1.0 ? hits : (unsigned long long)__double2ll_rn(w * 255.0 * FL3_F64_FIX);
+               }
             }
-            if (w == 1.0) {
-               for (int ch = 0; ch < 4; ch++) atomicAdd(&acc[ch * tile_cells + local], c[ch]);
-               atomicAdd(&acc[4 * tile_cells + local], 255.0);
-            } else {
-               for (int ch = 0; ch < 4; ch++) atomicAdd(&acc[ch * tile_cells + local], w * c[ch]);
-               atomicAdd(&acc[4 * tile_cells + local], w * 255.0);
-            }
+            for (int ch = 0; ch < 5; ch++) atomicAdd(&acc[ch * tile_cells + local], d[ch]);
          }
       }
       __syncthreads();
@@ -706,7 +721,7 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_f64(BinArgs a) {
       double val[K];
       for (int k = 0; k < K; k++) {
          const int e = threadIdx.x + k * FL3_ACC_THREADS;
-         val[k] = e < nvalid * 5 ? acc[(e % 5) * tile_cells + e / 5] : 0.0;
+         val[k] = e < nvalid * 5 ? (double)(long long)acc[(e % 5) * tile_cells + e / 5] * (1.0 / FL3_F64_FIX) : 0.0;
       }
       if (single) {
          double old[K];
@@ -720,7 +735,7 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_f64(BinArgs a) {
    }
 }
 
-extern "C" size_t fl3k_bin_f64_lds_bytes(int shift) { return (size_t)40 * ((size_t)1 << shift) + 8 * 1280 + 64; }
+extern "C" size_t fl3k_bin_f64_lds_bytes(int shift) { return (size_t)40 * ((size_t)1 << shift) + 8 * (1024 + 1024 + 256) + 64; }
 
 extern "C" size_t fl3k_bin_narrow_lds_bytes(int shift) { return (size_t)16 * ((size_t)1 << shift) + 4 * 1024 + 64; }
 extern "C" int fl3k_bin_pass_records(void) { return FL3_ACC_UNROLL * FL3_ACC_THREADS; }

@@ -363,10 +363,11 @@ BITS64_CASES = [
 
 @pytest.mark.parametrize("name,mk,walkers,chunk", BITS64_CASES, ids=[c[0] for c in BITS64_CASES])
 def test_binned_bits64_equals_atomic_bits64(prod, name, mk, walkers, chunk):
-    """Bits-64 buckets through the tile-binned deposit: each record adds in
-    double what the per-sample double atomics add for its point, so the
-    buckets differ only in the order of the additions (a few ulps), and the
-    8-bit images with density estimation off agree but for a rare rounding
+    """Bits-64 buckets through the tile-binned deposit: each record adds what
+    the per-sample double atomics add for its point, rounded to a multiple
+    of 2^-32 (summed in 64-bit integers in LDS), so the buckets differ by
+    about 1e-12 relative besides the order of the additions, and the 8-bit
+    images with density estimation off agree but for a rare rounding
     boundary."""
     g = mk()
     g.estimator_radius = 0.0
