@@ -626,9 +626,9 @@ __global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
 // LDS as signed 64-bit integers (integer atomics outpace double ones; a
 // work item of at most 2^31 / (largest |entry| + 1) records keeps a cell's
 // sum below 2^63 in size); the item's sums are then added to the double
-// buckets.  Against the
-// per-sample double atomics a cell's sum differs by at most 2^-33 per
-// record (about 1e-12 relative), besides the order of the additions.
+// buckets.  Against the per-sample double atomics a cell's sum differs by
+// at most 2^-33 per record (about 1e-12 relative), besides the order of the
+// additions.
 // 4-byte records (stepped palette, visibility 1) or 8-byte records
 // (make_record).  Persistent workgroups take work items as in k_bin_accum;
 // a tile with one item is added without atomics.
