@@ -135,8 +135,9 @@ __global__ __launch_bounds__(256, walker_waves<TIER>()) void k_iter_fast(IterArg
          deposit<float, BITS, EXACT>(G, v, buckets, satp);
    };
    if constexpr (WAVE) {
-      // One xform per wavefront and step (drawn from the first lane's
-      // stream), so a wavefront evaluates that xform's variations only; after
+      // One xform per wavefront and step (drawn from a hash of the
+      // wavefront's seed, the step and the draw within the step, in scalar
+      // registers), so a wavefront evaluates that xform's variations only; after
       // every step the workgroup's 256 points move to other lanes through a
       // random permutation (slot = m (t ^ x) + o mod 256, m odd, from a hash
       // of the job's key, the job-local workgroup and the step), so the
@@ -149,6 +150,17 @@ __global__ __launch_bounds__(256, walker_waves<TIER>()) void k_iter_fast(IterArg
       const uint32_t bjob = dsc ? (uint32_t)(blockIdx.x - dsc->block0)
                                 : (nj > 1 ? (uint32_t)(blockIdx.x / nj) : (uint32_t)blk);
       const uint32_t hseed = ((uint32_t)kk ^ (uint32_t)(kk >> 32) * 0x85ebca6bu) ^ bjob * 0xc2b2ae35u;
+      // the wavefront's xform draws: draw k of step st is
+      // fmix32(wseed ^ (st * 0xcc9e2d51 + k * 0x1b873593)), the 32-bit
+      // finaliser of MurmurHash3 (Appleby), over a seed mixed from the
+      // workgroup's and the wavefront's index (so it depends on the job-local
+      // workgroup, like the permutation, and not on the launch)
+      auto fmix32 = [](uint32_t h) {
+         h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+         return h;
+      };
+      const uint32_t wseed = fmix32(hseed + 0x632be5abu * ((uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) + 1u));
+      auto wdraw = [&](uint32_t step, uint32_t k) { return fmix32(wseed ^ (step * 0xcc9e2d51u + k * 0x1b873593u)); };
       // the xform at the first slot of each 64-slot bucket of the 14-bit
       // selection slot: a draw starts there and steps over the few
       // breakpoints inside its bucket (select_xform's count, in fewer steps)
@@ -160,16 +172,20 @@ __global__ __launch_bounds__(256, walker_waves<TIER>()) void k_iter_fast(IterArg
          sel[threadIdx.x] = (unsigned short)f;
       }
       __syncthreads();
-      uint32_t st = a.resume ? (uint32_t)lastxf : 0u;
+      // (the step count and the workgroup's step bound are the same in
+      // every lane: read from the first lane, they live in scalar
+      // registers, and so does the permutation's hash below)
+      uint32_t st = a.resume ? (uint32_t)__builtin_amdgcn_readfirstlane(lastxf) : 0u;
       // every lane takes one step at a time, so i is the same in all lanes
       // and the workgroup runs until its first walker (the one with the
       // most samples) is done
       long long nwg = nsj / Wj + ((wj - (long long)threadIdx.x) < nsj % Wj ? 1 : 0);
       if (dsc && RECORDS) nwg = nwg - a.row0 < 0 ? 0 : (nwg - a.row0 < a.rows_out ? nwg - a.row0 : a.rows_out);
+      nwg = (long long)(((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(nwg >> 32)) << 32) |
+                        (uint32_t)__builtin_amdgcn_readfirstlane((int)nwg));
       long long i = a.resume ? 0 : -a.fuse;
       for (;;) {
-         const uint32_t r = rng.next();
-         const uint32_t xslot = __builtin_amdgcn_readfirstlane(r) & 16383u;
+         const uint32_t xslot = wdraw(st, 0u) & 16383u;
          int fs = __builtin_amdgcn_readfirstlane(sel[xslot >> 6]);
          while (fs + 1 < G.nstd && xslot >= (uint32_t)__builtin_amdgcn_readfirstlane(G.starts[fs + 1])) fs++;
          int fn = fs;
@@ -177,19 +193,20 @@ __global__ __launch_bounds__(256, walker_waves<TIER>()) void k_iter_fast(IterArg
          // random replacement apply_xform left in q and the step is retried
          // with a new xform, up to four times; the fifth bad value in a row
          // is kept.  The retries stay inside the step (the lanes without a
-         // bad value wait), drawing each retry's xform from the first
-         // lane's stream, so every lane still plots exactly one
-         // point per step and chunked launches follow the single launch.
+         // bad value wait), each retry's xform the step's next wavefront
+         // draw, so every lane still plots exactly one point per step and
+         // chunked launches follow the single launch.
          // Bad values are counted on plotted steps only: the reference's
          // count holds its fuse steps too (15 per 10000-sample sub-batch,
          // 0.15% of its steps), but a walker fuses 100 steps, which for
          // small renders would be a large share of the count.
          bool pend = apply_xform<float, WalkerRng, TIER, true>(G, fn, p, q, rng);
          int nbad = 0;
+         uint32_t kdraw = 0;
          while (__builtin_amdgcn_ballot_w64(pend)) {
-            // (the retry's xform is drawn outside the branch, from the first
-            // lane's stream, like the step's own draw)
-            const uint32_t rs = __builtin_amdgcn_readfirstlane(rng.next()) & 16383u;
+            // (the retry's xform is drawn outside the branch, like the
+            // step's own draw)
+            const uint32_t rs = wdraw(st, ++kdraw) & 16383u;
             int f2 = __builtin_amdgcn_readfirstlane(sel[rs >> 6]);
             while (f2 + 1 < G.nstd && rs >= (uint32_t)__builtin_amdgcn_readfirstlane(G.starts[f2 + 1])) f2++;
             if (pend) {
