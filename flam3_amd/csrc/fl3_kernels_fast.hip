@@ -604,7 +604,16 @@ __global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
    for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) hist[b] = 0;
    __syncthreads();
    const RT *rec = (const RT *)a.rec;
-   for (long long i = 0; i < a.rows; i++) {
+   // (the rows' loads are issued in batches before their counts, so that a
+   // thread has many loads in flight)
+   long long i = 0;
+   for (; i + FL3_SCATTER_UNROLL <= a.rows; i += FL3_SCATTER_UNROLL) {
+      RT v[FL3_SCATTER_UNROLL];
+      for (int k = 0; k < FL3_SCATTER_UNROLL; k++) v[k] = rec_at(a, rec, (blockIdx.x * a.rows + i + k) * 256 + threadIdx.x);
+      for (int k = 0; k < FL3_SCATTER_UNROLL; k++)
+         if (!rec_none(v[k])) atomicAdd(&hist[rel_cell(a, v[k]) >> a.shift], 1u);
+   }
+   for (; i < a.rows; i++) {
       const RT v = rec_at(a, rec, (blockIdx.x * a.rows + i) * 256 + threadIdx.x);
       if (!rec_none(v)) atomicAdd(&hist[rel_cell(a, v) >> a.shift], 1u);
    }
