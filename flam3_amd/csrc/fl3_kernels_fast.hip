@@ -638,9 +638,10 @@ __global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
 // buckets.  Against the per-sample double atomics a cell's sum differs by
 // at most 2^-33 per record (about 1e-12 relative), besides the order of the
 // additions.
-// 4-byte records (stepped palette, visibility 1) or 8-byte records
-// (make_record).  Persistent workgroups take work items as in k_bin_accum;
-// a tile with one item is added without atomics.
+// 4-byte records (stepped palette, visibility 1; each adds exactly 255 to
+// the count, which is then a 32-bit record count scaled at the flush) or
+// 8-byte records (make_record).  Persistent workgroups take work items as
+// in k_bin_accum; a tile with one item is added without atomics.
 #define FL3_F64_FIX 4294967296.0    // 2^32: the fixed-point scale of the LDS sums
 template <typename RT>
 __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_f64(BinArgs a) {
@@ -694,9 +695,12 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_f64(BinArgs a) {
             const unsigned local = rel_cell(a, v[u]) - tb;
             unsigned long long d[5];
             if (sizeof(RT) == 4) {
+               // (every 4-byte record adds 255 hits: the count channel is a
+               // 32-bit record count, times 255 at the flush)
                const unsigned long long *e = dfx + ((unsigned)v[u] >> 24) * 4;
-               for (int ch = 0; ch < 4; ch++) d[ch] = e[ch];
-               d[4] = hits;
+               for (int ch = 0; ch < 4; ch++) atomicAdd(&acc[ch * tile_cells + local], e[ch]);
+               atomicAdd((unsigned *)(acc + 4 * tile_cells) + local, 1u);
+               continue;
             } else {
                const unsigned long long x = (unsigned long long)v[u];
                const int ci = (int)((x >> 32) & 0xff), fn = (int)((x >> 40) & 0xff);
@@ -730,7 +734,10 @@ __global__ __launch_bounds__(FL3_ACC_THREADS) void k_bin_accum_f64(BinArgs a) {
       double val[K];
       for (int k = 0; k < K; k++) {
          const int e = threadIdx.x + k * FL3_ACC_THREADS;
-         val[k] = e < nvalid * 5 ? (double)(long long)acc[(e % 5) * tile_cells + e / 5] * (1.0 / FL3_F64_FIX) : 0.0;
+         val[k] = e >= nvalid * 5 ? 0.0
+                  : sizeof(RT) == 4 && e % 5 == 4
+                      ? (double)((const unsigned *)(acc + 4 * tile_cells))[e / 5] * 255.0
+                      : (double)(long long)acc[(e % 5) * tile_cells + e / 5] * (1.0 / FL3_F64_FIX);
       }
       if (single) {
          double old[K];
