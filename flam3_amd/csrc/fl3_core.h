@@ -222,6 +222,9 @@ struct GenL {
    const double *dmapd;    // LDS [256*4] FP64 copy (FP64 histograms)
    const uint8_t *vlist;   // LDS [nxf * nunion] union slots with nonzero weight, ascending
    const uint8_t *vcount;  // LDS [nxf]
+   // global, read with scalar loads: per xform the count of nonzero weights,
+   // then (variation id, float weight bits) pairs (fl3_job_hdr::off_vl)
+   const __attribute__((address_space(4))) int32_t *vlc;
    int nxf, nstd, final_index, chaos_enable, nunion, cmap_size, palette_mode, rotate, width, height;
    R b0, b1, b2, b3, rot00, rot01, rot10, rot11, rc0, rc1, ws0, wb0s0, hs1, hb1s1, final_opacity;
 };
@@ -311,6 +314,7 @@ __device__ void load_genome(const unsigned char *job, unsigned char *lds, GenL<R
    }
    G.xf = xf; G.w = w; G.starts = st; G.uid = uid; G.dmap = dmap; G.dmapu = dmapu; G.dmapd = dmapd;
    G.vlist = vl; G.vcount = vc;
+   G.vlc = (const __attribute__((address_space(4))) int32_t *)(job + h->off_vl);
 }
 
 #include "fl3_variations.inc"
@@ -403,15 +407,31 @@ __device__ bool apply_xform(const GenL<R> &G, int fn, const R *p, R *q, RNG &rng
    const uint32_t xoff = (uint32_t)((const unsigned char *)&X - fl3_lds);
    if (UNI) {
       const int fu = __builtin_amdgcn_readfirstlane(fn);
-      const int nv = __builtin_amdgcn_readfirstlane(G.vcount[fu]);
-      for (int k = 0; k < nv; k++) {
-         const int u = __builtin_amdgcn_readfirstlane((int)G.vlist[fu * G.nunion + k]);
-         const int v = __builtin_amdgcn_readfirstlane(G.uid[u]);
-         const R w = wrow[u];
-         uint32_t o = xoff;
-         asm volatile("" : "+v"(o));
-         opaque_pre(f);
-         fl3_var_tier<R, RNG, TIER>(v, w, *(const XfL<R> *)(fl3_lds + o), f, rng);
+      if constexpr (sizeof(R) == 4) {
+         // the xform's variation ids and weights through scalar loads of
+         // the job's list (the same floats, in the same order, as the LDS
+         // tables hold)
+         const __attribute__((address_space(4))) int32_t *vl = G.vlc + fu * (2 + 2 * G.nunion);
+         const int nv = vl[0];
+         for (int k = 0; k < nv; k++) {
+            const int v = vl[2 + 2 * k];
+            const R w = __int_as_float(vl[3 + 2 * k]);
+            uint32_t o = xoff;
+            asm volatile("" : "+v"(o));
+            opaque_pre(f);
+            fl3_var_tier<R, RNG, TIER>(v, w, *(const XfL<R> *)(fl3_lds + o), f, rng);
+         }
+      } else {
+         const int nv = __builtin_amdgcn_readfirstlane(G.vcount[fu]);
+         for (int k = 0; k < nv; k++) {
+            const int u = __builtin_amdgcn_readfirstlane((int)G.vlist[fu * G.nunion + k]);
+            const int v = __builtin_amdgcn_readfirstlane(G.uid[u]);
+            const R w = wrow[u];
+            uint32_t o = xoff;
+            asm volatile("" : "+v"(o));
+            opaque_pre(f);
+            fl3_var_tier<R, RNG, TIER>(v, w, *(const XfL<R> *)(fl3_lds + o), f, rng);
+         }
       }
    } else {
       for (int u = 0; u < G.nunion; u++) {

@@ -136,7 +136,7 @@ void *fl3_pack_job(const flam3_genome *cp, const unsigned short *distrib, const 
 
    off = align8(sizeof(fl3_job_hdr));
    {
-      size_t o_xf = off, o_w, o_st, o_dm;
+      size_t o_xf = off, o_w, o_st, o_dm, o_vl;
       off += sizeof(fl3_xform_pack) * nxf;
       o_w = off;
       off = align8(off + sizeof(double) * nxf * (nunion > 0 ? nunion : 1));
@@ -144,6 +144,8 @@ void *fl3_pack_job(const flam3_genome *cp, const unsigned short *distrib, const 
       off = align8(off + sizeof(int32_t) * nrows * nstd);
       o_dm = off;
       off += sizeof(double) * 1024;
+      o_vl = off;
+      off = align8(off + sizeof(int32_t) * nxf * (2 + 2 * (size_t)nunion));
       total = off;
       blob = (unsigned char *)calloc(1, total);
       if (!blob) { free(own); return NULL; }
@@ -152,6 +154,7 @@ void *fl3_pack_job(const flam3_genome *cp, const unsigned short *distrib, const 
       h->off_w = (int64_t)o_w;
       h->off_starts = (int64_t)o_st;
       h->off_dmap = (int64_t)o_dm;
+      h->off_vl = (int64_t)o_vl;
    }
    h->bytes = (int32_t)total;
    h->nxf = nxf;
@@ -187,6 +190,23 @@ void *fl3_pack_job(const flam3_genome *cp, const unsigned short *distrib, const 
          }
       }
       fl3_distrib_breakpoints(distrib, nrows, nstd, st);
+      /* per xform, its nonzero weights (as the walkers' floats) and their
+       * variation ids, in union order: the list a wavefront that applies
+       * one xform reads with scalar loads */
+      for (i = 0; i < nxf; i++) {
+         int32_t *vl = (int32_t *)(blob + h->off_vl) + (size_t)i * (2 + 2 * (size_t)nunion);
+         int k = 0;
+         for (u = 0; u < nunion; u++) {
+            const float wf = (float)w[i * nunion + u];
+            if (wf != 0.0f) {
+               vl[2 + 2 * k] = h->union_ids[u];
+               memcpy(&vl[3 + 2 * k], &wf, sizeof(wf));
+               k++;
+            }
+         }
+         vl[0] = k;
+         vl[1] = 0;
+      }
    }
 
    if (cam) {
