@@ -413,14 +413,33 @@ __device__ bool apply_xform(const GenL<R> &G, int fn, const R *p, R *q, RNG &rng
          // tables hold)
          const __attribute__((address_space(4))) int32_t *vl = G.vlc + fu * (2 + 2 * G.nunion);
          const int nv = vl[0];
-         for (int k = 0; k < nv; k++) {
-            const int v = vl[2 + 2 * k];
-            const R w = __int_as_float(vl[3 + 2 * k]);
-            uint32_t o = xoff;
-            asm volatile("" : "+v"(o));
-            opaque_pre(f);
-            fl3_var_tier<R, RNG, TIER>(v, w, *(const XfL<R> *)(fl3_lds + o), f, rng);
-         }
+         if constexpr (TIER == 0) {
+            // Of the classic variations only julia draws (one word per
+            // application): its word is drawn before the loop, so the lane's
+            // stream does not change inside the variation switch (where its
+            // state would be merged after every case).
+            struct WordRng {
+               uint32_t w;
+               __device__ uint32_t next() { return w; }
+            } wr;
+            wr.w = (flags & FL3_DRAWS_JULIA) ? rng.next() : 0u;
+            for (int k = 0; k < nv; k++) {
+               const int v = vl[2 + 2 * k];
+               const R w = __int_as_float(vl[3 + 2 * k]);
+               uint32_t o = xoff;
+               asm volatile("" : "+v"(o));
+               opaque_pre(f);
+               fl3_var_tier<R, WordRng, TIER>(v, w, *(const XfL<R> *)(fl3_lds + o), f, wr);
+            }
+         } else
+            for (int k = 0; k < nv; k++) {
+               const int v = vl[2 + 2 * k];
+               const R w = __int_as_float(vl[3 + 2 * k]);
+               uint32_t o = xoff;
+               asm volatile("" : "+v"(o));
+               opaque_pre(f);
+               fl3_var_tier<R, RNG, TIER>(v, w, *(const XfL<R> *)(fl3_lds + o), f, rng);
+            }
       } else {
          const int nv = __builtin_amdgcn_readfirstlane(G.vcount[fu]);
          for (int k = 0; k < nv; k++) {

@@ -16,6 +16,7 @@
 #include "fl3_pack.h"
 
 #define VAR_PRE_BLUR 67
+#define VAR_JULIA 13
 
 static int usable(const flam3_genome *cp, int i) {
    if (i == cp->final_xform_index) return 1;
@@ -201,6 +202,7 @@ void *fl3_pack_job(const flam3_genome *cp, const unsigned short *distrib, const 
             if (wf != 0.0f) {
                vl[2 + 2 * k] = h->union_ids[u];
                memcpy(&vl[3 + 2 * k], &wf, sizeof(wf));
+               if (h->union_ids[u] == VAR_JULIA) px[i].flags |= FL3_DRAWS_JULIA;
                k++;
             }
          }

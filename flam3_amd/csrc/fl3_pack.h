@@ -59,6 +59,7 @@ enum fl3_param {
 #define FL3_PRE_ANGLES 2
 #define FL3_PRE_ATAN_YX 4
 #define FL3_HAS_POST 8
+#define FL3_DRAWS_JULIA 16   /* julia (variation 13) is among the xform's nonzero float weights */
 
 typedef struct {
    double c[6];        /* c00 c01 c10 c11 c20 c21 (x' = c00 x + c10 y + c20) */
