@@ -378,8 +378,13 @@ __device__ bool apply_xform(const GenL<R> &G, int fn, const R *p, R *q, RNG &rng
    R s1 = X.cspeed;
    q[2] = s1 * X.color + (R(1.0) - s1) * p[2];
    q[3] = X.vis;
-   f.tx = X.c[0] * p[0] + X.c[2] * p[1] + X.c[4];
-   f.ty = X.c[1] * p[0] + X.c[3] * p[1] + X.c[5];
+   if constexpr (sizeof(R) == 4) {   // (two fused multiply-adds per coordinate)
+      f.tx = fmaf(X.c[0], p[0], fmaf(X.c[2], p[1], X.c[4]));
+      f.ty = fmaf(X.c[1], p[0], fmaf(X.c[3], p[1], X.c[5]));
+   } else {
+      f.tx = X.c[0] * p[0] + X.c[2] * p[1] + X.c[4];
+      f.ty = X.c[1] * p[0] + X.c[3] * p[1] + X.c[5];
+   }
    if (X.preblur != R(0)) {  // pre_blur :1480
       R g0 = u01<R>(rng), g1 = u01<R>(rng), g2 = u01<R>(rng), g3 = u01<R>(rng);
       R rndG = X.preblur * (g0 + g1 + g2 + g3 - R(2.0));
@@ -468,8 +473,13 @@ __device__ bool apply_xform(const GenL<R> &G, int fn, const R *p, R *q, RNG &rng
       }
    }
    if (flags & FL3_HAS_POST) {
-      q[0] = X.post[0] * f.p0 + X.post[2] * f.p1 + X.post[4];
-      q[1] = X.post[1] * f.p0 + X.post[3] * f.p1 + X.post[5];
+      if constexpr (sizeof(R) == 4) {
+         q[0] = fmaf(X.post[0], f.p0, fmaf(X.post[2], f.p1, X.post[4]));
+         q[1] = fmaf(X.post[1], f.p0, fmaf(X.post[3], f.p1, X.post[5]));
+      } else {
+         q[0] = X.post[0] * f.p0 + X.post[2] * f.p1 + X.post[4];
+         q[1] = X.post[1] * f.p0 + X.post[3] * f.p1 + X.post[5];
+      }
    } else {
       q[0] = f.p0;
       q[1] = f.p1;
