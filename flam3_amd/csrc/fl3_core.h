@@ -592,7 +592,9 @@ __device__ __forceinline__ void deposit(const GenL<R> &G, const R *q, void *buck
 // blend fraction in 16-bit fixed point.
 #define FL3_REC_NONE 0xffffffffu
 
-template <typename R>
+// STEPPED: the caller knows the palette is stepped (4-byte records), so the
+// palette index is clamped without the linear mode's branches.
+template <typename R, bool STEPPED = false>
 __device__ __forceinline__ unsigned long long make_record(const GenL<R> &G, const R *q, int fn) {
 #pragma clang fp contract(off)
    R x = q[0], y = q[1];
@@ -612,6 +614,10 @@ __device__ __forceinline__ unsigned long long make_record(const GenL<R> &G, cons
    const int iy = (int)(G.hs1 * y - G.hb1s1);
    const unsigned cell = (unsigned)ix + __umul24((unsigned)G.width, (unsigned)iy);
    if (cell >= (unsigned)G.width * (unsigned)G.height) return FL3_REC_NONE;
+   if constexpr (STEPPED) {   // point_color's stepped branch
+      const int c = (int)(q[2] * (R)G.cmap_size), m1 = G.cmap_size - 1;
+      return (unsigned long long)cell | ((unsigned long long)(c < 0 ? 0 : (c >= m1 ? m1 : c)) << 32);
+   }
    int ci;
    R frac;
    point_color(G, q[2], ci, frac);

@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256, walker_waves<TIER>()) void k_iter_fast(IterArg
       } else if (RECORDS == FL3_REC_NARROW) {
          // (independent renders sharing one palette: the render's first
          // cell moves the cell into the launch's cell space)
-         const unsigned long long r = make_record<float>(G, v, 0);
+         const unsigned long long r = make_record<float, true>(G, v, 0);   // (4-byte records: a stepped palette)
          const unsigned cell = (unsigned)r == FL3_REC_NONE ? FL3_REC_NONE : (unsigned)(r + cell0);
          ((unsigned *)a.rec)[(blk * a.rows_out + i) * 256 + threadIdx.x] =
              cell == FL3_REC_NONE ? FL3_REC_NONE : (cell | ((unsigned)(r >> 32) & 0xffu) << 24);
