@@ -400,15 +400,21 @@ __device__ __forceinline__ void de_gather_body(const DEArgs &a, int bx, int by) 
          // this cell loads index 0 and adds it times 0 (the per-source
          // values are finite), in 32-bit index arithmetic, fused
          // multiply-adds.
+         // (unsigned distances, |a - b| in the absolute-difference form, a
+         // 24-bit multiply for the triangle number and the missed tap's
+         // index cleared by a mask, so that the body has no branch)
          const AT *ct = (const AT *)a.coefs_t;
+         const unsigned ux = (unsigned)x, uy = (unsigned)y;
 #pragma unroll 4
          for (unsigned i = 0; i < nb; i++) {
             const unsigned k = s_list[wave][i];
             const int4 src = s_src[k];
-            const int dx = abs(x - src.x), dy = abs(y - src.y);
-            const int hi = max(dx, dy), lo = min(dx, dy);
-            const bool tap = live && hi <= src.w;
-            const unsigned ci = tap ? (unsigned)src.z + ((unsigned)(hi * (hi + 1)) >> 1) + (unsigned)lo : 0u;
+            const unsigned sx = (unsigned)src.x, sy = (unsigned)src.y;
+            const unsigned dx = ux > sx ? ux - sx : sx - ux, dy = uy > sy ? uy - sy : sy - uy;
+            const unsigned hi = max(dx, dy), lo = min(dx, dy);
+            const bool tap = live && hi <= (unsigned)src.w;
+            const unsigned cr = (unsigned)src.z + (__umul24(hi, hi + 1u) >> 1) + lo;
+            const unsigned ci = cr & (0u - (unsigned)tap);
             const AT cv = ct[ci];
             const AT coef = tap ? cv : (AT)0;
             for (int c = 0; c < 4; c++) acc[c] = fma(coef, s_val[k][c], acc[c]);
