@@ -357,7 +357,11 @@ __device__ __forceinline__ void de_gather_body(const DEArgs &a, int bx, int by) 
       if (flag) {
          const unsigned idx = off + rank;
          const long cell = sx + (long)sy * W;
-         s_src[idx] = make_int4(sx, sy, fi * a.kernel_size, radius[fi]);
+         if constexpr (EXACT)
+            s_src[idx] = make_int4(sx, sy, fi * a.kernel_size, radius[fi]);
+         else   // (as floats: every value here is an integer below 2^24)
+            s_src[idx] = make_int4(__float_as_int((float)sx), __float_as_int((float)sy),
+                                   __float_as_int((float)(fi * a.kernel_size)), __float_as_int((float)radius[fi]));
          if constexpr (EXACT) {
             for (int c = 0; c < 4; c++) s_val[idx][c] = bucket_value(a.buckets, cell * 5 + c, (AT)0);
             s_val[idx][4] = a.srcl[cell];
@@ -400,21 +404,21 @@ __device__ __forceinline__ void de_gather_body(const DEArgs &a, int bx, int by) 
          // this cell loads index 0 and adds it times 0 (the per-source
          // values are finite), in 32-bit index arithmetic, fused
          // multiply-adds.
-         // (unsigned distances, |a - b| in the absolute-difference form, a
-         // 24-bit multiply for the triangle number and the missed tap's
-         // index cleared by a mask, so that the body has no branch)
+         // (distances in float, exact for these integers below 2^24, so
+         // that |.| comes free with the max and min; the tap's index
+         // z + hi (hi + 1) / 2 + lo by exact fused multiply-adds; a missed
+         // tap's index cleared by a mask, so that the body has no branch)
          const AT *ct = (const AT *)a.coefs_t;
-         const unsigned ux = (unsigned)x, uy = (unsigned)y;
+         const float fx = (float)x, fy = (float)y;
 #pragma unroll 4
          for (unsigned i = 0; i < nb; i++) {
             const unsigned k = s_list[wave][i];
             const int4 src = s_src[k];
-            const unsigned sx = (unsigned)src.x, sy = (unsigned)src.y;
-            const unsigned dx = ux > sx ? ux - sx : sx - ux, dy = uy > sy ? uy - sy : sy - uy;
-            const unsigned hi = max(dx, dy), lo = min(dx, dy);
-            const bool tap = live && hi <= (unsigned)src.w;
-            const unsigned cr = (unsigned)src.z + (__umul24(hi, hi + 1u) >> 1) + lo;
-            const unsigned ci = cr & (0u - (unsigned)tap);
+            const float ax = fx - __int_as_float(src.x), ay = fy - __int_as_float(src.y);
+            const float hi = fmaxf(fabsf(ax), fabsf(ay)), lo = fminf(fabsf(ax), fabsf(ay));
+            const bool tap = live && hi <= __int_as_float(src.w);
+            const float cf = fmaf(fmaf(hi, hi, hi), 0.5f, lo) + __int_as_float(src.z);
+            const unsigned ci = (unsigned)cf & (0u - (unsigned)tap);
             const AT cv = ct[ci];
             const AT coef = tap ? cv : (AT)0;
             for (int c = 0; c < 4; c++) acc[c] = fma(coef, s_val[k][c], acc[c]);
