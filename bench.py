@@ -93,8 +93,8 @@ def log(*a):
 def parse_args():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=5)
-    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--quality", type=float, default=500.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-quality", type=float, default=60.0)
