@@ -38,7 +38,8 @@ __device__ __forceinline__ float m_tan(float x) {
 __device__ __forceinline__ float m_atan2(float y, float x) {
    const float ax = fabsf(x), ay = fabsf(y);
    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
-   const float t = mx > 0.0f ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f;
+   // (mn <= mx: at the origin both are 0 and t is 0 * 1e30)
+   const float t = mn * __builtin_amdgcn_rcpf(fmaxf(mx, 1e-30f));
    const float s = t * t;
    float p = 0.00681174332f;
    p = p * s - 0.0336040774f;
@@ -377,7 +378,10 @@ __device__ bool apply_xform(const GenL<R> &G, int fn, const R *p, R *q, RNG &rng
    const XfL<R> &X = G.xf[fn];
    Pre<R> f;
    R s1 = X.cspeed;
-   q[2] = s1 * X.color + (R(1.0) - s1) * p[2];
+   if constexpr (sizeof(R) == 4)   // (the same blend, p + s (c - p), in one fused multiply-add)
+      q[2] = fmaf(s1, X.color - p[2], p[2]);
+   else
+      q[2] = s1 * X.color + (R(1.0) - s1) * p[2];
    q[3] = X.vis;
    if constexpr (sizeof(R) == 4) {   // (two fused multiply-adds per coordinate)
       f.tx = fmaf(X.c[0], p[0], fmaf(X.c[2], p[1], X.c[4]));
@@ -429,13 +433,13 @@ __device__ bool apply_xform(const GenL<R> &G, int fn, const R *p, R *q, RNG &rng
                __device__ uint32_t next() { return w; }
             } wr;
             wr.w = (flags & FL3_DRAWS_JULIA) ? rng.next() : 0u;
+            // (the classic cases read no xform parameters, so the xform
+            // needs no opaque offset here)
             for (int k = 0; k < nv; k++) {
                const int v = vl[2 + 2 * k];
                const R w = __int_as_float(vl[3 + 2 * k]);
-               uint32_t o = xoff;
-               asm volatile("" : "+v"(o));
                opaque_pre(f);
-               fl3_var_tier<R, WordRng, TIER>(v, w, *(const XfL<R> *)(fl3_lds + o), f, wr);
+               fl3_var_tier<R, WordRng, TIER>(v, w, X, f, wr);
             }
          } else
             for (int k = 0; k < nv; k++) {
@@ -604,7 +608,8 @@ __device__ __forceinline__ unsigned long long make_record(const GenL<R> &G, cons
       x = x0 * G.rot00 + y0 * G.rot01 + G.rc0;
       y = x0 * G.rot10 + y0 * G.rot11 + G.rc1;
    }
-   if (!(x >= G.b0 && y >= G.b1 && x <= G.b2 && y <= G.b3) || q[3] == R(0)) return FL3_REC_NONE;
+   // (4-byte records carry points of visibility 1 only: no visibility test)
+   if (!(x >= G.b0 && y >= G.b1 && x <= G.b2 && y <= G.b3) || (!STEPPED && q[3] == R(0))) return FL3_REC_NONE;
    // the reference's linear index ix + width * iy (rect.c:440-441) in 32-bit
    // unsigned arithmetic: histograms here stay below 2^31 cells, so a
    // negative index wraps above the cell count and is dropped like one past
