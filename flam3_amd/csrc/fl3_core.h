@@ -434,12 +434,17 @@ __device__ bool apply_xform(const GenL<R> &G, int fn, const R *p, R *q, RNG &rng
             } wr;
             wr.w = (flags & FL3_DRAWS_JULIA) ? rng.next() : 0u;
             // (the classic cases read no xform parameters, so the xform
-            // needs no opaque offset here)
+            // needs no opaque offset here; the first four pairs come in one
+            // scalar load before the loop -- past the list's end, its
+            // padding -- so that the loop waits for no load)
+            const __attribute__((address_space(4))) int32_t *pv = vl + 2;
+            int pr[8];
+            for (int q = 0; q < 8; q++) pr[q] = pv[q];
             for (int k = 0; k < nv; k++) {
-               const int v = vl[2 + 2 * k];
-               const R w = __int_as_float(vl[3 + 2 * k]);
+               const int v = k < 4 ? pr[2 * k] : pv[2 * k];
+               const int wb = k < 4 ? pr[2 * k + 1] : pv[2 * k + 1];
                opaque_pre(f);
-               fl3_var_tier<R, WordRng, TIER>(v, w, X, f, wr);
+               fl3_var_tier<R, WordRng, TIER>(v, __int_as_float(wb), X, f, wr);
             }
          } else
             for (int k = 0; k < nv; k++) {

@@ -146,7 +146,9 @@ void *fl3_pack_job(const flam3_genome *cp, const unsigned short *distrib, const 
       o_dm = off;
       off += sizeof(double) * 1024;
       o_vl = off;
-      off = align8(off + sizeof(int32_t) * nxf * (2 + 2 * (size_t)nunion));
+      /* (and four pairs of padding: a walker reads an xform's first four
+       * pairs at once, whatever its count) */
+      off = align8(off + sizeof(int32_t) * nxf * (2 + 2 * (size_t)nunion) + 8 * sizeof(int32_t));
       total = off;
       blob = (unsigned char *)calloc(1, total);
       if (!blob) { free(own); return NULL; }

@@ -94,9 +94,10 @@ typedef struct {
    int64_t off_w;          /* double[nxf * nunion] weights by union slot */
    int64_t off_starts;     /* int32[nrows * nstd] selection breakpoints */
    int64_t off_dmap;       /* double[256 * 4] palette x 255 x temporal weight */
-   int64_t off_vl;         /* int32[nxf][2 + 2 nunion]: per xform the number of its
-                              weights that are nonzero as floats, 0, then (variation
-                              id, the weight's float bits) for each, in union order */
+   int64_t off_vl;         /* int32[nxf][2 + 2 nunion] + 8 (padding): per xform the
+                              number of its weights that are nonzero as floats, 0,
+                              then (variation id, the weight's float bits) for each,
+                              in union order */
 } fl3_job_hdr;
 
 #endif
