@@ -116,7 +116,9 @@ __global__ __launch_bounds__(256, walker_waves<TIER>()) void k_iter_fast(IterArg
       } else if (RECORDS == FL3_REC_JOB) {
          // independent renders: the render's first cell moves the cell
          // into the launch's cell space, the job is the descriptor's
-         const unsigned long long r = make_record<float>(G, v, fn);
+         // (palettes as bytes: every job's palette is stepped and every
+         // visibility 1, the 4-byte records' case)
+         const unsigned long long r = a.pal8 ? make_record<float, true>(G, v, fn) : make_record<float>(G, v, fn);
          const unsigned cell = (unsigned)r == FL3_REC_NONE ? FL3_REC_NONE : (unsigned)(r + cell0);
          const unsigned long long hi = a.pal8 ? (unsigned long long)pal[(r >> 32) & 0xff] << 32
                                               : (r & 0xffff00000000ull) | ((unsigned long long)(dsc ? djob : jb) << 48);
