@@ -122,11 +122,27 @@ def test_replay_render_without_de_matches(prod, ref, name, mk, kw):
     assert gs["num_iters"] == ws["num_iters"]
 
 
+def _many_variations(seed=74, width=192, height=144, quality=40):
+    """Four xforms of seven classic variations each and a final xform of
+    six (weights summing to 1): the walkers' variation lists run past the
+    four pairs they load ahead."""
+    rng = np.random.default_rng(seed)
+    xs = [G._mixed_xform(rng, G.WORKLOAD_VARS, 7, -1.0, 1.0) for _ in range(4)]
+    fin = G._mixed_xform(rng, G.WORKLOAD_VARS, 6, -1.0, 1.0, weight=0.0)
+    fin.color_speed = 0.0
+    for x in xs + [fin]:
+        tot = sum(x.variations.values())
+        x.variations = {k: v / tot for k, v in x.variations.items()}
+    return G.Genome(width=width, height=height, scale=height / 3.0, quality=quality, supersample=1,
+                    estimator_radius=9.0, palette=G.hsv_palette(rng), xforms=xs, final=fin)
+
+
 @pytest.mark.parametrize("mk,kw", [
     (lambda: G.synthetic_small(seed=71, width=256, height=192, quality=40), {}),
     (lambda: G.synthetic_1080p(seed=72, width=256, height=144, quality=40, supersample=2), {}),
     (lambda: G.synthetic_8k(seed=73, width=192, height=108, quality=30, supersample=3), dict(bits=64)),
-], ids=["small", "ss2_final", "xaos_bits64"])
+    (_many_variations, {}),
+], ids=["small", "ss2_final", "xaos_bits64", "many_variations"])
 def test_production_render_within_noise_floor(prod, ref, mk, kw):
     """FP32 production renders vs reference renders of the same genome and
     quality with different seeds: the GPU images are as far from each as
