@@ -7,6 +7,7 @@
  * per-row step breakpoints instead of 32 KB tables.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -392,6 +393,98 @@ unsigned short *fl3_create_xform_distrib(flam3_genome *cp, int *nrows) {
    }
    if (nrows) *nrows = rows;
    return t;
+}
+
+/* Breakpoints of the selection rows the walkers use, without the table:
+ * each row filled as fl3_create_xform_distrib fills it, into one row buffer,
+ * and reduced to its breakpoints.  The last STARTS_CACHE inputs (weights
+ * and, with xaos, the xaos rows) and their breakpoints are kept, since the
+ * temporal samples of a frame, and the frames of an animation, usually share
+ * their weights; the lookup compares the inputs bit for bit. */
+#define STARTS_CACHE 8
+typedef struct {
+   double *key;
+   size_t nkey;
+   int *starts;
+   int nstarts, rows;
+} starts_entry;
+static starts_entry starts_cache[STARTS_CACHE];
+static int starts_next;
+static pthread_mutex_t starts_mu = PTHREAD_MUTEX_INITIALIZER;
+
+int fl3_xform_starts(flam3_genome *cp, int *starts, int *nrows) {
+   int ns = num_std(cp), i, j, r, unity = 1, rows, e;
+   size_t nkey, k = 0;
+   double *key;
+   unsigned short *row;
+   for (i = 0; i < ns; i++)
+      for (j = 0; j < ns; j++)
+         if (fabs(cp->chaos[i][j] - 1.0) > FL3_EPS) unity = 0;
+   cp->chaos_enable = !unity;
+   rows = unity ? 1 : ns + 1;
+   nkey = 2 + (size_t)ns + (unity ? 0 : (size_t)ns * ns);
+   key = (double *)malloc(sizeof(double) * nkey);
+   if (!key) return 1;
+   key[k++] = ns;
+   key[k++] = cp->final_xform_index;
+   for (i = 0; i < ns; i++) key[k++] = cp->xform[i].density;
+   if (!unity)
+      for (i = 0; i < ns; i++)
+         for (j = 0; j < ns; j++) key[k++] = cp->chaos[i][j];
+   pthread_mutex_lock(&starts_mu);
+   for (e = 0; e < STARTS_CACHE; e++) {
+      const starts_entry *c = &starts_cache[e];
+      if (c->key && c->nkey == nkey && memcmp(c->key, key, sizeof(double) * nkey) == 0) {
+         memcpy(starts, c->starts, sizeof(int) * (size_t)c->nstarts);
+         pthread_mutex_unlock(&starts_mu);
+         free(key);
+         if (nrows) *nrows = rows;
+         return 0;
+      }
+   }
+   pthread_mutex_unlock(&starts_mu);
+   row = (unsigned short *)malloc(sizeof(unsigned short) * FL3_CHOOSE_GRAIN);
+   if (!row) {
+      free(key);
+      return 1;
+   }
+   if (fill_row(cp, -1, row)) goto fail;
+   fl3_distrib_breakpoints(row, 1, ns, starts);
+   if (!unity) {
+      r = 0;
+      for (i = 0; i < cp->num_xforms; i++) {
+         if (i == cp->final_xform_index) continue;
+         r++;
+         if (fill_row(cp, i, row)) goto fail;
+         fl3_distrib_breakpoints(row, 1, ns, starts + (size_t)r * ns);
+      }
+   }
+   free(row);
+   {
+      int *copy = (int *)malloc(sizeof(int) * (size_t)rows * ns);
+      if (copy) {
+         memcpy(copy, starts, sizeof(int) * (size_t)rows * ns);
+         pthread_mutex_lock(&starts_mu);
+         starts_entry *c = &starts_cache[starts_next];
+         starts_next = (starts_next + 1) % STARTS_CACHE;
+         free(c->key);
+         free(c->starts);
+         c->key = key;
+         c->nkey = nkey;
+         c->starts = copy;
+         c->nstarts = rows * ns;
+         c->rows = rows;
+         pthread_mutex_unlock(&starts_mu);
+         key = NULL;
+      }
+   }
+   free(key);
+   if (nrows) *nrows = rows;
+   return 0;
+fail:
+   free(row);
+   free(key);
+   return 1;
 }
 
 void fl3_distrib_breakpoints(const unsigned short *distrib, int nrows, int nstd, int *starts) {

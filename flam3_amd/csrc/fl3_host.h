@@ -59,6 +59,13 @@ unsigned short *fl3_create_xform_distrib(flam3_genome *cp, int *nrows);
  * number of j >= 1 with starts[j] <= slot).  Exact for any table built by
  * fl3_create_xform_distrib. */
 void fl3_distrib_breakpoints(const unsigned short *distrib, int nrows, int nstd, int *starts);
+/* The breakpoints of the rows the walkers use (row 0 without xaos, rows
+ * 0..n_std with it; *nrows receives the count) without building the table:
+ * what fl3_distrib_breakpoints gives for fl3_create_xform_distrib's table.
+ * starts holds (n_std + 1) * n_std ints.  Sets cp->chaos_enable.  Returns 0,
+ * or 1 on negative or all-zero weights.  Thread-safe (a small cache of
+ * recent inputs and results). */
+int fl3_xform_starts(flam3_genome *cp, int *starts, int *nrows);
 
 /* ---------------- filters (filters.c) ---------------- */
 typedef struct {
@@ -86,6 +93,9 @@ double fl3_create_temporal_filter(int numsteps, int filter_type, double filter_e
  * none) into a malloc'd job blob; *bytes receives its size. */
 void *fl3_pack_job(const flam3_genome *cp, const unsigned short *distrib, const fl3h_camera *cam,
                    size_t *bytes);
+/* The same from the table's breakpoints (fl3_xform_starts, which also set
+ * cp->chaos_enable). */
+void *fl3_pack_job_starts(const flam3_genome *cp, const int *starts, const fl3h_camera *cam, size_t *bytes);
 void *fl3_pack_camera_job(const fl3h_camera *cam, size_t *bytes);
 
 /* Several renders in two phases (fl3h_render_many = prepare + execute):

@@ -99,8 +99,10 @@ static void pack_xform(const flam3_xform *x, fl3_xform_pack *d) {
 
 static size_t align8(size_t n) { return (n + 7) & ~(size_t)7; }
 
-void *fl3_pack_job(const flam3_genome *cp, const unsigned short *distrib, const fl3h_camera *cam,
-                   size_t *bytes) {
+/* distrib: the selection table, or starts: its breakpoints
+ * (fl3_xform_starts), or neither (built here). */
+static void *pack_job(const flam3_genome *cp, const unsigned short *distrib, const int *starts,
+                      const fl3h_camera *cam, size_t *bytes) {
    int nxf = cp->num_xforms, nstd = nxf - (cp->final_xform_index >= 0);
    int used[FL3_NVAR], nunion = 0, i, j, u, nrows;
    unsigned short *own = NULL;
@@ -128,7 +130,7 @@ void *fl3_pack_job(const flam3_genome *cp, const unsigned short *distrib, const 
    for (j = 0; j < FL3_NVAR; j++)
       if (used[j]) nunion++;
 
-   if (!distrib) {
+   if (!distrib && !starts) {
       own = fl3_create_xform_distrib((flam3_genome *)cp, &nrows);
       if (!own) return NULL;
       distrib = own;
@@ -192,7 +194,10 @@ void *fl3_pack_job(const flam3_genome *cp, const unsigned short *distrib, const 
             if (v >= 0 && v < FL3_NVAR && v != VAR_PRE_BLUR) w[i * nunion + slot[v]] = x->active_var_weights[j];
          }
       }
-      fl3_distrib_breakpoints(distrib, nrows, nstd, st);
+      if (starts)
+         memcpy(st, starts, sizeof(int32_t) * (size_t)nrows * nstd);
+      else
+         fl3_distrib_breakpoints(distrib, nrows, nstd, st);
       /* per xform, its nonzero weights (as the walkers' floats) and their
        * variation ids, in union order: the list a wavefront that applies
        * one xform reads with scalar loads */
@@ -233,6 +238,15 @@ void *fl3_pack_job(const flam3_genome *cp, const unsigned short *distrib, const 
    free(own);
    *bytes = total;
    return blob;
+}
+
+void *fl3_pack_job(const flam3_genome *cp, const unsigned short *distrib, const fl3h_camera *cam,
+                   size_t *bytes) {
+   return pack_job(cp, distrib, NULL, cam, bytes);
+}
+
+void *fl3_pack_job_starts(const flam3_genome *cp, const int *starts, const fl3h_camera *cam, size_t *bytes) {
+   return pack_job(cp, NULL, starts, cam, bytes);
 }
 
 /* A job with one identity xform and only the camera/palette set: what the

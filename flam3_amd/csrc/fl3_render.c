@@ -328,8 +328,8 @@ static int plan_batch_de(render_plan *P, int batch_num, fl3_de_filters *de) {
 static int plan_ts_job(render_plan *P, int tidx, int first, void **job, size_t *job_bytes, double *batch_size) {
    flam3_genome *cp = &P->cp;
    const double color_scalar = P->temporal_filter[tidx];
-   unsigned short *distrib;
-   int i, j, nrows;
+   int *starts;
+   int i, j, nrows, ns;
    *job = NULL;
    if (fl3_interpolate(P->spec->genomes, P->spec->ngenomes, P->spec->time + P->temporal_deltas[tidx], 0, cp))
       return 1;
@@ -338,8 +338,11 @@ static int plan_ts_job(render_plan *P, int tidx, int first, void **job, size_t *
       return 1;
    }
    for (i = 0; i < cp->num_xforms; i++) fl3_xform_precalc(&cp->xform[i]);
-   distrib = fl3_create_xform_distrib(cp, &nrows);
-   if (!distrib) {
+   /* the selection table's breakpoints (what the job holds) */
+   ns = cp->num_xforms - (cp->final_xform_index >= 0);
+   starts = (int *)malloc(sizeof(int) * (size_t)(ns + 1) * (size_t)(ns > 0 ? ns : 1));
+   if (!starts || fl3_xform_starts(cp, starts, &nrows)) {
+      free(starts);
       fprintf(stderr, "create xform distrib returned error: aborting.\n");
       return 1;
    }
@@ -349,7 +352,7 @@ static int plan_ts_job(render_plan *P, int tidx, int first, void **job, size_t *
    P->cam->cmap_size = P->cmap_size;
    P->cam->palette_mode = cp->palette_mode;
    if (setup_camera(P->spec, cp, P->field, P->oversample, &P->geo, P->cam)) {
-      free(distrib);
+      free(starts);
       return 1;
    }
    *batch_size = P->geo.sample_density * P->geo.image_width * P->geo.image_height / (P->nbatches * P->nts);
@@ -362,8 +365,8 @@ static int plan_ts_job(render_plan *P, int tidx, int first, void **job, size_t *
       P->background[2] += cp->background[2];
       P->vib_gam_n++;
    }
-   *job = fl3_pack_job(cp, distrib, P->cam, job_bytes);
-   free(distrib);
+   *job = fl3_pack_job_starts(cp, starts, P->cam, job_bytes);
+   free(starts);
    return *job ? 0 : 1;
 }
 

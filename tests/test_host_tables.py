@@ -34,6 +34,8 @@ def host(prod):
     L.fl3_create_xform_distrib.argtypes = [P(abi.flam3_genome), P(ctypes.c_int)]
     L.fl3_create_xform_distrib.restype = P(ctypes.c_ushort)
     L.fl3_distrib_breakpoints.argtypes = [P(ctypes.c_ushort), ctypes.c_int, ctypes.c_int, P(ctypes.c_int)]
+    L.fl3_xform_starts.argtypes = [P(abi.flam3_genome), P(ctypes.c_int), P(ctypes.c_int)]
+    L.fl3_xform_starts.restype = ctypes.c_int
     L.fl3_create_de_filters.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, P(DEF)]
     L.fl3_create_de_filters.restype = ctypes.c_int
     L.fl3_create_spatial_filter.argtypes = [P(abi.flam3_genome), ctypes.c_double, ctypes.c_int,
@@ -116,6 +118,42 @@ def test_precalc_and_selection_tables_bit_exact(host, ref, name):
             ref.ref().ref_free(got)
     finally:
         G.free_struct(mine)
+
+
+def test_xform_starts_equal_table_breakpoints(host):
+    """fl3_xform_starts (the breakpoints the jobs hold, without the table,
+    with a cache of recent inputs) gives what fl3_distrib_breakpoints gives
+    for fl3_create_xform_distrib's table, with and without xaos: on a first
+    call, on a repeated one (from the cache), and after a weight changes (a
+    new entry)."""
+    libc = ctypes.CDLL(None)
+    libc.free.argtypes = [ctypes.c_void_p]
+    for name, mk in GENOMES.items():
+        g = mk()
+        for changed in (False, True):
+            if changed:
+                g.xforms[0].weight *= 1.37
+            mine = g.to_struct()
+            try:
+                assert host.fl3_prepare_precalc_flags(mine) == 0
+                n = mine.contents.num_xforms
+                nstd = n - (mine.contents.final_xform_index >= 0)
+                nrows = ctypes.c_int()
+                t = host.fl3_create_xform_distrib(mine, ctypes.byref(nrows))
+                used = nrows.value if mine.contents.chaos_enable else 1
+                want = (ctypes.c_int * (used * nstd))()
+                host.fl3_distrib_breakpoints(t, used, nstd, want)
+                libc.free(ctypes.cast(t, ctypes.c_void_p))
+                for _ in range(2):
+                    got = (ctypes.c_int * ((nstd + 1) * nstd))()
+                    rows = ctypes.c_int()
+                    mine.contents.chaos_enable = -1
+                    assert host.fl3_xform_starts(mine, got, ctypes.byref(rows)) == 0
+                    assert rows.value == used, (name, rows.value, used)
+                    assert mine.contents.chaos_enable == (1 if used > 1 else 0)
+                    assert list(got[:used * nstd]) == list(want[:]), (name, changed)
+            finally:
+                G.free_struct(mine)
 
 
 def test_selection_table_weight_shares(host):
