@@ -1376,7 +1376,13 @@ static int iterate_replay(const void *job, size_t job_bytes, uint32_t *isaac_hos
 // the float accumulator, a float copy of the coefficients.
 static int upload_de(const fl3_de_filters *de, double **coefs, double **widths, float **coefs_f) {
    size_t nc = (size_t)de->nfilters * de->kernel_size, nw = de->nfilters;
-   if (g_ctx.table.ensure(sizeof(double) * (nc + nw) + sizeof(float) * nc)) return 1;
+   const void *p0 = g_ctx.table.p;
+   const size_t cap0 = g_ctx.table.cap;
+   if (g_ctx.table.ensure(sizeof(double) * (nc + nw) + sizeof(float) * nc)) {
+      g_ctx.de_up_p = nullptr;
+      return 1;
+   }
+   if (g_ctx.table.p != p0 || g_ctx.table.cap != cap0) g_ctx.de_up_p = nullptr;   // (a new allocation)
    *coefs = (double *)g_ctx.table.p;
    *widths = *coefs + nc;
    *coefs_f = (float *)(*widths + nw);
