@@ -59,6 +59,11 @@ extern "C" const char *fl3h_last_error(void) { return g_err; }
 
 // ------------------------------------------------------------------ device context
 
+// A single image of at least FL3_STRIP_MIN_BYTES comes back in this many
+// strips of rows, the host copying each one out while the next arrives.
+#define FL3_IMAGE_STRIPS 4
+#define FL3_STRIP_MIN_BYTES (2u << 20)
+
 struct DevBuf {
    void *p = nullptr;
    size_t cap = 0;
@@ -109,6 +114,7 @@ struct Ctx {
    DevBuf blk2, soff2, bins2;          // banded histograms: second-level block counts, offsets, counters
    unsigned *band_host = nullptr;      // pinned: first-level tile starts of the last chunk
    hipEvent_t ev_band = nullptr;
+   hipEvent_t ev_strip[FL3_IMAGE_STRIPS] = {};   // fl3d_images_fetch: strip k of the image is in staging
    unsigned long long *counters = nullptr;   // [0] badvals, [1] saturation flag
    void *staging = nullptr;                   // pinned host buffer for the image download
    size_t staging_cap = 0;
@@ -143,6 +149,8 @@ static int ctx_init() {
    }
    HIPCHK(hipEventCreateWithFlags(&g_ctx.ev_join, hipEventDisableTiming));
    HIPCHK(hipEventCreateWithFlags(&g_ctx.ev_band, hipEventDisableTiming));
+   for (int i = 0; i < FL3_IMAGE_STRIPS; i++)
+      HIPCHK(hipEventCreateWithFlags(&g_ctx.ev_strip[i], hipEventDisableTiming));
    HIPCHK(hipHostMalloc((void **)&g_ctx.band_host, sizeof(unsigned) * (FL3_MAX_BINS + 1), hipHostMallocDefault));
    HIPCHK(hipMalloc((void **)&g_ctx.counters, 64));
    g_ctx.ready = 1;
@@ -1800,6 +1808,30 @@ extern "C" int fl3d_images_fetch(const fl3d_image_dst *d, int n) {
       g_ctx.staging_cap = 0;
       HIPCHK(hipHostMalloc(&g_ctx.staging, want, hipHostMallocDefault));
       g_ctx.staging_cap = want;
+   }
+   if (n == 1 && end >= FL3_STRIP_MIN_BYTES && d[0].rows >= FL3_IMAGE_STRIPS) {
+      // One image in strips of rows: the host copies strip k out of staging
+      // while the device copies strip k + 1 in.
+      const size_t rb = d[0].row_bytes;
+      int r0[FL3_IMAGE_STRIPS + 1];
+      for (int k = 0; k <= FL3_IMAGE_STRIPS; k++) r0[k] = (int)((long)d[0].rows * k / FL3_IMAGE_STRIPS);
+      for (int k = 0; k < FL3_IMAGE_STRIPS; k++) {
+         const size_t o = d[0].img_off + rb * (size_t)r0[k];
+         HIPCHK(hipMemcpyAsync((unsigned char *)g_ctx.staging + o, (const unsigned char *)g_ctx.image.p + o,
+                               rb * (size_t)(r0[k + 1] - r0[k]), hipMemcpyDeviceToHost, g_ctx.stream));
+         HIPCHK(hipEventRecord(g_ctx.ev_strip[k], g_ctx.stream));
+      }
+      const unsigned char *src = (const unsigned char *)g_ctx.staging + d[0].img_off;
+      unsigned char *dst = (unsigned char *)d[0].out;
+      for (int k = 0; k < FL3_IMAGE_STRIPS; k++) {
+         HIPCHK(hipEventSynchronize(g_ctx.ev_strip[k]));
+         if (d[0].out_stride == rb)
+            memcpy(dst + rb * (size_t)r0[k], src + rb * (size_t)r0[k], rb * (size_t)(r0[k + 1] - r0[k]));
+         else
+            for (int j = r0[k]; j < r0[k + 1]; j++)
+               memcpy(dst + (size_t)j * d[0].out_stride, src + (size_t)j * rb, rb);
+      }
+      return 0;
    }
    HIPCHK(hipMemcpyAsync(g_ctx.staging, g_ctx.image.p, end, hipMemcpyDeviceToHost, g_ctx.stream));
    HIPCHK(hipStreamSynchronize(g_ctx.stream));
