@@ -105,6 +105,7 @@ struct Ctx {
    std::vector<double> de_up;
    void *de_up_p = nullptr;
    size_t de_up_cap = 0;
+   int de_up_nf = 0, de_up_ks = 0;   // its row count in use (max_filter_index + 1) and row length
    DevBuf crec[FL3_CHUNK_BUFS], cblk[FL3_CHUNK_BUFS], ctot[FL3_CHUNK_BUFS];   // tile-binned deposit
    DevBuf sorted, wstate, bins, jobtab, soff;
    DevBuf srcv, srcl;                  // density estimation per-source values
@@ -1380,13 +1381,44 @@ static int iterate_replay(const void *job, size_t job_bytes, uint32_t *isaac_hos
    return read_counters(badvals, nullptr);
 }
 
+// The rows of a filter bank cut to the coefficients a tap can reach (row i,
+// radius r = ceil(width) - 1 as the gather computes it: its first
+// (r + 1)(r + 2) / 2 entries), back to back as floats (rounded to nearest,
+// as the device's conversion), with each row's start.  Returns the float
+// count, or 0 (the gather then reads the full table) if a row would need
+// more entries than the table's rows hold or the cut bank exceeds
+// FL3_DE_LDS_COEFS floats.
+static int de_cut_bank(const fl3_de_filters *de, std::vector<int> &off, std::vector<float> &cf) {
+   const int nf = de->max_filter_index + 1;
+   off.assign((size_t)nf, 0);
+   cf.clear();
+   for (int i = 0; i < nf; i++) {
+      const long r = (long)ceil(de->filter_widths[i]) - 1;
+      const long len = (r + 1) * (r + 2) / 2;
+      if (r < 0 || len > de->kernel_size || (long)cf.size() + len > FL3_DE_LDS_COEFS) {
+         cf.clear();
+         return 0;
+      }
+      off[(size_t)i] = (int)cf.size();
+      for (long t = 0; t < len; t++) cf.push_back((float)de->filter_coefs[(size_t)i * de->kernel_size + (size_t)t]);
+   }
+   return (int)cf.size();
+}
+
 // DE tables on the device: coefficients (double), widths (double) and, for
-// the float accumulator, a float copy of the coefficients.
-static int upload_de(const fl3_de_filters *de, double **coefs, double **widths, float **coefs_f) {
+// the float accumulator, a float copy of the coefficients and the cut bank
+// (de_cut_bank; *ncoef 0 if there is none).
+static int upload_de(const fl3_de_filters *de, double **coefs, double **widths, float **coefs_f,
+                     const int **coff = nullptr, const float **ccoef = nullptr, int *ncoef = nullptr) {
    size_t nc = (size_t)de->nfilters * de->kernel_size, nw = de->nfilters;
+   std::vector<int> off;
+   std::vector<float> cf;
+   const int ncut = de_cut_bank(de, off, cf);
+   const size_t cut0 = (sizeof(double) * (nc + nw) + sizeof(float) * nc + 15) & ~(size_t)15;
+   const size_t cut1 = cut0 + sizeof(int) * off.size();
    const void *p0 = g_ctx.table.p;
    const size_t cap0 = g_ctx.table.cap;
-   if (g_ctx.table.ensure(sizeof(double) * (nc + nw) + sizeof(float) * nc)) {
+   if (g_ctx.table.ensure(cut1 + sizeof(float) * (size_t)ncut)) {
       g_ctx.de_up_p = nullptr;
       return 1;
    }
@@ -1394,16 +1426,28 @@ static int upload_de(const fl3_de_filters *de, double **coefs, double **widths, 
    *coefs = (double *)g_ctx.table.p;
    *widths = *coefs + nc;
    *coefs_f = (float *)(*widths + nw);
+   if (ncoef) {
+      *coff = (const int *)((unsigned char *)g_ctx.table.p + cut0);
+      *ccoef = (const float *)((unsigned char *)g_ctx.table.p + cut1);
+      *ncoef = ncut;
+   }
    // the same tables as the last upload (equal estimator parameters, as in
    // every frame of a still or an animation): they are on the device
    // already, so no copies, conversion or wait
    if (g_ctx.de_up_p == g_ctx.table.p && g_ctx.de_up_cap == g_ctx.table.cap && g_ctx.de_up.size() == nc + nw &&
+       g_ctx.de_up_nf == de->max_filter_index + 1 && g_ctx.de_up_ks == de->kernel_size &&
        memcmp(g_ctx.de_up.data(), de->filter_coefs, sizeof(double) * nc) == 0 &&
        memcmp(g_ctx.de_up.data() + nc, de->filter_widths, sizeof(double) * nw) == 0)
       return 0;
    g_ctx.de_up_p = nullptr;
    HIPCHK(hipMemcpyAsync(*coefs, de->filter_coefs, sizeof(double) * nc, hipMemcpyHostToDevice, g_ctx.stream));
    HIPCHK(hipMemcpyAsync(*widths, de->filter_widths, sizeof(double) * nw, hipMemcpyHostToDevice, g_ctx.stream));
+   if (ncut) {
+      HIPCHK(hipMemcpyAsync((unsigned char *)g_ctx.table.p + cut0, off.data(), sizeof(int) * off.size(),
+                            hipMemcpyHostToDevice, g_ctx.stream));
+      HIPCHK(hipMemcpyAsync((unsigned char *)g_ctx.table.p + cut1, cf.data(), sizeof(float) * (size_t)ncut,
+                            hipMemcpyHostToDevice, g_ctx.stream));
+   }
    if (fl3k_launch_f64_to_f32(*coefs, *coefs_f, (long)nc, g_ctx.stream)) {
       set_err("k_f64_to_f32 launch failed");
       return 1;
@@ -1414,6 +1458,8 @@ static int upload_de(const fl3_de_filters *de, double **coefs, double **widths, 
    g_ctx.de_up.insert(g_ctx.de_up.end(), de->filter_widths, de->filter_widths + nw);
    g_ctx.de_up_p = g_ctx.table.p;
    g_ctx.de_up_cap = g_ctx.table.cap;
+   g_ctx.de_up_nf = de->max_filter_index + 1;
+   g_ctx.de_up_ks = de->kernel_size;
    return 0;
 }
 
@@ -1423,12 +1469,20 @@ extern "C" int fl3d_density(const fl3d_de_params *p, const fl3_de_filters *de) {
    memset(&a, 0, sizeof(a));
    double *coefs, *widths;
    float *coefs_f;
-   if (upload_de(de, &coefs, &widths, &coefs_f)) return 1;
+   const int *coff;
+   const float *ccoef;
+   int ncoef;
+   if (upload_de(de, &coefs, &widths, &coefs_f, &coff, &ccoef, &ncoef)) return 1;
    a.buckets = cur_buckets();
    a.accum = cur_accum();
    a.coefs = coefs;
    a.widths = widths;
    a.coefs_t = g_ctx.bits == 64 ? (const void *)coefs : (const void *)coefs_f;
+   if (g_ctx.bits != 64) {
+      a.coff = coff;
+      a.ccoef = ccoef;
+      a.ncoef = ncoef;
+   }
    a.width = g_ctx.width;
    a.height = g_ctx.height;
    a.oversample = p->oversample;

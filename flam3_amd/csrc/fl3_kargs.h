@@ -155,7 +155,14 @@ struct DEArgs {
    int *blkmax;
    int nbx;                 // blocks per row
    unsigned long long *taps;   // if set: 64 counters, summed: the gather footprint (2r + 1)^2 of all sources
+   // float accumulator, ncoef > 0: the rows cut to what a tap reaches (row i
+   // from ccoef[coff[i]], (r + 1)(r + 2) / 2 entries for radius r), which the
+   // gather stages in ncoef floats of dynamic LDS; 0: taps read coefs_t
+   const float *ccoef;
+   const int *coff;
+   int ncoef;
 };
+#define FL3_DE_LDS_COEFS 8192   // most floats of a cut filter bank the gather stages in LDS
 
 struct TMArgs {
    const void *accum;

@@ -313,6 +313,11 @@ __device__ __forceinline__ void de_gather_body(const DEArgs &a, int bx, int by) 
          if (bm >= 0 && gx <= bm && gy <= bm) atomicMax(&s_ext, min(bm, R));
       }
    }
+   // the cut filter bank (float accumulator), for the taps below
+   float *s_coef = (float *)fl3_lds;
+   constexpr bool CUT = !EXACT && std::is_same<AT, float>::value;
+   if (CUT && a.ncoef > 0)
+      for (int e = tid; e < a.ncoef; e += DG_CHUNK) s_coef[e] = a.ccoef[e];
    __syncthreads();
    if (a.blkmax) E = s_ext;
    const int hx0 = max(0, rx0 - E), hx1 = min(W, rx1 + 1 + E);
@@ -361,7 +366,8 @@ __device__ __forceinline__ void de_gather_body(const DEArgs &a, int bx, int by) 
             s_src[idx] = make_int4(sx, sy, fi * a.kernel_size, radius[fi]);
          else   // (as floats: every value here is an integer below 2^24)
             s_src[idx] = make_int4(__float_as_int((float)sx), __float_as_int((float)sy),
-                                   __float_as_int((float)(fi * a.kernel_size)), __float_as_int((float)radius[fi]));
+                                   __float_as_int((float)(CUT && a.ncoef > 0 ? a.coff[fi] : fi * a.kernel_size)),
+                                   __float_as_int((float)radius[fi]));
          if constexpr (EXACT) {
             for (int c = 0; c < 4; c++) s_val[idx][c] = bucket_value(a.buckets, cell * 5 + c, (AT)0);
             s_val[idx][4] = a.srcl[cell];
@@ -408,21 +414,29 @@ __device__ __forceinline__ void de_gather_body(const DEArgs &a, int bx, int by) 
          // that |.| comes free with the max and min; the tap's index
          // z + hi (hi + 1) / 2 + lo by exact fused multiply-adds; a missed
          // tap's index cleared by a mask, so that the body has no branch)
+         // (coefficients from the cut bank in LDS when it is there, else
+         // from the full table: one loop each, no per-tap choice)
          const AT *ct = (const AT *)a.coefs_t;
          const float fx = (float)x, fy = (float)y;
+         auto taps = [&](auto coef_at) {
 #pragma unroll 4
-         for (unsigned i = 0; i < nb; i++) {
-            const unsigned k = s_list[wave][i];
-            const int4 src = s_src[k];
-            const float ax = fx - __int_as_float(src.x), ay = fy - __int_as_float(src.y);
-            const float hi = fmaxf(fabsf(ax), fabsf(ay)), lo = fminf(fabsf(ax), fabsf(ay));
-            const bool tap = live && hi <= __int_as_float(src.w);
-            const float cf = fmaf(fmaf(hi, hi, hi), 0.5f, lo) + __int_as_float(src.z);
-            const unsigned ci = (unsigned)cf & (0u - (unsigned)tap);
-            const AT cv = ct[ci];
-            const AT coef = tap ? cv : (AT)0;
-            for (int c = 0; c < 4; c++) acc[c] = fma(coef, s_val[k][c], acc[c]);
-         }
+            for (unsigned i = 0; i < nb; i++) {
+               const unsigned k = s_list[wave][i];
+               const int4 src = s_src[k];
+               const float ax = fx - __int_as_float(src.x), ay = fy - __int_as_float(src.y);
+               const float hi = fmaxf(fabsf(ax), fabsf(ay)), lo = fminf(fabsf(ax), fabsf(ay));
+               const bool tap = live && hi <= __int_as_float(src.w);
+               const float cf = fmaf(fmaf(hi, hi, hi), 0.5f, lo) + __int_as_float(src.z);
+               const unsigned ci = (unsigned)cf & (0u - (unsigned)tap);
+               const AT cv = coef_at(ci);
+               const AT coef = tap ? cv : (AT)0;
+               for (int c = 0; c < 4; c++) acc[c] = fma(coef, s_val[k][c], acc[c]);
+            }
+         };
+         if (CUT && a.ncoef > 0)
+            taps([&](unsigned c) { return (AT)s_coef[c]; });
+         else
+            taps([&](unsigned c) { return ct[c]; });
       }
       __syncthreads();
    }
@@ -645,7 +659,8 @@ extern "C" int fl3k_launch_de_tiled(const DEArgs *a, hipStream_t s, int bits) {
       if (a->exact)
          hipLaunchKernelGGL((k_de_gather<float, true>), g3, dim3(DG_CHUNK), 0, s, *a);
       else
-         hipLaunchKernelGGL((k_de_gather<float, false>), g3, dim3(DG_CHUNK), 0, s, *a);
+         hipLaunchKernelGGL((k_de_gather<float, false>), g3, dim3(DG_CHUNK),
+                            a->ncoef > 0 ? sizeof(float) * (size_t)a->ncoef : 0, s, *a);
    }
    return hipGetLastError() != hipSuccess;
 }
