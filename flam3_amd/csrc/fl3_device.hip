@@ -1384,25 +1384,35 @@ static int iterate_replay(const void *job, size_t job_bytes, uint32_t *isaac_hos
 // The rows of a filter bank cut to the coefficients a tap can reach (row i,
 // radius r = ceil(width) - 1 as the gather computes it: its first
 // (r + 1)(r + 2) / 2 entries), back to back as floats (rounded to nearest,
-// as the device's conversion), with each row's start.  Returns the float
-// count, or 0 (the gather then reads the full table) if a row would need
-// more entries than the table's rows hold or the cut bank exceeds
-// FL3_DE_LDS_COEFS floats.
-static int de_cut_bank(const fl3_de_filters *de, std::vector<int> &off, std::vector<float> &cf) {
+// as the device's conversion) into cf, each row's start into off (both
+// skipped if cf is null: the count alone).  Returns the float count, or 0
+// (the gather then reads the full table) if a row would need more entries
+// than the table's rows hold or the cut bank exceeds FL3_DE_LDS_COEFS floats.
+static int de_cut_bank(const fl3_de_filters *de, int *off, float *cf) {
    const int nf = de->max_filter_index + 1;
-   off.assign((size_t)nf, 0);
-   cf.clear();
+   long n = 0;
    for (int i = 0; i < nf; i++) {
       const long r = (long)ceil(de->filter_widths[i]) - 1;
       const long len = (r + 1) * (r + 2) / 2;
-      if (r < 0 || len > de->kernel_size || (long)cf.size() + len > FL3_DE_LDS_COEFS) {
-         cf.clear();
-         return 0;
+      if (r < 0 || len > de->kernel_size || n + len > FL3_DE_LDS_COEFS) return 0;
+      if (cf) {
+         off[i] = (int)n;
+         for (long t = 0; t < len; t++) cf[n + t] = (float)de->filter_coefs[(size_t)i * de->kernel_size + (size_t)t];
       }
-      off[(size_t)i] = (int)cf.size();
-      for (long t = 0; t < len; t++) cf.push_back((float)de->filter_coefs[(size_t)i * de->kernel_size + (size_t)t]);
+      n += len;
    }
-   return (int)cf.size();
+   return (int)n;
+}
+
+// A cut bank as the batched tables hold it: row starts, then the floats,
+// each part padded to 16 bytes.  Bytes of the row starts, and of the whole
+// (0 without a cut bank).
+static size_t de_cut_rows_bytes(const fl3_de_filters *de) {
+   return (sizeof(int) * (size_t)(de->max_filter_index + 1) + 15) & ~(size_t)15;
+}
+static size_t de_cut_bytes(const fl3_de_filters *de, int ncut) {
+   if (ncut <= 0) return 0;
+   return de_cut_rows_bytes(de) + ((sizeof(float) * (size_t)ncut + 15) & ~(size_t)15);
 }
 
 // DE tables on the device: coefficients (double), widths (double) and, for
@@ -1411,9 +1421,10 @@ static int de_cut_bank(const fl3_de_filters *de, std::vector<int> &off, std::vec
 static int upload_de(const fl3_de_filters *de, double **coefs, double **widths, float **coefs_f,
                      const int **coff = nullptr, const float **ccoef = nullptr, int *ncoef = nullptr) {
    size_t nc = (size_t)de->nfilters * de->kernel_size, nw = de->nfilters;
-   std::vector<int> off;
-   std::vector<float> cf;
-   const int ncut = de_cut_bank(de, off, cf);
+   const int ncut = de_cut_bank(de, nullptr, nullptr);
+   std::vector<int> off(ncut ? (size_t)de->max_filter_index + 1 : 0);
+   std::vector<float> cf((size_t)ncut);
+   if (ncut) de_cut_bank(de, off.data(), cf.data());
    const size_t cut0 = (sizeof(double) * (nc + nw) + sizeof(float) * nc + 15) & ~(size_t)15;
    const size_t cut1 = cut0 + sizeof(int) * off.size();
    const void *p0 = g_ctx.table.p;
@@ -1554,7 +1565,8 @@ extern "C" int fl3d_density_many(const fl3d_de_params *p, const fl3_de_filters *
           g_ctx.rheight[i] < 65536) {
          dej.push_back(i);
          const size_t nc = (size_t)de[i]->nfilters * de[i]->kernel_size, nw = de[i]->nfilters;
-         tab += (nc + nw) * sizeof(double) + ((nc * sizeof(float) + 15) & ~(size_t)15);
+         tab += (nc + nw) * sizeof(double) + ((nc * sizeof(float) + 15) & ~(size_t)15) +
+                de_cut_bytes(de[i], de_cut_bank(de[i], nullptr, nullptr));
       }
    }
    const int nd = (int)dej.size();
@@ -1599,20 +1611,30 @@ static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *d
    size_t off = 0;
    // renders with equal filter banks (the same estimator parameters and
    // oversampling) share one uploaded table
-   std::vector<std::pair<const fl3_de_filters *, size_t>> uniq;
+   // (with its cut bank after the float copy, if it has one)
+   struct Uniq {
+      const fl3_de_filters *d;
+      size_t off;
+      int ncut;
+   };
+   std::vector<Uniq> uniq;
+   int ncut_max = 0;
    for (int k = 0; k < nd; k++) {
       const int i = dej[k];
       const fl3_de_filters *d = de[i];
       const size_t nc = (size_t)d->nfilters * d->kernel_size, nw = d->nfilters;
+      const size_t base = (nc + nw) * sizeof(double) + ((nc * sizeof(float) + 15) & ~(size_t)15);
       size_t toff = off;
+      int ncut = 0;
       bool found = false;
       for (const auto &u : uniq) {
-         const fl3_de_filters *e = u.first;
+         const fl3_de_filters *e = u.d;
          if (e->nfilters == d->nfilters && e->kernel_size == d->kernel_size &&
              e->max_filter_index == d->max_filter_index &&
              !memcmp(e->filter_coefs, d->filter_coefs, nc * sizeof(double)) &&
              !memcmp(e->filter_widths, d->filter_widths, nw * sizeof(double))) {
-            toff = u.second;
+            toff = u.off;
+            ncut = u.ncut;
             found = true;
             break;
          }
@@ -1623,9 +1645,14 @@ static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *d
          // the float copy, rounded to nearest as a device conversion would
          float *ft = (float *)(hostt.data() + off + (nc + nw) * sizeof(double));
          for (size_t e = 0; e < nc; e++) ft[e] = (float)d->filter_coefs[e];
-         uniq.emplace_back(d, off);
-         off += (nc + nw) * sizeof(double) + ((nc * sizeof(float) + 15) & ~(size_t)15);
+         ncut = de_cut_bank(d, nullptr, nullptr);
+         if (ncut)
+            de_cut_bank(d, (int *)(hostt.data() + off + base),
+                        (float *)(hostt.data() + off + base + de_cut_rows_bytes(d)));
+         uniq.push_back({d, off, ncut});
+         off += base + de_cut_bytes(d, ncut);
       }
+      ncut_max = ncut > ncut_max ? ncut : ncut_max;
       unsigned char *tb = (unsigned char *)g_ctx.detab.p + toff;
       DEArgs &a = as[k];
       memset(&a, 0, sizeof(a));
@@ -1635,6 +1662,11 @@ static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *d
       a.coefs = (const double *)tb;
       a.widths = (const double *)(tb + nc * sizeof(double));
       a.coefs_t = tb + (nc + nw) * sizeof(double);
+      if (ncut) {
+         a.coff = (const int *)(tb + base);
+         a.ccoef = (const float *)(tb + base + de_cut_rows_bytes(d));
+         a.ncoef = ncut;
+      }
       a.findex = (uint16_t *)g_ctx.findex.p + c0;
       a.srcv = (unsigned char *)g_ctx.srcv.p + c0 * 16;
       a.srcl = (double *)g_ctx.srcl.p + c0;
@@ -1678,7 +1710,7 @@ static int de_many_batch(const fl3d_de_params *p, const fl3_de_filters *const *d
    const unsigned *dfirst = (const unsigned *)((unsigned char *)g_ctx.deargs.p + ab);
    const int lrc = fl3k_launch_de_many((const DEArgs *)g_ctx.deargs.p, dfirst, (unsigned)pb, dfirst + nd + 1,
                                        (unsigned)gb, nd, g_ctx.stream, halo ? (int *)g_ctx.deblk.p : nullptr,
-                                       (long)nblk);
+                                       (long)nblk, sizeof(float) * (size_t)ncut_max);
    prof_end(pk);
    if (lrc) {
       set_err("density estimation launch failed");

@@ -496,13 +496,13 @@ __global__ __launch_bounds__(DG_CHUNK) void k_de_gather_many(const DEArgs *as, c
 // into (cleared here), or nullptr.
 extern "C" int fl3k_launch_de_many(const DEArgs *as, const unsigned *prep_first, unsigned prep_blocks,
                                    const unsigned *gather_first, unsigned gather_blocks, int n, hipStream_t s,
-                                   int *blkmax, long nblk) {
+                                   int *blkmax, long nblk, size_t gather_lds) {
    if (n <= 0) return 0;
    if (blkmax)
       hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, s, blkmax, nblk, 0);
    hipLaunchKernelGGL((k_de_prep_many<uint32_t, float>), dim3(prep_blocks), dim3(256), 0, s, as, prep_first, n);
-   hipLaunchKernelGGL((k_de_gather_many<float, false>), dim3(gather_blocks), dim3(DG_CHUNK), 0, s, as, gather_first,
-                      n);
+   hipLaunchKernelGGL((k_de_gather_many<float, false>), dim3(gather_blocks), dim3(DG_CHUNK), gather_lds, s, as,
+                      gather_first, n);
    return hipGetLastError() != hipSuccess;
 }
 

@@ -37,7 +37,7 @@ int fl3k_launch_deposit(const unsigned char *job, const double *pts, long long n
 int fl3k_launch_de_tiled(const DEArgs *a, hipStream_t s, int bits);
 int fl3k_launch_de_many(const DEArgs *as, const unsigned *prep_first, unsigned prep_blocks,
                         const unsigned *gather_first, unsigned gather_blocks, int n, hipStream_t s,
-                        int *blkmax, long nblk);
+                        int *blkmax, long nblk, size_t gather_lds);
 int fl3k_launch_f64_to_f32(const double *src, float *dst, long n, hipStream_t s);
 int fl3k_launch_logscale(const void *b, void *acc, long n, double k1, double k2, hipStream_t s, int bits);
 int fl3k_launch_earlyclip(void *acc, long n, const TMArgs *a, hipStream_t s, int bits);
